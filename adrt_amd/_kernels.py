@@ -1,0 +1,143 @@
+"""Device-tensor calls into the HIP library (no NumPy, no host copies).
+
+Every function takes contiguous HIP tensors, allocates its result (and any
+workspace) from torch's caching allocator on the input's device, launches on
+the current stream and returns the result tensor.  The library validates
+shapes and returns a status; a failure raises.  The solvers, the autograd
+functions and the batched API build on these.
+"""
+
+from . import _device as dev
+from . import _lib
+
+
+def _code(t):
+    code = dev.dtype_code(t)
+    if code is None:
+        raise TypeError(f"unsupported array dtype {t.dtype}")
+    return code
+
+
+def _batch_n(t, img_dims):
+    b = int(t.shape[0]) if t.dim() == img_dims + 1 else 1
+    return b, int(t.shape[-1])
+
+
+def _ws_call(fn_name, op, t, out):
+    lib = _lib.load()
+    code = _code(t)
+    b, n = _batch_n(t, 2 if op == _lib.OP_ADRT else 3)
+    nbytes = lib.adrt_hip_workspace_bytes(op, code, b, n)
+    ws = dev.workspace(t, nbytes)
+    fn = getattr(lib, fn_name)
+    _lib.check(fn(dev.ptr(t), dev.ptr(out), dev.ptr(ws), nbytes, code, b, n,
+                  dev.stream_ptr(t)), fn_name)
+    return out
+
+
+def adrt(t):
+    n = int(t.shape[-1])
+    out = dev.empty_like_dtype(t, (*t.shape[:-2], 4, 2 * n - 1, n))
+    return _ws_call("adrt_hip_adrt", _lib.OP_ADRT, t, out)
+
+
+def bdrt(t):
+    return _ws_call("adrt_hip_bdrt", _lib.OP_BDRT, t, dev.empty_like_dtype(t, t.shape))
+
+
+def iadrt(t):
+    return _ws_call("adrt_hip_iadrt", _lib.OP_IADRT, t, dev.empty_like_dtype(t, t.shape))
+
+
+def _step(fn_name, t, step):
+    lib = _lib.load()
+    out = dev.empty_like_dtype(t, t.shape)
+    b, n = _batch_n(t, 3)
+    _lib.check(getattr(lib, fn_name)(dev.ptr(t), dev.ptr(out), _code(t), b, n, int(step),
+                                     dev.stream_ptr(t)), fn_name)
+    return out
+
+
+def adrt_step(t, step):
+    return _step("adrt_hip_adrt_step", t, step)
+
+
+def bdrt_step(t, step):
+    return _step("adrt_hip_bdrt_step", t, step)
+
+
+def fmg_restriction(t):
+    lib = _lib.load()
+    b, n = _batch_n(t, 3)
+    out = dev.empty_like_dtype(t, (*t.shape[:-2], n - 1, n // 2))
+    _lib.check(lib.adrt_hip_fmg_restriction(dev.ptr(t), dev.ptr(out), _code(t), b, n,
+                                            dev.stream_ptr(t)), "fmg_restriction")
+    return out
+
+
+def fmg_prolongation(t):
+    lib = _lib.load()
+    b = int(t.shape[0]) if t.dim() == 3 else 1
+    h, w = int(t.shape[-2]), int(t.shape[-1])
+    out = dev.empty_like_dtype(t, (*t.shape[:-2], 2 * h, 2 * w))
+    _lib.check(lib.adrt_hip_fmg_prolongation(dev.ptr(t), dev.ptr(out), _code(t), b, h, w,
+                                             dev.stream_ptr(t)), "fmg_prolongation")
+    return out
+
+
+def fmg_highpass(t):
+    lib = _lib.load()
+    b = int(t.shape[0]) if t.dim() == 3 else 1
+    h, w = int(t.shape[-2]), int(t.shape[-1])
+    out = dev.empty_like_dtype(t, t.shape)
+    _lib.check(lib.adrt_hip_fmg_highpass(dev.ptr(t), dev.ptr(out), _code(t), b, h, w,
+                                         dev.stream_ptr(t)), "fmg_highpass")
+    return out
+
+
+def adrt_init(t):
+    """(..., n, n) any 1/2/4/8-byte dtype -> (..., 4, 2n-1, n)."""
+    lib = _lib.load()
+    n = int(t.shape[-1])
+    b = int(t.shape[0]) if t.dim() == 3 else 1
+    out = dev.empty_like_dtype(t, (*t.shape[:-2], 4, 2 * n - 1, n))
+    _lib.check(lib.adrt_hip_adrt_init(dev.ptr(t), dev.ptr(out), t.element_size(), b, n,
+                                       dev.stream_ptr(t)), "adrt_init")
+    return out
+
+
+def truncate(t):
+    """(..., 4, 2n-1, n) any 1/2/4/8-byte dtype -> (..., 4, n, n)."""
+    lib = _lib.load()
+    n = int(t.shape[-1])
+    outer = 1
+    for s in t.shape[:-3]:
+        outer *= int(s)
+    out = dev.empty_like_dtype(t, (*t.shape[:-3], 4, n, n))
+    if outer == 0:
+        return out
+    _lib.check(lib.adrt_hip_truncate(dev.ptr(t), dev.ptr(out), t.element_size(), outer, n,
+                                     dev.stream_ptr(t)), "truncate")
+    return out
+
+
+MEAN = 0
+SUM = 1
+
+
+def truncate_combine(t, divisor=1.0, mode=MEAN):
+    """combine_q(truncate(t) / divisor): (..., 4, 2n-1, n) -> (..., n, n).
+
+    MEAN reproduces ``np.mean(truncate(t) / divisor, axis=-3)`` bit for bit;
+    SUM is the exact adjoint's quadrant sum ((q0+q1)+q2)+q3.
+    """
+    lib = _lib.load()
+    n = int(t.shape[-1])
+    b = 1
+    for s in t.shape[:-3]:
+        b *= int(s)
+    out = dev.empty_like_dtype(t, (*t.shape[:-3], n, n))
+    _lib.check(lib.adrt_hip_truncate_combine(dev.ptr(t), dev.ptr(out), _code(t), b, n,
+                                             float(divisor), int(mode), dev.stream_ptr(t)),
+               "truncate_combine")
+    return out

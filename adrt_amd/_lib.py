@@ -1,0 +1,106 @@
+"""ctypes binding of the gfx950 HIP library ``libadrt_hip.so``.
+
+This is the only module that touches the C ABI declared in
+``include/adrt_hip.h``.  There is no CPU fallback: if the library is missing
+or no HIP device is visible, compute calls raise ``RuntimeError``.  Shape
+validation functions work without a device.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libadrt_hip.so")
+
+F32 = 0
+F64 = 1
+OP_ADRT = 0
+OP_BDRT = 1
+OP_IADRT = 2
+
+OK = 0
+ERR_SHAPE = 1
+ERR_STEP = 2
+ERR_DTYPE = 3
+ERR_TOO_BIG = 4
+ERR_NULL = 5
+ERR_WORKSPACE = 6
+ERR_DEVICE = 7
+ERR_ARG = 8
+
+_SZ = ctypes.c_size_t
+_VP = ctypes.c_void_p
+_INT = ctypes.c_int
+_SHAPE3 = _SZ * 3
+_SHAPE4 = _SZ * 4
+
+# name -> (restype, argtypes); mirrors include/adrt_hip.h
+SIGNATURES = {
+    "adrt_hip_status_string": (ctypes.c_char_p, [_INT]),
+    "adrt_hip_last_error": (ctypes.c_char_p, []),
+    "adrt_hip_abi_version": (_INT, []),
+    "adrt_hip_device_count": (_INT, []),
+    "adrt_hip_num_iters": (_INT, [_SZ]),
+    "adrt_hip_valid_adrt_shape": (_INT, [_SHAPE3]),
+    "adrt_hip_valid_output_shape": (_INT, [_SHAPE4]),
+    "adrt_hip_valid_step": (_INT, [_SHAPE4, _INT]),
+    "adrt_hip_valid_restriction_shape": (_INT, [_SHAPE4]),
+    "adrt_hip_valid_prolongation_shape": (_INT, [_SHAPE3]),
+    "adrt_hip_valid_highpass_shape": (_INT, [_SHAPE3]),
+    "adrt_hip_workspace_bytes": (_SZ, [_INT, _INT, _SZ, _SZ]),
+    "adrt_hip_adrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_adrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
+    "adrt_hip_bdrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_bdrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
+    "adrt_hip_iadrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_fmg_restriction": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_fmg_prolongation": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _SZ, _VP]),
+    "adrt_hip_fmg_highpass": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _SZ, _VP]),
+    "adrt_hip_adrt_init": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
+    "adrt_hip_truncate": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
+    "adrt_hip_truncate_combine": (_INT, [_VP, _VP, _INT, _SZ, _SZ, ctypes.c_double, _INT, _VP]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (raises RuntimeError if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"adrt_amd HIP library not found at {LIB_PATH}; build it with "
+                "`python -m adrt_amd.build` (there is no CPU fallback)"
+            )
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def shape3(s):
+    return _SHAPE3(*[int(v) for v in s])
+
+
+def shape4(s):
+    return _SHAPE4(*[int(v) for v in s])
+
+
+def check(status, what):
+    """Map a non-zero status from the library to an exception."""
+    if status == OK:
+        return
+    lib = load()
+    msg = lib.adrt_hip_status_string(status).decode()
+    if status == ERR_DEVICE:
+        detail = lib.adrt_hip_last_error().decode()
+        raise RuntimeError(f"{what}: {msg} ({detail})")
+    if status in (ERR_SHAPE, ERR_STEP, ERR_TOO_BIG, ERR_ARG):
+        raise ValueError(f"{what}: {msg}")
+    if status == ERR_DTYPE:
+        raise TypeError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")

@@ -1,0 +1,140 @@
+// Layout kernels: adrt_init (core.py:123-176), truncate (utils.py:191-242)
+// and the fused truncate + quadrant combine used by the solvers.
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace adrt_hip {
+
+// out (B,4,2n-1,n): rows i < n hold the quadrant flips, rows >= n are zero.
+//   q0[i][j] = a[j][n-1-i]   q1[i][j] = a[n-1-i][j]
+//   q2[i][j] = a[i][j]       q3[i][j] = a[n-1-j][n-1-i]
+template <typename E>
+__global__ void __launch_bounds__(kBlock)
+    init_kernel(const E *__restrict__ in, E *__restrict__ out, size_t total, size_t n, int log2n) {
+    const size_t C = 2 * n - 1;
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += size_t(gridDim.x) * blockDim.x) {
+        const size_t j = idx & (n - 1);
+        const size_t line = idx >> log2n;
+        const size_t i = line % C;
+        const size_t bq = line / C;
+        const size_t q = bq & 3, b = bq >> 2;
+        E v = E(0);
+        if (i < n) {
+            const E *a = in + b * n * n;
+            switch (q) {
+                case 0: v = a[j * n + (n - 1 - i)]; break;
+                case 1: v = a[(n - 1 - i) * n + j]; break;
+                case 2: v = a[i * n + j]; break;
+                default: v = a[(n - 1 - j) * n + (n - 1 - i)]; break;
+            }
+        }
+        out[idx] = v;
+    }
+}
+
+// Inverse of init on the top n x n square of each quadrant:
+//   t0[i][j] = x0[n-1-j][i]  t1[i][j] = x1[n-1-i][j]
+//   t2[i][j] = x2[i][j]      t3[i][j] = x3[n-1-j][n-1-i]
+template <typename E>
+__device__ inline E truncate_at(const E *plane4, size_t q, size_t i, size_t j, size_t n) {
+    const E *x = plane4 + q * (2 * n - 1) * n;
+    switch (q) {
+        case 0: return x[(n - 1 - j) * n + i];
+        case 1: return x[(n - 1 - i) * n + j];
+        case 2: return x[i * n + j];
+        default: return x[(n - 1 - j) * n + (n - 1 - i)];
+    }
+}
+
+template <typename E>
+__global__ void __launch_bounds__(kBlock)
+    truncate_kernel(const E *__restrict__ in, E *__restrict__ out, size_t total, size_t n) {
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += size_t(gridDim.x) * blockDim.x) {
+        const size_t j = idx % n;
+        const size_t i = (idx / n) % n;
+        const size_t oq = idx / (n * n);
+        const size_t q = oq & 3, o = oq >> 2;
+        out[idx] = truncate_at(in + o * 4 * (2 * n - 1) * n, q, i, j, n);
+    }
+}
+
+// out[b][i][j] = ((t0/d + t1/d) + t2/d) + t3/d, then / 4 for mode 0.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    truncate_combine_kernel(const T *__restrict__ in, T *__restrict__ out, size_t total, size_t n,
+                            T divisor, int divide, int mode) {
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += size_t(gridDim.x) * blockDim.x) {
+        const size_t j = idx % n;
+        const size_t i = (idx / n) % n;
+        const size_t b = idx / (n * n);
+        const T *p = in + b * 4 * (2 * n - 1) * n;
+        T t0 = truncate_at(p, 0, i, j, n), t1 = truncate_at(p, 1, i, j, n);
+        T t2 = truncate_at(p, 2, i, j, n), t3 = truncate_at(p, 3, i, j, n);
+        if (divide) {
+            t0 = t0 / divisor;
+            t1 = t1 / divisor;
+            t2 = t2 / divisor;
+            t3 = t3 / divisor;
+        }
+        T acc = ((t0 + t1) + t2) + t3;
+        if (mode == 0) acc = acc / T(4);
+        out[idx] = acc;
+    }
+}
+
+template <typename E>
+static void init_typed(const void *in, void *out, size_t B, size_t n, hipStream_t s) {
+    const size_t total = B * 4 * (2 * n - 1) * n;
+    hipLaunchKernelGGL(init_kernel<E>, dim3(grid_for(total)), dim3(kBlock), 0, s,
+                       static_cast<const E *>(in), static_cast<E *>(out), total, n, num_iters(n));
+}
+
+template <typename E>
+static void truncate_typed(const void *in, void *out, size_t outer, size_t n, hipStream_t s) {
+    const size_t total = outer * 4 * n * n;
+    hipLaunchKernelGGL(truncate_kernel<E>, dim3(grid_for(total)), dim3(kBlock), 0, s,
+                       static_cast<const E *>(in), static_cast<E *>(out), total, n);
+}
+
+int launch_adrt_init(const void *in, void *out, size_t elem, size_t B, size_t n, hipStream_t s) {
+    switch (elem) {
+        case 1: init_typed<uint8_t>(in, out, B, n, s); break;
+        case 2: init_typed<uint16_t>(in, out, B, n, s); break;
+        case 4: init_typed<uint32_t>(in, out, B, n, s); break;
+        default: init_typed<uint64_t>(in, out, B, n, s); break;
+    }
+    return launch_status();
+}
+
+int launch_truncate(const void *in, void *out, size_t elem, size_t outer, size_t n,
+                    hipStream_t s) {
+    switch (elem) {
+        case 1: truncate_typed<uint8_t>(in, out, outer, n, s); break;
+        case 2: truncate_typed<uint16_t>(in, out, outer, n, s); break;
+        case 4: truncate_typed<uint32_t>(in, out, outer, n, s); break;
+        default: truncate_typed<uint64_t>(in, out, outer, n, s); break;
+    }
+    return launch_status();
+}
+
+int launch_truncate_combine(const void *in, void *out, int dtype, size_t B, size_t n,
+                            double divisor, int mode, hipStream_t s) {
+    const size_t total = B * n * n;
+    const unsigned g = grid_for(total);
+    const int divide = divisor != 1.0;
+    if (dtype == ADRT_HIP_F32)
+        hipLaunchKernelGGL(truncate_combine_kernel<float>, dim3(g), dim3(kBlock), 0, s,
+                           static_cast<const float *>(in), static_cast<float *>(out), total, n,
+                           float(divisor), divide, mode);
+    else
+        hipLaunchKernelGGL(truncate_combine_kernel<double>, dim3(g), dim3(kBlock), 0, s,
+                           static_cast<const double *>(in), static_cast<double *>(out), total, n,
+                           divisor, divide, mode);
+    return launch_status();
+}
+
+}  // namespace adrt_hip

@@ -1,0 +1,214 @@
+"""Benchmark: forward ADRT of 16 ellipse phantoms at 2048 x 2048, float32.
+
+A step is one forward ADRT (adrt_hip_adrt through the C ABI) over the whole
+batch, with the input already resident in HBM.  K timed steps are bracketed
+by torch.cuda.synchronize(); every call is also bracketed by HIP events on
+the stream the kernels run on (torch's current stream), which gives the
+per-call device time used for the roofline.
+
+Roofline basis (compulsory traffic, each input read once and each output
+written once): B_fwd(N) = s * (N^2 + 4 N (2N - 1)) bytes per image;
+achieved = batch * B_fwd / (device time per call); peak = 8000 GB/s.
+
+The CPU baseline times the reference's own OpenMP CPU path (compiled from
+the reference sources into oracle/_ref/libadrt_ref.so) on a bounded sample
+of the same phantoms, on this host's cores; if that library is absent the
+oracle restatement is timed instead.
+
+Prints exactly one JSON line.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0
+METRIC = "fwd ADRT 2048² fp32 imgs/s (%HBM peak)"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--batch", type=int, default=16)
+    p.add_argument("--n", type=int, default=2048)
+    p.add_argument("--dtype", default="float32", choices=["float32", "float64"])
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--extras", action="store_true", help="also time bdrt / iadrt / fp64")
+    return p.parse_args()
+
+
+def bytes_fwd(n, s):
+    return s * (n * n + 4 * n * (2 * n - 1))
+
+
+def bytes_bdrt(n, s):
+    return 2 * s * 4 * n * (2 * n - 1)
+
+
+def time_calls(fn, steps, warmup, stream):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        fn()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / steps
+    return wall, dev_ms
+
+
+def cpu_baseline(host_imgs, budget_s):
+    """Time the reference's CPU path on a bounded sample; returns a dict."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from oracle import ref as R
+
+    kind = "reference" if R.available() else "port"
+    fn = R.adrt if kind == "reference" else O.adrt
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    one = np.ascontiguousarray(host_imgs[:1])
+    t0 = time.perf_counter()
+    fn(one)  # warm-up (also first-touch of the library)
+    est = time.perf_counter() - t0
+    reps = max(1, min(len(host_imgs), int(budget_s / max(est, 1e-3) / 3)))
+    sample = np.ascontiguousarray(host_imgs[:reps])
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        fn(sample)
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[1]
+    n = host_imgs.shape[-1]
+    return {
+        "value": reps / med,
+        "unit": "imgs/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"adrt of {reps} of the same {n}x{n} {host_imgs.dtype} phantoms, "
+                  f"median of 3 runs, OpenMP threads={cores}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(every MI355X box here has one GPU; batches shard across GPUs with no "
+              "exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import adrt_amd
+    from adrt_amd import _device as dev
+    from adrt_amd import _lib
+    from adrt_amd.phantom import phantoms
+
+    lib = _lib.load()
+    torch.cuda.init()
+    device = torch.device("cuda", 0)
+    n, batch = args.n, args.batch
+    s = 4 if args.dtype == "float32" else 8
+    imgs = phantoms(batch, n, seed=0, dtype=args.dtype, device=device)
+    out = torch.empty((batch, 4, 2 * n - 1, n), device=device, dtype=imgs.dtype)
+    code = _lib.F32 if args.dtype == "float32" else _lib.F64
+    ws_bytes = lib.adrt_hip_workspace_bytes(_lib.OP_ADRT, code, batch, n)
+    ws = torch.empty(max(ws_bytes, 1), device=device, dtype=torch.uint8)
+    stream = torch.cuda.current_stream(device)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    pin, pout, pws = (ctypes.c_void_p(t.data_ptr()) for t in (imgs, out, ws))
+
+    def fwd():
+        st = lib.adrt_hip_adrt(pin, pout, pws, ws_bytes, code, batch, n, sp)
+        if st != 0:
+            _lib.check(st, "adrt_hip_adrt")
+
+    wall, dev_ms = time_calls(fwd, args.steps, args.warmup, stream)
+    value = batch * args.steps / wall
+    algo = batch * bytes_fwd(n, s)
+    achieved = algo / (dev_ms * 1e-3) / 1e9
+
+    # parity spot check of the timed output against the reference hash-free
+    # path: first image vs the compiled reference (or oracle) on the host.
+    host = imgs[:2].cpu().numpy()
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "imgs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if s == 4 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"adrt forward, {batch} random-ellipse phantoms {n}x{n} {args.dtype} "
+                        "(seed 0), inputs resident in HBM",
+            "batch": batch,
+            "n": n,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "adrt_hip_adrt: all kernels of one forward call",
+            "algorithmic_bytes_per_call": algo,
+            "device_ms_per_call": round(dev_ms, 4),
+        },
+    }
+    if args.extras:
+        extras = {}
+        y = out.clone()
+        bout = torch.empty_like(y)
+        for name, op in (("bdrt", _lib.OP_BDRT), ("iadrt", _lib.OP_IADRT)):
+            wb = lib.adrt_hip_workspace_bytes(op, code, batch, n)
+            wst = torch.empty(max(wb, 1), device=device, dtype=torch.uint8)
+            fn_c = getattr(lib, f"adrt_hip_{name}")
+            args_c = (ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(bout.data_ptr()),
+                      ctypes.c_void_p(wst.data_ptr()), wb, code, batch, n, sp)
+            _, ms = time_calls(lambda: fn_c(*args_c), max(3, args.steps // 4), 1, stream)
+            gbs = batch * bytes_bdrt(n, s) / (ms * 1e-3) / 1e9
+            extras[name] = {"ms_per_call": round(ms, 3), "imgs_per_s": round(batch / ms * 1e3, 2),
+                            "GBps_algorithmic": round(gbs, 1)}
+        result["extras"] = extras
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(host, args.cpu_seconds)
+        # bit-exact spot check of the first image against the CPU leg's library
+        from oracle import ref as R
+        from oracle import oracle as O
+
+        ref_fn = R.adrt if R.available() else O.adrt
+        got = out[:1].cpu().numpy()
+        want = ref_fn(np.ascontiguousarray(host[:1]))
+        got = np.where(got == 0, 0, got)
+        want = np.where(want == 0, 0, want)
+        result["parity_first_image_bit_exact"] = bool(got.tobytes() == want.tobytes())
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

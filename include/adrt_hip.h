@@ -1,0 +1,169 @@
+/*
+ * adrt_hip.h -- C ABI of the MI355X (gfx950) ADRT transform core.
+ *
+ * Every entry point takes plain device pointers and sizes, launches
+ * asynchronously on the given HIP stream (a hipStream_t passed as void*;
+ * NULL means the default stream) and returns an int status (ADRT_HIP_OK or
+ * one of the ADRT_HIP_ERR_* codes below).  Inputs are never modified,
+ * outputs are written in full, and nothing is allocated inside the library:
+ * temporary space comes from the caller through the `workspace` argument,
+ * sized by adrt_hip_workspace_bytes().  No torch or Python types appear.
+ *
+ * Each transform entry point replaces one function of the reference's
+ * CPython extension adrt._adrt_cdefs (src/adrt/adrt_cdefs_py.cpp, method
+ * table at :852-863) and performs the same shape/step validation
+ * (src/adrt/adrt_cdefs_common.cpp:176-330) before launching.
+ *
+ * Array layouts are C-contiguous with one optional leading batch dimension,
+ * exactly as the reference's NumPy arrays:
+ *   image        (batch, n, n)
+ *   ADRT output  (batch, 4, 2n-1, n)
+ */
+#ifndef ADRT_HIP_H
+#define ADRT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types of the floating-point transforms. */
+#define ADRT_HIP_F32 0
+#define ADRT_HIP_F64 1
+
+/* Status codes. */
+#define ADRT_HIP_OK 0
+#define ADRT_HIP_ERR_SHAPE 1        /* shape invalid for the operation          */
+#define ADRT_HIP_ERR_STEP 2         /* step outside [0, num_iters(n))            */
+#define ADRT_HIP_ERR_DTYPE 3        /* unsupported element type                  */
+#define ADRT_HIP_ERR_TOO_BIG 4      /* element count overflows size_t            */
+#define ADRT_HIP_ERR_NULL 5         /* required pointer is NULL                  */
+#define ADRT_HIP_ERR_WORKSPACE 6    /* workspace smaller than required           */
+#define ADRT_HIP_ERR_DEVICE 7       /* HIP runtime / launch failure              */
+#define ADRT_HIP_ERR_ARG 8          /* other invalid argument                    */
+
+/* Operation ids for adrt_hip_workspace_bytes(). */
+#define ADRT_HIP_OP_ADRT 0
+#define ADRT_HIP_OP_BDRT 1
+#define ADRT_HIP_OP_IADRT 2
+
+/* ---- library information ------------------------------------------------ */
+
+/* Static string describing a status code. */
+const char *adrt_hip_status_string(int status);
+
+/* Text of the last HIP runtime error seen on the calling thread ("" if none). */
+const char *adrt_hip_last_error(void);
+
+/* Library ABI version (major * 100 + minor). */
+int adrt_hip_abi_version(void);
+
+/* Number of visible HIP devices (0 when none or the runtime is unusable). */
+int adrt_hip_device_count(void);
+
+/* ---- shape rules (no device work) --------------------------------------- */
+
+/* num_iters(n) = bit_width(n) - has_single_bit(n); adrt_cdefs_common.cpp:140-142 */
+int adrt_hip_num_iters(size_t n);
+
+/* (batch, n, n) is a valid adrt input: all > 0, square, n a power of two,
+ * n <= 2^63.  adrt_cdefs_common.cpp:176-182 */
+int adrt_hip_valid_adrt_shape(const size_t shape[3]);
+
+/* (batch, 4, 2n-1, n) is a valid ADRT-output shape (bdrt, iadrt, adrt_step,
+ * bdrt_step input).  adrt_cdefs_common.cpp:185-192 */
+int adrt_hip_valid_output_shape(const size_t shape[4]);
+
+/* 0 <= step < num_iters(shape[3]).  adrt_cdefs_common.cpp:195-197 */
+int adrt_hip_valid_step(const size_t shape[4], int step);
+
+/* Restriction input: (batch, 4, 2n-1, n) with n even, n >= 2.
+ * adrt_cdefs_common.cpp:289-296 */
+int adrt_hip_valid_restriction_shape(const size_t shape[4]);
+
+/* Prolongation input: all > 0 and both image dims <= SIZE_MAX/2.
+ * adrt_cdefs_common.cpp:307-312 */
+int adrt_hip_valid_prolongation_shape(const size_t shape[3]);
+
+/* High-pass input: all > 0 and both image dims >= 2.
+ * adrt_cdefs_common.cpp:322-326 */
+int adrt_hip_valid_highpass_shape(const size_t shape[3]);
+
+/* Bytes of workspace `op` needs for `batch` images of side n (0 = none). */
+size_t adrt_hip_workspace_bytes(int op, int dtype, size_t batch, size_t n);
+
+/* ---- transforms ----------------------------------------------------------- */
+
+/* Forward ADRT: (batch, n, n) -> (batch, 4, 2n-1, n).
+ * Replaces adrt._adrt_cdefs.adrt (adrt_cdefs_py.cpp:275-341 ->
+ * adrt_basic, adrt_cdefs_adrt.hpp:100-212).  Bit-exact to the reference. */
+int adrt_hip_adrt(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                  int dtype, size_t batch, size_t n, void *stream);
+
+/* One forward stage on the ADRT-output layout, 0 <= step < num_iters(n).
+ * Replaces adrt._adrt_cdefs.adrt_step (adrt_cdefs_py.cpp:343-412 ->
+ * adrt_step, adrt_cdefs_adrt.hpp:214-258). */
+int adrt_hip_adrt_step(const void *in, void *out, int dtype, size_t batch, size_t n,
+                       int step, void *stream);
+
+/* Back-projection: (batch, 4, 2n-1, n) -> same shape.
+ * Replaces adrt._adrt_cdefs.bdrt (adrt_cdefs_py.cpp:482-548 ->
+ * bdrt_basic, adrt_cdefs_bdrt.hpp:120-187). */
+int adrt_hip_bdrt(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                  int dtype, size_t batch, size_t n, void *stream);
+
+/* One back-projection stage.  Replaces adrt._adrt_cdefs.bdrt_step
+ * (adrt_cdefs_py.cpp:550-619 -> bdrt_step, adrt_cdefs_bdrt.hpp:189-244). */
+int adrt_hip_bdrt_step(const void *in, void *out, int dtype, size_t batch, size_t n,
+                       int step, void *stream);
+
+/* Exact inverse: (batch, 4, 2n-1, n) -> same shape.
+ * Replaces adrt._adrt_cdefs.iadrt (adrt_cdefs_py.cpp:414-480 ->
+ * iadrt_basic, adrt_cdefs_iadrt.hpp:109-176). */
+int adrt_hip_iadrt(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                   int dtype, size_t batch, size_t n, void *stream);
+
+/* Multigrid restriction (batch, 4, 2n-1, n) -> (batch, 4, n-1, n/2).
+ * Replaces adrt._adrt_cdefs.press_fmg_restriction (adrt_cdefs_py.cpp:684-738
+ * -> fmg_restriction, adrt_cdefs_fmg.hpp:53-73). */
+int adrt_hip_fmg_restriction(const void *in, void *out, int dtype, size_t batch, size_t n,
+                             void *stream);
+
+/* Multigrid prolongation (batch, h, w) -> (batch, 2h, 2w).
+ * Replaces adrt._adrt_cdefs.press_fmg_prolongation (adrt_cdefs_py.cpp:740-794
+ * -> fmg_prolongation, adrt_cdefs_fmg.hpp:75-95). */
+int adrt_hip_fmg_prolongation(const void *in, void *out, int dtype, size_t batch, size_t h,
+                              size_t w, void *stream);
+
+/* Multigrid high-pass filter (batch, h, w) -> same shape, h, w >= 2.
+ * Replaces adrt._adrt_cdefs.press_fmg_highpass (adrt_cdefs_py.cpp:796-850
+ * -> fmg_highpass, adrt_cdefs_fmg.hpp:97-171). */
+int adrt_hip_fmg_highpass(const void *in, void *out, int dtype, size_t batch, size_t h,
+                          size_t w, void *stream);
+
+/* ---- layout helpers (any element size: 1, 2, 4 or 8 bytes) --------------- */
+
+/* adrt.core.adrt_init (core.py:123-176): (batch, n, n) -> (batch, 4, 2n-1, n),
+ * quadrant flips into the top n offsets, zero elsewhere. */
+int adrt_hip_adrt_init(const void *in, void *out, size_t elem_bytes, size_t batch, size_t n,
+                       void *stream);
+
+/* adrt.utils.truncate (utils.py:191-242): (outer, 4, 2n-1, n) -> (outer, 4, n, n). */
+int adrt_hip_truncate(const void *in, void *out, size_t elem_bytes, size_t outer, size_t n,
+                      void *stream);
+
+/* ---- fused helpers for the solvers (no reference counterpart) ------------ */
+
+/* out[b] = combine_q(truncate(in)[b, q] / divisor), in (batch, 4, 2n-1, n),
+ * out (batch, n, n).  mode 0: ((q0+q1)+q2)+q3 then / 4 (NumPy mean over the
+ * quadrant axis, as in core.py:329); mode 1: plain sum ((q0+q1)+q2)+q3.
+ * divisor == 1 skips the per-element divide. */
+int adrt_hip_truncate_combine(const void *in, void *out, int dtype, size_t batch, size_t n,
+                              double divisor, int mode, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADRT_HIP_H */
