@@ -146,6 +146,15 @@ class TestAdrtCompute:
         x[-1, -1, -1] = np.nan
         assert_bitwise(adrt_amd.adrt(x), O.adrt(x), f"adrt {dt} n={n}")
 
+    @pytest.mark.parametrize("dt", ["float32", "float64"])
+    def test_parity_2048(self, cuda, dt):
+        x = np.random.default_rng(2048).standard_normal((2, 2048, 2048)).astype(dt)
+        assert_bitwise(adrt_amd.adrt(x), O.adrt(x), f"adrt {dt} n=2048")
+
+    def test_parity_4096(self, cuda):
+        x = np.random.default_rng(4096).standard_normal((1, 4096, 4096)).astype(np.float32)
+        assert_bitwise(adrt_amd.adrt(x), O.adrt(x), "adrt n=4096")
+
     def test_parity_1024_hash(self, cuda):
         big = np.random.default_rng(1).random((1, 1024, 1024), dtype=np.float32)
         assert canonical_sha256(adrt_amd.adrt(big)) == golden_hashes()["adrt_f32_1x1024_seed1"]
