@@ -8,77 +8,55 @@
 //   pass 1: for every row block J of 2^k1 rows, the first k1 stages
 //           -> X[J][s][c] for the 2^k1 stage-k1 angles s
 //   pass 2: for every angle s, the last k2 = L - k1 stages over the 2^k2
-//           blocks X[J][s][.], where block J is pre-shifted by s*J; the pass
-//           produces the final angles s*2^k2 + t directly in output layout.
+//           blocks X[J][s][.], block J pre-shifted by s*J; the pass writes
+//           the final angles s*2^k2 + t directly in the output layout.
 //
-// Each pass is itself two "levels" of at most 3 stages.  A level holds 2^K
-// rows x W offsets per thread in registers and runs its K stages with
-// compile-time shifts (ceil(t/2) of the local angle t); the level input angle
-// m contributes m * j to the shift of local row j, which is applied as a skew
-// when the rows are loaded.  Levels exchange data through LDS.  Every output
-// element is produced by the same sequence of single IEEE adds, with the same
-// operands, as in the reference, so results are bit-identical.  The
-// reference's plain copy for c < ceil(a/2) is reproduced by adding -0.0
-// (x + -0.0 == x for every x), so offsets below zero load as -0.0 and
-// offsets past the data load as +0.0 (the reference's zero padding).
+// Each pass is two "levels": a 3-stage level and a level of k-3 stages.  A
+// level runs on a strip: one wavefront, lane l holding 4 consecutive offsets
+// 4l..4l+3 of 2^K rows in registers.  Its K stages use compile-time shifts
+// ceil(t/2) of the local angle t; operands that lie in the previous lane are
+// read with DPP wave_shr:1, which the compiler folds into v_add_f32_dpp, so a
+// stage is exactly one add per element with no halo loads.  The level input
+// angle m contributes m*j to the shift of local row j and is applied as a
+// skew when row j is loaded.  Offsets of a strip below 2^K - 1 have no valid
+// halo and are discarded; each strip delivers offsets [64, 256) (lanes 16+),
+// which leaves room for the skew spread of the next level's input.  Levels
+// exchange through LDS.
+//
+// Exactness.  Every output element is the same sequence of single IEEE adds
+// of the same operands as in the reference, so results are bit-identical.
+// The reference's plain copy for c < ceil(a/2) is reproduced by adding -0.0
+// (x + -0.0 == x for every x): offsets below zero load as -0.0, offsets past
+// the data load as +0.0 (the reference's zero padding).
+//
+// Host emulation.  The per-lane load/store functions are __host__ __device__;
+// with ADRT_HOST_EMULATION defined only those and the whole-strip compute are
+// compiled, so tests/host_emulation can run every block on the CPU.
 #include "common.hpp"
 
 namespace adrt_hip {
 namespace fused {
+
+constexpr int kW = 4;                  // offsets per lane
+constexpr int kLanes = 64;             // lanes per strip (one wavefront)
+constexpr int kStrip = kW * kLanes;    // offsets per strip
+constexpr int kOut = 192;              // offsets a strip delivers: [kLead, kStrip)
+constexpr int kLead = kStrip - kOut;   // 64
+constexpr int kWaves = 8;              // wavefronts per work-group
+constexpr int kThreads = kWaves * kLanes;
 
 template <typename T>
 __host__ __device__ __forceinline__ T neg_zero() {
     return -T(0);
 }
 
-// K butterfly stages on 2^K rows held in registers.
-//   v[j][i]: level-input row j, slot i <-> offset base - H + i (H = 2^K - 1)
-//            after row j's skew.
-//   o[t][i]: local angle t at offset base + i.
-// Slot i of a stage output is only valid for i >= (accumulated shift); the
-// final slots H.. are always valid because the total shift of row j is <= j.
-template <int K, int W, typename T>
-__host__ __device__ __forceinline__ void small_adrt(const T (&v)[1 << K][W + (1 << K) - 1],
-                                           T (&o)[1 << K][W]) {
-    constexpr int R0 = 1 << K;
-    constexpr int H = R0 - 1;
-    constexpr int S = W + H;
-    T a[R0][S];
-#pragma unroll
-    for (int j = 0; j < R0; ++j)
-#pragma unroll
-        for (int i = 0; i < S; ++i) a[j][i] = v[j][i];
-#pragma unroll
-    for (int u = 0; u < K; ++u) {
-        const int A = 1 << u;    // angles per block before this stage
-        const int R = R0 >> u;   // blocks before this stage
-        T b[R0][S];
-#pragma unroll
-        for (int r = 0; r < R / 2; ++r) {
-#pragma unroll
-            for (int t = 0; t < 2 * A; ++t) {
-                const int sh = (t + 1) >> 1;
-                const int xa = (2 * r) * A + (t >> 1);
-                const int ya = (2 * r + 1) * A + (t >> 1);
-#pragma unroll
-                for (int i = 0; i < S; ++i) {
-                    if (i >= sh)
-                        b[r * 2 * A + t][i] = a[xa][i] + a[ya][i - sh];
-                    else
-                        b[r * 2 * A + t][i] = a[xa][i];  // invalid slot, never read
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < R0; ++j)
-#pragma unroll
-            for (int i = 0; i < S; ++i) a[j][i] = b[j][i];
-    }
-#pragma unroll
-    for (int t = 0; t < R0; ++t)
-#pragma unroll
-        for (int i = 0; i < W; ++i) o[t][i] = a[t][H + i];
+__host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
 }
+
+// LDS row of kStrip columns, stored interleaved so that the column
+// 4*lane + i read by every lane of a wave for fixed i is bank-consecutive.
+__host__ __device__ __forceinline__ int lds_col(int c) { return (c & (kW - 1)) * kLanes + (c >> 2); }
 
 // Quadrant view of the zero-padded init buffer (adrt_cdefs_adrt.hpp:123-186):
 //   q0: img[r][n-1-c]  q1: img[n-1-c][r]  q2: img[c][r]  q3: img[n-1-r][n-1-c]
@@ -95,60 +73,145 @@ __host__ __device__ __forceinline__ T quad_at(const T *__restrict__ img, int q, 
     }
 }
 
-// LDS rows are stored lane-interleaved: column c of a row of length LR (a
-// multiple of W) lives at (c % W) * (LR / W) + c / W, so the W-strided
-// columns read by consecutive lanes fall in consecutive banks.
-template <int W>
-__host__ __device__ __forceinline__ int lds_col(int c, int LR) {
-    return (c % W) * (LR / W) + c / W;
-}
+template <typename T>
+struct Vec4 {
+    T v[4];
+};
 
-constexpr int kThreads = 256;
-
-// Whole windows of width W needed below a chunk start to cover `halo` offsets.
-template <int W>
-__host__ __device__ constexpr int pre_windows(int halo) {
-    return (halo + W - 1) / W;
-}
-
-template <int R, int G, int W>
-__host__ __device__ constexpr int pre_windows_total() {
-    int t = 0;
-    for (int g = 0; g < G; ++g) t += pre_windows<W>(R * g);
-    return t;
+// Four consecutive elements [c, c+4) of a padded row row[0, lim): -0.0
+// below 0 and +0.0 from lim on.  Interior windows use one 16-byte load.
+template <typename T>
+__host__ __device__ __forceinline__ void load4_padded(const T *__restrict__ row, int c, int lim,
+                                                      T (&out)[4]) {
+    if (c >= 0 && c + 4 <= lim) {
+        Vec4<T> t;
+        __builtin_memcpy(&t, row + c, sizeof(t));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[i] = t.v[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cc = c + i;
+            out[i] = cc < 0 ? neg_zero<T>() : (cc >= lim ? T(0) : row[cc]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1: rows J*2^k1 .. +2^k1 of quadrant q, offsets [c0, c0+D) of X.
-// Level A: sub-blocks g of 2^KA rows -> 2^KA angles; level B: 2^KB sub-blocks
-// at angle mA -> angles s = mA*2^KB + tB.  The work of one block is split in
-// phases separated by a block barrier; the phases are plain functions of
-// (block, thread) so they also run on the host for checking.
+// Strip compute: K stages on R = 2^K rows, a[j][i] = row j at strip offset
+// 4*lane + i -> a[t][i] = local angle t.  Valid for strip offsets >= R - 1.
 // ---------------------------------------------------------------------------
-template <typename T, int KA, int KB, int W, int D>
+#ifndef ADRT_HOST_EMULATION
+template <typename T>
+__device__ __forceinline__ T lane_shr1(T v);
+
+template <>
+__device__ __forceinline__ float lane_shr1<float>(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+
+template <>
+__device__ __forceinline__ double lane_shr1<double>(double v) {
+    const long long bits = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(bits & 0xffffffff), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(bits >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                  (unsigned long long)(unsigned)lo));
+}
+
+template <int K, typename T>
+__device__ __forceinline__ void strip_adrt(T (&a)[1 << K][kW]) {
+    constexpr int R0 = 1 << K;
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const int A = 1 << u;   // angles per block before this stage
+        const int R = R0 >> u;  // blocks before this stage
+        T b[R0][kW];
+#pragma unroll
+        for (int r = 0; r < R / 2; ++r) {
+#pragma unroll
+            for (int t = 0; t < 2 * A; ++t) {
+                const int sh = (t + 1) >> 1;  // <= kW for K <= 3
+                const int xa = (2 * r) * A + (t >> 1);
+                const int ya = (2 * r + 1) * A + (t >> 1);
+#pragma unroll
+                for (int i = 0; i < kW; ++i) {
+                    const T y = (i >= sh) ? a[ya][i - sh] : lane_shr1(a[ya][kW + i - sh]);
+                    b[r * 2 * A + t][i] = a[xa][i] + y;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R0; ++j)
+#pragma unroll
+            for (int i = 0; i < kW; ++i) a[j][i] = b[j][i];
+    }
+}
+#endif
+
+// Whole-strip restatement for host emulation: a[lane][row][i]; offsets with
+// no valid halo are poisoned with NaN.
+template <int K, typename T>
+inline void strip_adrt_host(T (*a)[1 << K][kW]) {
+    constexpr int R0 = 1 << K;
+    static T cur[R0][kStrip], nxt[R0][kStrip];
+    for (int l = 0; l < kLanes; ++l)
+        for (int j = 0; j < R0; ++j)
+            for (int i = 0; i < kW; ++i) cur[j][l * kW + i] = a[l][j][i];
+    for (int u = 0; u < K; ++u) {
+        const int A = 1 << u, R = R0 >> u;
+        for (int r = 0; r < R / 2; ++r)
+            for (int t = 0; t < 2 * A; ++t) {
+                const int sh = (t + 1) >> 1;
+                const int xa = (2 * r) * A + (t >> 1), ya = (2 * r + 1) * A + (t >> 1);
+                for (int o = 0; o < kStrip; ++o)
+                    nxt[r * 2 * A + t][o] = cur[xa][o] + (o >= sh ? cur[ya][o - sh] : T(NAN));
+            }
+        for (int j = 0; j < R0; ++j)
+            for (int o = 0; o < kStrip; ++o) cur[j][o] = nxt[j][o];
+    }
+    for (int l = 0; l < kLanes; ++l)
+        for (int j = 0; j < R0; ++j)
+            for (int i = 0; i < kW; ++i) a[l][j][i] = cur[j][l * kW + i];
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: rows row0 .. row0 + 2^k1 of quadrant q, X offsets [c0, c0 + kOut).
+//   phase 0 (level A): strip g (2^KB of them): rows row0 + 8g + j, base
+//     c0 - HB - 7g - 7 -> LA[(g, tA)] (kStrip columns from that base)
+//   phase 1 (level B): strip mA (8 of them): rows LA[(g, mA)] skewed by
+//     mA*g, base c0 - kLead -> X[J][mA*2^KB + tB][c] for c in [c0, c0+kOut)
+// With KB == 0 phase 0 writes X directly and phase 1 is empty.
+// ---------------------------------------------------------------------------
+template <typename T, int KB>
 struct Pass1 {
+    static constexpr int KA = 3;
     static constexpr int k1 = KA + KB;
     static constexpr int RA = 1 << KA, RB = 1 << KB;
-    static constexpr int HA = RA - 1, HB = RB - 1;
-    static constexpr int LR = D + (1 << k1);  // LDS row: offsets [c0 - 2^k1, c0 + D)
-    static constexpr int wins_base = D / W;
-    static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) * LR;
+    static constexpr int HB = RB - 1;
+    static constexpr int R0 = RA, R1 = RB;
+    static constexpr int tasks0 = RB;
+    static constexpr int tasks1 = KB == 0 ? 0 : RA;
+    static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) * kStrip;
 
     struct Item {
         const T *img;
-        T *X;
+        T *X;  // rows of block J: X + s * P1
         int q, n, c0, row0, W1, P1;
         bool valid;
     };
 
-    __host__ __device__ static int num_chunks(int n) { return (n + (1 << k1) - 1 + D - 1) / D; }
+    __host__ __device__ static int num_chunks(int n) {
+        return (n + (1 << k1) - 1 + kOut - 1) / kOut;
+    }
 
     __host__ __device__ static Item item(const T *img_base, T *X_base, int n, int first_image,
                                          int nimg, int block) {
         Item it;
         it.n = n;
         it.W1 = n + (1 << k1) - 1;  // valid offsets of X
-        it.P1 = n + (1 << k1);      // row pitch of X
+        it.P1 = n + (1 << k1);      // row pitch of X (multiple of 4)
         const int nJ = n >> k1;
         const int nchunk = num_chunks(n);
         const int chunk = block % nchunk;
@@ -158,7 +221,7 @@ struct Pass1 {
         it.q = block & 3;
         const int b = first_image + (block >> 2);
         it.valid = b < first_image + nimg;
-        it.c0 = chunk * D;
+        it.c0 = chunk * kOut;
         it.img = img_base + (size_t)b * n * n;
         it.X = X_base + ((size_t)(b - first_image) * 4 + it.q) * nJ * ((size_t)1 << k1) * it.P1 +
                (size_t)J * ((size_t)1 << k1) * it.P1;
@@ -166,135 +229,132 @@ struct Pass1 {
         return it;
     }
 
-    // Level A (or the only level when KB == 0).
-    __host__ __device__ static void phase0(const Item &it, T *lds, int tid) {
-        if constexpr (KB == 0) {
-            for (int task = tid; task < wins_base; task += kThreads) {
-                const int w0 = it.c0 + task * W;
-                T v[RA][W + HA];
+    __host__ __device__ static int base0(const Item &it, int g) {
+        return KB == 0 ? it.c0 - kLead : it.c0 - HB - (RA - 1) * g - (RA - 1);
+    }
+
+    // Level A input: 8 quadrant rows at offsets base0 + 4*lane + i.
+    __host__ __device__ static void load0(const Item &it, int g, int lane, T (&v)[RA][kW]) {
+        const int c = base0(it, g) + kW * lane;
+        const int r0 = it.row0 + g * RA;
+        const int n = it.n;
+        if (it.q == 1 || it.q == 2) {
+            // Rows are image columns r0..r0+7, offsets index image rows.
 #pragma unroll
-                for (int j = 0; j < RA; ++j)
+            for (int i = 0; i < kW; ++i) {
+                const int cc = c + i;
+                if (cc < 0 || cc >= n) {
+                    const T pad = cc < 0 ? neg_zero<T>() : T(0);
 #pragma unroll
-                    for (int i = 0; i < W + HA; ++i)
-                        v[j][i] = quad_at(it.img, it.q, it.n, it.row0 + j, w0 - HA + i);
-                T o[RA][W];
-                small_adrt<KA, W, T>(v, o);
+                    for (int j = 0; j < RA; ++j) v[j][i] = pad;
+                } else {
+                    const int irow = it.q == 1 ? n - 1 - cc : cc;
+                    Vec4<T> lo, hi;
+                    __builtin_memcpy(&lo, it.img + (size_t)irow * n + r0, sizeof(lo));
+                    __builtin_memcpy(&hi, it.img + (size_t)irow * n + r0 + 4, sizeof(hi));
 #pragma unroll
-                for (int t = 0; t < RA; ++t)
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        if (w0 + i < it.W1) it.X[(size_t)t * it.P1 + w0 + i] = o[t][i];
-            }
-        } else {
-            // Sub-block g needs offsets [c0 - RA*g, c0 + D), in whole windows.
-            constexpr int tasks_a = RB * wins_base + pre_windows_total<RA, RB, W>();
-            for (int task = tid; task < tasks_a; task += kThreads) {
-                int g = 0, rem = task;
-#pragma unroll
-                for (int gg = 0; gg < RB; ++gg) {
-                    const int cnt = wins_base + pre_windows<W>(RA * gg);
-                    if (g == gg && rem >= cnt) {
-                        rem -= cnt;
-                        g = gg + 1;
+                    for (int j = 0; j < 4; ++j) {
+                        v[j][i] = lo.v[j];
+                        v[j + 4][i] = hi.v[j];
                     }
                 }
-                const int w0 = it.c0 - pre_windows<W>(RA * g) * W + rem * W;
-                T v[RA][W + HA];
+            }
+        } else {
+            // Rows are image rows (q0: r, q3: n-1-r); offsets c..c+3 are the
+            // columns n-1-c-3 .. n-1-c in reverse order.
 #pragma unroll
-                for (int j = 0; j < RA; ++j)
+            for (int j = 0; j < RA; ++j) {
+                const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
+                const T *row = it.img + (size_t)irow * n;
+                if (c >= 0 && c + kW <= n) {
+                    Vec4<T> t;
+                    __builtin_memcpy(&t, row + (n - kW - c), sizeof(t));
 #pragma unroll
-                    for (int i = 0; i < W + HA; ++i)
-                        v[j][i] = quad_at(it.img, it.q, it.n, it.row0 + g * RA + j, w0 - HA + i);
-                T o[RA][W];
-                small_adrt<KA, W, T>(v, o);
-                const int cbase = w0 - (it.c0 - (1 << k1));
+                    for (int i = 0; i < kW; ++i) v[j][i] = t.v[kW - 1 - i];
+                } else {
 #pragma unroll
-                for (int t = 0; t < RA; ++t)
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        lds[(g * RA + t) * LR + lds_col<W>(cbase + i, LR)] = o[t][i];
+                    for (int i = 0; i < kW; ++i) v[j][i] = quad_at(it.img, it.q, n, r0 + j, c + i);
+                }
             }
         }
     }
 
-    // Level B: angle mA, windows in [c0, c0 + D); sub-block g skewed by mA*g.
-    __host__ __device__ static void phase1(const Item &it, const T *lds, int tid) {
-        if constexpr (KB != 0) {
-            for (int task = tid; task < RA * wins_base; task += kThreads) {
-                const int mA = task / wins_base;
-                const int w0 = it.c0 + (task % wins_base) * W;
-                T v[RB][W + HB];
+    // X rows mA*R + t at offsets c0 - kLead + 4*lane + i; lanes 16..63 deliver
+    // [c0, c0 + kOut).  A window may run past W1 up to P1 - 1: harmless padding.
+    template <int R>
+    __host__ __device__ static void store_x(const Item &it, int mA, int lane, const T (&o)[R][kW]) {
+        const int c = it.c0 - kLead + kW * lane;
+        if (lane < kLead / kW || c >= it.W1) return;
 #pragma unroll
-                for (int g = 0; g < RB; ++g) {
-                    const T *row = lds + (g * RA + mA) * LR;
-                    const int cb = w0 - HB - mA * g - (it.c0 - (1 << k1));  // >= RA > 0
+        for (int t = 0; t < R; ++t) {
+            Vec4<T> v;
 #pragma unroll
-                    for (int i = 0; i < W + HB; ++i) v[g][i] = row[lds_col<W>(cb + i, LR)];
-                }
-                T o[RB][W];
-                small_adrt<KB, W, T>(v, o);
-#pragma unroll
-                for (int t = 0; t < RB; ++t) {
-                    T *xr = it.X + (size_t)(mA * RB + t) * it.P1;
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        if (w0 + i < it.W1) xr[w0 + i] = o[t][i];
-                }
-            }
+            for (int i = 0; i < kW; ++i) v.v[i] = o[t][i];
+            __builtin_memcpy(it.X + (size_t)(mA * R + t) * it.P1 + c, &v, sizeof(v));
         }
+    }
+
+    __host__ __device__ static void store0(const Item &it, int g, int lane, const T (&o)[RA][kW],
+                                           T *lds) {
+        if constexpr (KB == 0) {
+            (void)lds;
+            store_x<RA>(it, g, lane, o);
+        } else {
+#pragma unroll
+            for (int t = 0; t < RA; ++t)
+#pragma unroll
+                for (int i = 0; i < kW; ++i)
+                    lds[(g * RA + t) * kStrip + lds_col(kW * lane + i)] = o[t][i];
+        }
+    }
+
+    // Level B input: rows g of LA at angle mA, skewed by mA*g.
+    __host__ __device__ static void load1(const Item &it, int mA, int lane, const T *lds,
+                                          T (&v)[RB][kW]) {
+        (void)it;
+#pragma unroll
+        for (int g = 0; g < RB; ++g) {
+            // strip column of offset (c0 - kLead + 4*lane + i - mA*g) - base0(g)
+            const int cb = kW * lane - kLead + HB + (RA - 1) + (RA - 1 - mA) * g;
+            const T *row = lds + (g * RA + mA) * kStrip;
+#pragma unroll
+            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_col(clampi(cb + i, 0, kStrip - 1))];
+        }
+    }
+
+    __host__ __device__ static void store1(const Item &it, int mA, int lane, const T (&o)[RB][kW]) {
+        store_x<RB>(it, mA, lane, o);
     }
 };
 
-#ifndef ADRT_HOST_EMULATION
-template <typename T, int KA, int KB, int W, int D>
-__global__ void __launch_bounds__(kThreads)
-    fwd_pass1(const T *__restrict__ img_base, T *__restrict__ X_base, int n, int first_image,
-              int nimg) {
-    using P = Pass1<T, KA, KB, W, D>;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-    const typename P::Item it = P::item(img_base, X_base, n, first_image, nimg, blockIdx.x);
-    if (!it.valid) return;
-    P::phase0(it, lds, threadIdx.x);
-    if constexpr (KB != 0) {
-        __syncthreads();
-        P::phase1(it, lds, threadIdx.x);
-    }
-}
-#endif  // ADRT_HOST_EMULATION
-
 // ---------------------------------------------------------------------------
-// Pass 2: angle s of quadrant q, output offsets [d0, d0 + D).
-// Level C: groups g of 2^KC blocks J = g*2^KC + j at angle s (row j skewed by
-// s*j) -> angles tC; level D: 2^KD groups at angle m = s*2^KC + tC (group g
-// skewed by m*g) -> final angles s*2^k2 + tC*2^KD + tD.
+// Pass 2: angle s of quadrant q, output offsets [d0, d0 + kOut).
+//   phase 0 (level C): strip g (2^KD of them): blocks J = 8g + j of X at
+//     angle s, row j skewed by s*j, base d0 - HD - (8s+7)g - 7 -> LC[(g, tC)]
+//   phase 1 (level D): strip tC (8 of them): rows LC[(g, tC)] skewed by
+//     (8s + tC)*g, base d0 - kLead -> out[d][s*2^k2 + tC*2^KD + tD]
+// With KD == 0 phase 0 writes the output directly.
 // ---------------------------------------------------------------------------
-template <typename T>
-__host__ __device__ __forceinline__ T x_at(const T *__restrict__ xrow, int c, int W1) {
-    if (c < 0) return neg_zero<T>();
-    if (c >= W1) return T(0);
-    return xrow[c];
-}
-
-template <typename T, int KC, int KD, int W, int D>
+template <typename T, int KD>
 struct Pass2 {
+    static constexpr int KC = 3;
     static constexpr int k2 = KC + KD;
     static constexpr int RC = 1 << KC, RD = 1 << KD;
-    static constexpr int HC = RC - 1, HD = RD - 1;
-    static constexpr int LR = D + (1 << k2);
-    static constexpr int LP = LR + 8;  // row pitch: spreads level-D lanes over banks
-    static constexpr int wins_base = D / W;
-    static constexpr size_t lds_elems = KD == 0 ? 0 : size_t(1 << k2) * LP;
+    static constexpr int HD = RD - 1;
+    static constexpr int R0 = RC, R1 = RD;
+    static constexpr int tasks0 = RD;
+    static constexpr int tasks1 = KD == 0 ? 0 : RC;
+    static constexpr size_t lds_elems = KD == 0 ? 0 : size_t(1 << k2) * kStrip;
 
     struct Item {
-        const T *X;
-        T *out;
+        const T *X;  // row (J, s) at X + J * xrow_stride
+        T *out;      // plane (b, q)
         int n, s, d0, C, W1, abase;
         size_t xrow_stride;
         bool valid;
     };
 
-    __host__ __device__ static int num_chunks(int n) { return (2 * n - 1 + D - 1) / D; }
+    __host__ __device__ static int num_chunks(int n) { return (2 * n - 1 + kOut - 1) / kOut; }
 
     __host__ __device__ static Item item(const T *X_base, T *out_base, int n, int k1,
                                          int first_image, int nimg, int block) {
@@ -312,7 +372,7 @@ struct Pass2 {
         const int q = block & 3;
         const int b = first_image + (block >> 2);
         it.valid = b < first_image + nimg;
-        it.d0 = chunk * D;
+        it.d0 = chunk * kOut;
         it.xrow_stride = (size_t)nS * P1;  // between blocks J at fixed s
         it.X = X_base + ((size_t)(b - first_image) * 4 + q) * ((size_t)1 << k2) * it.xrow_stride +
                (size_t)it.s * P1;
@@ -321,208 +381,126 @@ struct Pass2 {
         return it;
     }
 
-    // Level C (or the only level when KD == 0).
-    __host__ __device__ static void phase0(const Item &it, T *lds, int tid) {
-        const int s = it.s;
-        if constexpr (KD == 0) {
-            for (int task = tid; task < wins_base; task += kThreads) {
-                const int w0 = it.d0 + task * W;
-                T v[RC][W + HC];
+    __host__ __device__ static int base0(const Item &it, int g) {
+        return KD == 0 ? it.d0 - kLead : it.d0 - HD - (RC * it.s + RC - 1) * g - (RC - 1);
+    }
+
+    // Level C input: blocks 8g + j at angle s, row j skewed by s*j.
+    __host__ __device__ static void load0(const Item &it, int g, int lane, T (&v)[RC][kW]) {
+        const int c = base0(it, g) + kW * lane;
 #pragma unroll
-                for (int j = 0; j < RC; ++j) {
-                    const T *xr = it.X + (size_t)j * it.xrow_stride;
+        for (int j = 0; j < RC; ++j)
+            load4_padded(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
+    }
+
+    // Output rows d = d0 - kLead + 4*lane + i (lanes 16..63), angles
+    // abase + tC*R + t.
+    template <int R>
+    __host__ __device__ static void store_out(const Item &it, int tC, int lane, const T (&o)[R][kW]) {
+        const int d = it.d0 - kLead + kW * lane;
+        if (lane < kLead / kW) return;
 #pragma unroll
-                    for (int i = 0; i < W + HC; ++i) v[j][i] = x_at(xr, w0 - HC + i - s * j, it.W1);
+        for (int i = 0; i < kW; ++i) {
+            if (d + i >= it.C) break;
+            T *orow = it.out + (size_t)(d + i) * it.n + it.abase + tC * R;
+            if constexpr (R % 4 == 0) {
+#pragma unroll
+                for (int t = 0; t < R; t += 4) {
+                    Vec4<T> v;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v.v[k] = o[t + k][i];
+                    __builtin_memcpy(orow + t, &v, sizeof(v));
                 }
-                T o[RC][W];
-                small_adrt<KC, W, T>(v, o);
+            } else {
 #pragma unroll
-                for (int i = 0; i < W; ++i)
-                    if (w0 + i < it.C) {
-                        T *orow = it.out + (size_t)(w0 + i) * it.n + it.abase;
-#pragma unroll
-                        for (int t = 0; t < RC; ++t) orow[t] = o[t][i];
-                    }
-            }
-        } else {
-            // Group g covers [Sg, d0 + D - s*RC*g), Sg = d0 - s*RC*g - pre(RC*g)*W.
-            constexpr int tasks_c = RD * wins_base + pre_windows_total<RC, RD, W>();
-            for (int task = tid; task < tasks_c; task += kThreads) {
-                int g = 0, rem = task;
-#pragma unroll
-                for (int gg = 0; gg < RD; ++gg) {
-                    const int cnt = wins_base + pre_windows<W>(RC * gg);
-                    if (g == gg && rem >= cnt) {
-                        rem -= cnt;
-                        g = gg + 1;
-                    }
-                }
-                const int Sg = it.d0 - s * RC * g - pre_windows<W>(RC * g) * W;
-                const int w0 = Sg + rem * W;
-                T v[RC][W + HC];
-#pragma unroll
-                for (int j = 0; j < RC; ++j) {
-                    const T *xr = it.X + (size_t)(g * RC + j) * it.xrow_stride;
-#pragma unroll
-                    for (int i = 0; i < W + HC; ++i) v[j][i] = x_at(xr, w0 - HC + i - s * j, it.W1);
-                }
-                T o[RC][W];
-                small_adrt<KC, W, T>(v, o);
-#pragma unroll
-                for (int t = 0; t < RC; ++t)
-#pragma unroll
-                    for (int i = 0; i < W; ++i)
-                        lds[(g * RC + t) * LP + lds_col<W>(rem * W + i, LR)] = o[t][i];
+                for (int t = 0; t < R; ++t) orow[t] = o[t][i];
             }
         }
     }
 
-    // Level D: m = s*RC + tC; group g read at offset w0 + i - HD - m*g.
-    __host__ __device__ static void phase1(const Item &it, const T *lds, int tid) {
-        if constexpr (KD != 0) {
-            for (int task = tid; task < RC * wins_base; task += kThreads) {
-                const int tC = task % RC;
-                const int w0 = it.d0 + (task / RC) * W;
-                T v[RD][W + HD];
+    __host__ __device__ static void store0(const Item &it, int g, int lane, const T (&o)[RC][kW],
+                                           T *lds) {
+        if constexpr (KD == 0) {
+            (void)lds;
+            store_out<RC>(it, 0, lane, o);
+        } else {
 #pragma unroll
-                for (int g = 0; g < RD; ++g) {
-                    const T *row = lds + (g * RC + tC) * LP;
-                    const int cb = (w0 - it.d0) - HD - tC * g + pre_windows<W>(RC * g) * W;
+            for (int t = 0; t < RC; ++t)
 #pragma unroll
-                    for (int i = 0; i < W + HD; ++i) {
-                        int col = cb + i;
-                        col = col < 0 ? 0 : col;  // clamped slots are never used
-                        v[g][i] = row[lds_col<W>(col, LR)];
-                    }
-                }
-                T o[RD][W];
-                small_adrt<KD, W, T>(v, o);
-                const int a0 = it.abase + tC * RD;
-#pragma unroll
-                for (int i = 0; i < W; ++i)
-                    if (w0 + i < it.C) {
-                        T *orow = it.out + (size_t)(w0 + i) * it.n + a0;
-#pragma unroll
-                        for (int t = 0; t < RD; ++t) orow[t] = o[t][i];
-                    }
-            }
+                for (int i = 0; i < kW; ++i)
+                    lds[(g * RC + t) * kStrip + lds_col(kW * lane + i)] = o[t][i];
         }
+    }
+
+    // Level D input: rows g of LC at angle tC, skewed by (8s + tC)*g.
+    __host__ __device__ static void load1(const Item &it, int tC, int lane, const T *lds,
+                                          T (&v)[RD][kW]) {
+        (void)it;
+#pragma unroll
+        for (int g = 0; g < RD; ++g) {
+            // strip column of offset (d0 - kLead + 4*lane + i - (8s+tC)*g) - base0(g)
+            const int cb = kW * lane - kLead + HD + (RC - 1) + (RC - 1 - tC) * g;
+            const T *row = lds + (g * RC + tC) * kStrip;
+#pragma unroll
+            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_col(clampi(cb + i, 0, kStrip - 1))];
+        }
+    }
+
+    __host__ __device__ static void store1(const Item &it, int tC, int lane, const T (&o)[RD][kW]) {
+        store_out<RD>(it, tC, lane, o);
     }
 };
 
 #ifndef ADRT_HOST_EMULATION
-template <typename T, int KC, int KD, int W, int D>
+constexpr int log2_small(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
+
+template <typename P, typename T>
+__device__ __forceinline__ void run_block(const typename P::Item &it, T *lds) {
+    const int wave = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    for (int task = wave; task < P::tasks0; task += kWaves) {
+        T v[P::R0][kW];
+        P::load0(it, task, lane, v);
+        strip_adrt<log2_small(P::R0), T>(v);
+        P::store0(it, task, lane, v, lds);
+    }
+    if constexpr (P::tasks1 > 0) {
+        __syncthreads();
+        for (int task = wave; task < P::tasks1; task += kWaves) {
+            T v[P::R1][kW];
+            P::load1(it, task, lane, lds, v);
+            strip_adrt<log2_small(P::R1), T>(v);
+            P::store1(it, task, lane, v);
+        }
+    }
+}
+
+template <typename T, int KB>
+__global__ void __launch_bounds__(kThreads)
+    fwd_pass1(const T *__restrict__ img_base, T *__restrict__ X_base, int n, int first_image,
+              int nimg) {
+    using P = Pass1<T, KB>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    const typename P::Item it = P::item(img_base, X_base, n, first_image, nimg, blockIdx.x);
+    if (!it.valid) return;
+    run_block<P, T>(it, lds);
+}
+
+template <typename T, int KD>
 __global__ void __launch_bounds__(kThreads)
     fwd_pass2(const T *__restrict__ X_base, T *__restrict__ out_base, int n, int k1,
               int first_image, int nimg) {
-    using P = Pass2<T, KC, KD, W, D>;
+    using P = Pass2<T, KD>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     const typename P::Item it = P::item(X_base, out_base, n, k1, first_image, nimg, blockIdx.x);
     if (!it.valid) return;
-    P::phase0(it, lds, threadIdx.x);
-    if constexpr (KD != 0) {
-        __syncthreads();
-        P::phase1(it, lds, threadIdx.x);
-    }
+    run_block<P, T>(it, lds);
 }
 #endif  // ADRT_HOST_EMULATION
 
-}  // namespace fused
-}  // namespace adrt_hip
-
-namespace adrt_hip {
-namespace fused {
-
-// Launch geometry per element type: offsets per thread (W) and per work
-// item (D1 for pass 1, D2 for pass 2).
-template <typename T>
-struct Geo;
-template <>
-struct Geo<float> {
-    static constexpr int W = 8, D1 = 256, D2 = 256;
-};
-template <>
-struct Geo<double> {
-    static constexpr int W = 4, D1 = 128, D2 = 128;
-};
-
-template <typename T, int KA, int KB>
-using P1Of = Pass1<T, KA, KB, Geo<T>::W, Geo<T>::D1>;
-template <typename T, int KC, int KD>
-using P2Of = Pass2<T, KC, KD, Geo<T>::W, Geo<T>::D2>;
-
-#ifndef ADRT_HOST_EMULATION
-template <typename T, int KA, int KB>
-static void launch_pass1(const T *img, T *X, int n, int first, int count, hipStream_t s) {
-    using P = P1Of<T, KA, KB>;
-    const unsigned grid = unsigned(count) * 4u * unsigned(n >> P::k1) * unsigned(P::num_chunks(n));
-    constexpr size_t lds = P::lds_elems * sizeof(T);
-    auto kern = fwd_pass1<T, KA, KB, Geo<T>::W, Geo<T>::D1>;
-    static bool attr_set = false;
-    if (lds > 0 && !attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, img, X, n, first, count);
-}
-
-template <typename T, int KC, int KD>
-static void launch_pass2(const T *X, T *out, int n, int k1, int first, int count,
-                         hipStream_t s) {
-    using P = P2Of<T, KC, KD>;
-    const unsigned grid = unsigned(count) * 4u * unsigned(1 << k1) * unsigned(P::num_chunks(n));
-    constexpr size_t lds = P::lds_elems * sizeof(T);
-    auto kern = fwd_pass2<T, KC, KD, Geo<T>::W, Geo<T>::D2>;
-    static bool attr_set = false;
-    if (lds > 0 && !attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, X, out, n, k1, first, count);
-}
-
-// Split of L = log2 n stages: pass 1 takes k1 = ceil(L/2), pass 2 the rest;
-// each pass is a 3-stage level plus a level of k-3 stages.
-template <typename T>
-static void run_group(const T *img, T *X, T *out, int n, int L, int first, int count,
-                      hipStream_t s) {
-    switch (L) {
-        case 6:
-            launch_pass1<T, 3, 0>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 0>(X, out, n, 3, first, count, s);
-            break;
-        case 7:
-            launch_pass1<T, 3, 1>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 0>(X, out, n, 4, first, count, s);
-            break;
-        case 8:
-            launch_pass1<T, 3, 1>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 1>(X, out, n, 4, first, count, s);
-            break;
-        case 9:
-            launch_pass1<T, 3, 2>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 1>(X, out, n, 5, first, count, s);
-            break;
-        case 10:
-            launch_pass1<T, 3, 2>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 2>(X, out, n, 5, first, count, s);
-            break;
-        case 11:
-            launch_pass1<T, 3, 3>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 2>(X, out, n, 6, first, count, s);
-            break;
-        default:  // 12
-            launch_pass1<T, 3, 3>(img, X, n, first, count, s);
-            launch_pass2<T, 3, 3>(X, out, n, 6, first, count, s);
-            break;
-    }
-}
-
-#endif  // ADRT_HOST_EMULATION
-
+// Split of L = log2 n stages: pass 1 takes k1 = ceil(L/2) = 3 + KB stages,
+// pass 2 the remaining k2 = 3 + KD.
 inline int pass1_stages(int L) { return (L + 1) / 2; }
 
 // Images per group: keep one group's intermediate at or below ~64 MiB so it
@@ -538,10 +516,60 @@ inline size_t x_bytes_per_image(int n, int L, size_t elem) {
     return size_t(4) * n * size_t(n + (1 << k1)) * elem;
 }
 
+#ifndef ADRT_HOST_EMULATION
+template <typename T, int KB>
+static void launch_pass1(const T *img, T *X, int n, int first, int count, hipStream_t s) {
+    using P = Pass1<T, KB>;
+    const unsigned grid = unsigned(count) * 4u * unsigned(n >> P::k1) * unsigned(P::num_chunks(n));
+    constexpr size_t lds = P::lds_elems * sizeof(T);
+    auto kern = fwd_pass1<T, KB>;
+    static bool attr_set = false;
+    if (lds > 0 && !attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, img, X, n, first, count);
+}
+
+template <typename T, int KD>
+static void launch_pass2(const T *X, T *out, int n, int k1, int first, int count,
+                         hipStream_t s) {
+    using P = Pass2<T, KD>;
+    const unsigned grid = unsigned(count) * 4u * unsigned(1 << k1) * unsigned(P::num_chunks(n));
+    constexpr size_t lds = P::lds_elems * sizeof(T);
+    auto kern = fwd_pass2<T, KD>;
+    static bool attr_set = false;
+    if (lds > 0 && !attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, X, out, n, k1, first, count);
+}
+
+template <typename T>
+static void run_group(const T *img, T *X, T *out, int n, int L, int first, int count,
+                      hipStream_t s) {
+    const int k1 = pass1_stages(L);
+    switch (k1 - 3) {
+        case 0: launch_pass1<T, 0>(img, X, n, first, count, s); break;
+        case 1: launch_pass1<T, 1>(img, X, n, first, count, s); break;
+        case 2: launch_pass1<T, 2>(img, X, n, first, count, s); break;
+        default: launch_pass1<T, 3>(img, X, n, first, count, s); break;
+    }
+    switch (L - k1 - 3) {
+        case 0: launch_pass2<T, 0>(X, out, n, k1, first, count, s); break;
+        case 1: launch_pass2<T, 1>(X, out, n, k1, first, count, s); break;
+        case 2: launch_pass2<T, 2>(X, out, n, k1, first, count, s); break;
+        default: launch_pass2<T, 3>(X, out, n, k1, first, count, s); break;
+    }
+}
+#endif  // ADRT_HOST_EMULATION
+
 }  // namespace fused
 
 #ifndef ADRT_HOST_EMULATION
-
 bool fused_supported(size_t n) {
     const int L = num_iters(n);
     return is_pow2(n) && L >= 6 && L <= 12;
@@ -573,7 +601,6 @@ int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n
     return run_fused_t(static_cast<const double *>(in), static_cast<double *>(out),
                        static_cast<double *>(ws), B, n, s);
 }
-
 #endif  // ADRT_HOST_EMULATION
 
 }  // namespace adrt_hip

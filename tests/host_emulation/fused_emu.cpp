@@ -1,11 +1,15 @@
 // Host emulation of the fused forward kernels (adrt_amd/csrc/fused.hip).
 //
-// Compiles the exact Pass1/Pass2 phase functions for the CPU and runs every
-// (block, thread) of every launch serially, with a block barrier between the
-// phases, LDS modelled as an exactly sized heap buffer and X sized exactly as
-// the library's workspace.  Built with AddressSanitizer on the host, so any
-// out-of-bounds access of the kernels' index math aborts.  Usage:
+// Compiles the exact per-lane load/store functions of Pass1/Pass2 for the
+// CPU and runs every launch block by block: for each wavefront task all 64
+// lanes load, the strip is computed with the whole-strip restatement (offsets
+// without a valid halo poisoned with NaN), then all lanes store.  LDS is an
+// exactly sized heap buffer filled with NaN before every block and X is sized
+// exactly as the library's workspace, so any out-of-bounds access aborts
+// under AddressSanitizer and any read of an unwritten or invalid slot shows
+// up as NaN in the result.  Usage:
 //   fused_emu f32|f64 B n in.bin out.bin
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,29 +21,40 @@
 using namespace adrt_hip;
 using namespace adrt_hip::fused;
 
-template <typename P, typename... Args>
+constexpr int log2i(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
+
+template <typename P, typename T, typename... Args>
 static void emulate_grid(unsigned grid, Args... args) {
-    using T = std::remove_cv_t<std::remove_pointer_t<decltype(P::Item::X)>>;
-    std::vector<T> lds_buf(P::lds_elems > 0 ? P::lds_elems : 1);
+    std::vector<T> lds(P::lds_elems > 0 ? P::lds_elems : 1);
+    std::vector<T> buf0(size_t(kLanes) * P::R0 * kW), buf1(size_t(kLanes) * P::R1 * kW);
+    auto *v0 = reinterpret_cast<T(*)[P::R0][kW]>(buf0.data());
+    auto *v1 = reinterpret_cast<T(*)[P::R1][kW]>(buf1.data());
     for (unsigned blk = 0; blk < grid; ++blk) {
         const auto it = P::item(args..., int(blk));
         if (!it.valid) continue;
-        // Fill LDS with NaN so reads of never-written slots would poison results.
-        for (auto &v : lds_buf) v = T(NAN);
-        T *lds = P::lds_elems > 0 ? lds_buf.data() : nullptr;
-        for (int tid = 0; tid < kThreads; ++tid) P::phase0(it, lds, tid);
-        for (int tid = 0; tid < kThreads; ++tid) P::phase1(it, lds, tid);
+        for (auto &x : lds) x = T(NAN);
+        T *l = P::lds_elems > 0 ? lds.data() : nullptr;
+        for (int task = 0; task < P::tasks0; ++task) {
+            for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
+            strip_adrt_host<log2i(P::R0), T>(v0);
+            for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
+        }
+        for (int task = 0; task < P::tasks1; ++task) {
+            for (int lane = 0; lane < kLanes; ++lane) P::load1(it, task, lane, l, v1[lane]);
+            strip_adrt_host<log2i(P::R1), T>(v1);
+            for (int lane = 0; lane < kLanes; ++lane) P::store1(it, task, lane, v1[lane]);
+        }
     }
 }
 
-template <typename T, int KA, int KB, int KC, int KD>
+template <typename T, int KB, int KD>
 static void emulate_group(const T *img, T *X, T *out, int n, int first, int count) {
-    using P1 = P1Of<T, KA, KB>;
-    using P2 = P2Of<T, KC, KD>;
+    using P1 = Pass1<T, KB>;
+    using P2 = Pass2<T, KD>;
     const unsigned g1 = unsigned(count) * 4u * unsigned(n >> P1::k1) * unsigned(P1::num_chunks(n));
-    emulate_grid<P1>(g1, img, X, n, first, count);
+    emulate_grid<P1, T>(g1, img, X, n, first, count);
     const unsigned g2 = unsigned(count) * 4u * unsigned(1 << P1::k1) * unsigned(P2::num_chunks(n));
-    emulate_grid<P2>(g2, (const T *)X, out, n, P1::k1, first, count);
+    emulate_grid<P2, T>(g2, (const T *)X, out, n, P1::k1, first, count);
 }
 
 template <typename T>
@@ -54,16 +69,18 @@ static int run(int B, int n, const char *fin, const char *fout) {
     const size_t per = x_bytes_per_image(n, L, sizeof(T));
     const size_t G = group_images(size_t(B), per);
     std::vector<T> X(G * per / sizeof(T), T(NAN));
+    const int k1 = pass1_stages(L);
     for (int first = 0; first < B; first += int(G)) {
         const int count = (B - first) < int(G) ? (B - first) : int(G);
-        switch (L) {
-            case 6: emulate_group<T, 3, 0, 3, 0>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 7: emulate_group<T, 3, 1, 3, 0>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 8: emulate_group<T, 3, 1, 3, 1>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 9: emulate_group<T, 3, 2, 3, 1>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 10: emulate_group<T, 3, 2, 3, 2>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 11: emulate_group<T, 3, 3, 3, 2>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 12: emulate_group<T, 3, 3, 3, 3>(img.data(), X.data(), out.data(), n, first, count); break;
+        const int code = (k1 - 3) * 4 + (L - k1 - 3);
+        switch (code) {
+            case 0: emulate_group<T, 0, 0>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 4: emulate_group<T, 1, 0>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 5: emulate_group<T, 1, 1>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 9: emulate_group<T, 2, 1>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 10: emulate_group<T, 2, 2>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 14: emulate_group<T, 3, 2>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 15: emulate_group<T, 3, 3>(img.data(), X.data(), out.data(), n, first, count); break;
             default: return 3;
         }
     }
