@@ -44,19 +44,45 @@ constexpr int kOut = 192;              // offsets a strip delivers: [kLead, kStr
 constexpr int kLead = kStrip - kOut;   // 64
 constexpr int kWaves = 8;              // wavefronts per work-group
 constexpr int kThreads = kWaves * kLanes;
+constexpr int kPad = 64;               // LDS row front padding (columns)
+constexpr int kRow = kStrip + kPad;    // LDS row length (columns)
+constexpr int kSlice = kRow / kW;      // columns per interleave slice
 
 template <typename T>
 __host__ __device__ __forceinline__ T neg_zero() {
     return -T(0);
 }
 
-__host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
-    return v < lo ? lo : (v > hi ? hi : v);
+// An LDS row holds strip columns c in [-kPad, kStrip) at position
+// (c' & 3) * kSlice + (c' >> 2), c' = c + kPad: interleaved by residue so
+// that column 4*lane + u, read by every lane of a wave for one wave-uniform
+// u, is bank-consecutive.  Its address is lane + lds_uniform(u).
+__host__ __device__ __forceinline__ int lds_uniform(int u) {
+    const int cp = u + kPad;
+    return (cp & (kW - 1)) * kSlice + (cp >> 2);
 }
 
-// LDS row of kStrip columns, stored interleaved so that the column
-// 4*lane + i read by every lane of a wave for fixed i is bank-consecutive.
-__host__ __device__ __forceinline__ int lds_col(int c) { return (c & (kW - 1)) * kLanes + (c >> 2); }
+// Wave-uniform value in a scalar register (identity on the host).
+__host__ __device__ __forceinline__ int wave_uniform(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
+}
+
+// Work-groups are dispatched to the 8 XCDs round-robin; consecutive logical
+// items (which share input rows) are remapped onto one XCD so they share its
+// L2.  grid = 8 * ceil(items / 8); returns -1 for the padding blocks.
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int xcd_remap(int block, int items) {
+    const int per = (items + kXcds - 1) / kXcds;
+    const int logical = (block % kXcds) * per + block / kXcds;
+    return logical < items ? logical : -1;
+}
+__host__ __device__ __forceinline__ unsigned xcd_grid(int items) {
+    return unsigned((items + kXcds - 1) / kXcds * kXcds);
+}
 
 // Quadrant view of the zero-padded init buffer (adrt_cdefs_adrt.hpp:123-186):
 //   q0: img[r][n-1-c]  q1: img[n-1-c][r]  q2: img[c][r]  q3: img[n-1-r][n-1-c]
@@ -73,10 +99,37 @@ __host__ __device__ __forceinline__ T quad_at(const T *__restrict__ img, int q, 
     }
 }
 
+// 16-byte (fp32) / 32-byte (fp64) vector moves through a native vector type
+// so the temporaries stay in registers; p needs only element alignment.
+#ifndef ADRT_HOST_EMULATION
 template <typename T>
-struct Vec4 {
-    T v[4];
+using vec4_t = T __attribute__((ext_vector_type(4)));
+#else
+template <typename T>
+struct vec4_t {
+    T x, y, z, w;
 };
+#endif
+
+template <typename T>
+__host__ __device__ __forceinline__ void load_vec4(const T *p, T &a, T &b, T &c, T &d) {
+    vec4_t<T> t;
+    __builtin_memcpy(&t, p, sizeof(t));
+    a = t.x;
+    b = t.y;
+    c = t.z;
+    d = t.w;
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void store_vec4(T *p, T a, T b, T c, T d) {
+    vec4_t<T> t;
+    t.x = a;
+    t.y = b;
+    t.z = c;
+    t.w = d;
+    __builtin_memcpy(p, &t, sizeof(t));
+}
 
 // Four consecutive elements [c, c+4) of a padded row row[0, lim): -0.0
 // below 0 and +0.0 from lim on.  Interior windows use one 16-byte load.
@@ -84,10 +137,7 @@ template <typename T>
 __host__ __device__ __forceinline__ void load4_padded(const T *__restrict__ row, int c, int lim,
                                                       T (&out)[4]) {
     if (c >= 0 && c + 4 <= lim) {
-        Vec4<T> t;
-        __builtin_memcpy(&t, row + c, sizeof(t));
-#pragma unroll
-        for (int i = 0; i < 4; ++i) out[i] = t.v[i];
+        load_vec4(row + unsigned(c), out[0], out[1], out[2], out[3]);
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -179,8 +229,8 @@ inline void strip_adrt_host(T (*a)[1 << K][kW]) {
 // ---------------------------------------------------------------------------
 // Pass 1: rows row0 .. row0 + 2^k1 of quadrant q, X offsets [c0, c0 + kOut).
 //   phase 0 (level A): strip g (2^KB of them): rows row0 + 8g + j, base
-//     c0 - HB - 7g - 7 -> LA[(g, tA)] (kStrip columns from that base)
-//   phase 1 (level B): strip mA (8 of them): rows LA[(g, mA)] skewed by
+//     c0 - HB - 7g - 7 -> LDS rows (g, tA)
+//   phase 1 (level B): strip mA (8 of them): LDS rows (g, mA) skewed by
 //     mA*g, base c0 - kLead -> X[J][mA*2^KB + tB][c] for c in [c0, c0+kOut)
 // With KB == 0 phase 0 writes X directly and phase 1 is empty.
 // ---------------------------------------------------------------------------
@@ -193,7 +243,7 @@ struct Pass1 {
     static constexpr int R0 = RA, R1 = RB;
     static constexpr int tasks0 = RB;
     static constexpr int tasks1 = KB == 0 ? 0 : RA;
-    static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) * kStrip;
+    static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) * kRow;
 
     struct Item {
         const T *img;
@@ -202,16 +252,22 @@ struct Pass1 {
         bool valid;
     };
 
-    __host__ __device__ static int num_chunks(int n) {
+    __host__ __device__ __forceinline__ static int num_chunks(int n) {
         return (n + (1 << k1) - 1 + kOut - 1) / kOut;
     }
+    __host__ __device__ __forceinline__ static int num_items(int n, int nimg) {
+        return nimg * 4 * (n >> k1) * num_chunks(n);
+    }
 
-    __host__ __device__ static Item item(const T *img_base, T *X_base, int n, int first_image,
-                                         int nimg, int block) {
+    __host__ __device__ __forceinline__ static Item item(const T *img_base, T *X_base, int n,
+                                                         int first_image, int nimg, int block) {
         Item it;
         it.n = n;
         it.W1 = n + (1 << k1) - 1;  // valid offsets of X
         it.P1 = n + (1 << k1);      // row pitch of X (multiple of 4)
+        const int logical = xcd_remap(block, num_items(n, nimg));
+        it.valid = logical >= 0;
+        block = it.valid ? logical : 0;
         const int nJ = n >> k1;
         const int nchunk = num_chunks(n);
         const int chunk = block % nchunk;
@@ -220,7 +276,6 @@ struct Pass1 {
         block /= nJ;
         it.q = block & 3;
         const int b = first_image + (block >> 2);
-        it.valid = b < first_image + nimg;
         it.c0 = chunk * kOut;
         it.img = img_base + (size_t)b * n * n;
         it.X = X_base + ((size_t)(b - first_image) * 4 + it.q) * nJ * ((size_t)1 << k1) * it.P1 +
@@ -229,12 +284,13 @@ struct Pass1 {
         return it;
     }
 
-    __host__ __device__ static int base0(const Item &it, int g) {
+    __host__ __device__ __forceinline__ static int base0(const Item &it, int g) {
         return KB == 0 ? it.c0 - kLead : it.c0 - HB - (RA - 1) * g - (RA - 1);
     }
 
     // Level A input: 8 quadrant rows at offsets base0 + 4*lane + i.
-    __host__ __device__ static void load0(const Item &it, int g, int lane, T (&v)[RA][kW]) {
+    __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
+                                                          T (&v)[RA][kW]) {
         const int c = base0(it, g) + kW * lane;
         const int r0 = it.row0 + g * RA;
         const int n = it.n;
@@ -249,14 +305,9 @@ struct Pass1 {
                     for (int j = 0; j < RA; ++j) v[j][i] = pad;
                 } else {
                     const int irow = it.q == 1 ? n - 1 - cc : cc;
-                    Vec4<T> lo, hi;
-                    __builtin_memcpy(&lo, it.img + (size_t)irow * n + r0, sizeof(lo));
-                    __builtin_memcpy(&hi, it.img + (size_t)irow * n + r0 + 4, sizeof(hi));
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        v[j][i] = lo.v[j];
-                        v[j + 4][i] = hi.v[j];
-                    }
+                    const T *src = it.img + unsigned(irow * n + r0);
+                    load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
+                    load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
                 }
             }
         } else {
@@ -267,10 +318,7 @@ struct Pass1 {
                 const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
                 const T *row = it.img + (size_t)irow * n;
                 if (c >= 0 && c + kW <= n) {
-                    Vec4<T> t;
-                    __builtin_memcpy(&t, row + (n - kW - c), sizeof(t));
-#pragma unroll
-                    for (int i = 0; i < kW; ++i) v[j][i] = t.v[kW - 1 - i];
+                    load_vec4(row + unsigned(n - kW - c), v[j][3], v[j][2], v[j][1], v[j][0]);
                 } else {
 #pragma unroll
                     for (int i = 0; i < kW; ++i) v[j][i] = quad_at(it.img, it.q, n, r0 + j, c + i);
@@ -282,20 +330,18 @@ struct Pass1 {
     // X rows mA*R + t at offsets c0 - kLead + 4*lane + i; lanes 16..63 deliver
     // [c0, c0 + kOut).  A window may run past W1 up to P1 - 1: harmless padding.
     template <int R>
-    __host__ __device__ static void store_x(const Item &it, int mA, int lane, const T (&o)[R][kW]) {
+    __host__ __device__ __forceinline__ static void store_x(const Item &it, int mA, int lane,
+                                                            const T (&o)[R][kW]) {
         const int c = it.c0 - kLead + kW * lane;
         if (lane < kLead / kW || c >= it.W1) return;
 #pragma unroll
-        for (int t = 0; t < R; ++t) {
-            Vec4<T> v;
-#pragma unroll
-            for (int i = 0; i < kW; ++i) v.v[i] = o[t][i];
-            __builtin_memcpy(it.X + (size_t)(mA * R + t) * it.P1 + c, &v, sizeof(v));
-        }
+        for (int t = 0; t < R; ++t)
+            store_vec4(it.X + (size_t)(mA * R + t) * it.P1 + unsigned(c), o[t][0], o[t][1], o[t][2],
+                       o[t][3]);
     }
 
-    __host__ __device__ static void store0(const Item &it, int g, int lane, const T (&o)[RA][kW],
-                                           T *lds) {
+    __host__ __device__ __forceinline__ static void store0(const Item &it, int g, int lane,
+                                                           const T (&o)[RA][kW], T *lds) {
         if constexpr (KB == 0) {
             (void)lds;
             store_x<RA>(it, g, lane, o);
@@ -303,37 +349,44 @@ struct Pass1 {
 #pragma unroll
             for (int t = 0; t < RA; ++t)
 #pragma unroll
-                for (int i = 0; i < kW; ++i)
-                    lds[(g * RA + t) * kStrip + lds_col(kW * lane + i)] = o[t][i];
+                for (int i = 0; i < kW; ++i) lds[(g * RA + t) * kRow + lds_uniform(i) + lane] = o[t][i];
         }
     }
 
-    // Level B input: rows g of LA at angle mA, skewed by mA*g.
-    __host__ __device__ static void load1(const Item &it, int mA, int lane, const T *lds,
-                                          T (&v)[RB][kW]) {
+    // Level B input: LDS rows (g, mA), skewed by mA*g.  Strip column of
+    // offset c0 - kLead + 4*lane + i - mA*g relative to base0(g):
+    // 4*lane + i - kLead + HB + 7 + (7 - mA)*g, always in [-kPad, kStrip).
+    __host__ __device__ __forceinline__ static void load1(const Item &it, int mA, int lane,
+                                                          const T *lds, T (&v)[RB][kW]) {
         (void)it;
 #pragma unroll
         for (int g = 0; g < RB; ++g) {
-            // strip column of offset (c0 - kLead + 4*lane + i - mA*g) - base0(g)
-            const int cb = kW * lane - kLead + HB + (RA - 1) + (RA - 1 - mA) * g;
-            const T *row = lds + (g * RA + mA) * kStrip;
+            const int ub = -kLead + HB + (RA - 1) + (RA - 1 - mA) * g;
+            const T *row = lds + (g * RA + mA) * kRow + lane;
 #pragma unroll
-            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_col(clampi(cb + i, 0, kStrip - 1))];
+            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_uniform(ub + i)];
         }
     }
 
-    __host__ __device__ static void store1(const Item &it, int mA, int lane, const T (&o)[RB][kW]) {
+    __host__ __device__ __forceinline__ static void store1(const Item &it, int mA, int lane,
+                                                           const T (&o)[RB][kW], T *lds) {
+        (void)lds;
         store_x<RB>(it, mA, lane, o);
     }
+
+    __host__ __device__ __forceinline__ static void phase2(const Item &, const T *, int) {}
 };
 
 // ---------------------------------------------------------------------------
 // Pass 2: angle s of quadrant q, output offsets [d0, d0 + kOut).
 //   phase 0 (level C): strip g (2^KD of them): blocks J = 8g + j of X at
-//     angle s, row j skewed by s*j, base d0 - HD - (8s+7)g - 7 -> LC[(g, tC)]
-//   phase 1 (level D): strip tC (8 of them): rows LC[(g, tC)] skewed by
-//     (8s + tC)*g, base d0 - kLead -> out[d][s*2^k2 + tC*2^KD + tD]
-// With KD == 0 phase 0 writes the output directly.
+//     angle s, row j skewed by s*j, base d0 - HD - (8s+7)g - 7 -> LDS rows (g, tC)
+//   phase 1 (level D): strip tC (8 of them): LDS rows (g, tC) skewed by
+//     (8s + tC)*g, base d0 - kLead -> output tile [d][tC*2^KD + tD], written
+//     over the level-C rows after every strip has loaded
+//   phase 2: the tile (kOut rows of 2^k2 angles, swizzled by 16-byte group)
+//     is stored row by row: out[d0 + d][s*2^k2 + a], 2^k2 contiguous values.
+// With KD == 0 phase 0 writes the tile directly.
 // ---------------------------------------------------------------------------
 template <typename T, int KD>
 struct Pass2 {
@@ -342,9 +395,14 @@ struct Pass2 {
     static constexpr int RC = 1 << KC, RD = 1 << KD;
     static constexpr int HD = RD - 1;
     static constexpr int R0 = RC, R1 = RD;
+    static constexpr int A2 = 1 << k2;   // angles per tile row
+    static constexpr int G = A2 / 4;     // 4-element groups per tile row
     static constexpr int tasks0 = RD;
     static constexpr int tasks1 = KD == 0 ? 0 : RC;
-    static constexpr size_t lds_elems = KD == 0 ? 0 : size_t(1 << k2) * kStrip;
+    static constexpr size_t lc_elems = KD == 0 ? 0 : size_t(1 << k2) * kRow;
+    static constexpr size_t tile_elems = size_t(kOut) * A2;
+    // The tile reuses the level-C rows once every level-D strip has loaded.
+    static constexpr size_t lds_elems = lc_elems > tile_elems ? lc_elems : tile_elems;
 
     struct Item {
         const T *X;  // row (J, s) at X + J * xrow_stride
@@ -354,16 +412,24 @@ struct Pass2 {
         bool valid;
     };
 
-    __host__ __device__ static int num_chunks(int n) { return (2 * n - 1 + kOut - 1) / kOut; }
+    __host__ __device__ __forceinline__ static int num_chunks(int n) {
+        return (2 * n - 1 + kOut - 1) / kOut;
+    }
+    __host__ __device__ __forceinline__ static int num_items(int n, int k1, int nimg) {
+        return nimg * 4 * (1 << k1) * num_chunks(n);
+    }
 
-    __host__ __device__ static Item item(const T *X_base, T *out_base, int n, int k1,
-                                         int first_image, int nimg, int block) {
+    __host__ __device__ __forceinline__ static Item item(const T *X_base, T *out_base, int n, int k1,
+                                                         int first_image, int nimg, int block) {
         Item it;
         it.n = n;
         it.C = 2 * n - 1;
         it.W1 = n + (1 << k1) - 1;
         const int P1 = n + (1 << k1);
         const int nS = 1 << k1;
+        const int logical = xcd_remap(block, num_items(n, k1, nimg));
+        it.valid = logical >= 0;
+        block = it.valid ? logical : 0;
         const int nchunk = num_chunks(n);
         const int chunk = block % nchunk;
         block /= nchunk;
@@ -371,7 +437,6 @@ struct Pass2 {
         block /= nS;
         const int q = block & 3;
         const int b = first_image + (block >> 2);
-        it.valid = b < first_image + nimg;
         it.d0 = chunk * kOut;
         it.xrow_stride = (size_t)nS * P1;  // between blocks J at fixed s
         it.X = X_base + ((size_t)(b - first_image) * 4 + q) * ((size_t)1 << k2) * it.xrow_stride +
@@ -381,97 +446,123 @@ struct Pass2 {
         return it;
     }
 
-    __host__ __device__ static int base0(const Item &it, int g) {
+    __host__ __device__ __forceinline__ static int base0(const Item &it, int g) {
         return KD == 0 ? it.d0 - kLead : it.d0 - HD - (RC * it.s + RC - 1) * g - (RC - 1);
     }
 
+    // Swizzled tile position of (row d, angle a): 16-byte group a/4 is XORed
+    // with (d/4) mod G so that lanes writing consecutive rows hit distinct banks.
+    __host__ __device__ __forceinline__ static int tile_pos(int d, int a) {
+        return d * A2 + 4 * ((a >> 2) ^ ((d >> 2) & (G - 1))) + (a & 3);
+    }
+
     // Level C input: blocks 8g + j at angle s, row j skewed by s*j.
-    __host__ __device__ static void load0(const Item &it, int g, int lane, T (&v)[RC][kW]) {
+    __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
+                                                          T (&v)[RC][kW]) {
         const int c = base0(it, g) + kW * lane;
 #pragma unroll
         for (int j = 0; j < RC; ++j)
             load4_padded(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
     }
 
-    // Output rows d = d0 - kLead + 4*lane + i (lanes 16..63), angles
-    // abase + tC*R + t.
+    // Tile rows d = 4*lane + i - kLead (lanes 16..63), angles a0 + t.
     template <int R>
-    __host__ __device__ static void store_out(const Item &it, int tC, int lane, const T (&o)[R][kW]) {
-        const int d = it.d0 - kLead + kW * lane;
+    __host__ __device__ __forceinline__ static void store_tile(int a0, int lane, const T (&o)[R][kW],
+                                                               T *tile) {
         if (lane < kLead / kW) return;
 #pragma unroll
         for (int i = 0; i < kW; ++i) {
-            if (d + i >= it.C) break;
-            T *orow = it.out + (size_t)(d + i) * it.n + it.abase + tC * R;
+            const int d = kW * lane + i - kLead;
             if constexpr (R % 4 == 0) {
 #pragma unroll
                 for (int t = 0; t < R; t += 4) {
-                    Vec4<T> v;
+                    T *p = tile + tile_pos(d, a0 + t);  // 4-element group, 16-byte aligned
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) v.v[k] = o[t + k][i];
-                    __builtin_memcpy(orow + t, &v, sizeof(v));
+                    for (int k = 0; k < 4; ++k) p[k] = o[t + k][i];
                 }
             } else {
 #pragma unroll
-                for (int t = 0; t < R; ++t) orow[t] = o[t][i];
+                for (int t = 0; t < R; ++t) tile[tile_pos(d, a0 + t)] = o[t][i];
             }
         }
     }
 
-    __host__ __device__ static void store0(const Item &it, int g, int lane, const T (&o)[RC][kW],
-                                           T *lds) {
+    __host__ __device__ __forceinline__ static void store0(const Item &it, int g, int lane,
+                                                           const T (&o)[RC][kW], T *lds) {
+        (void)it;
         if constexpr (KD == 0) {
-            (void)lds;
-            store_out<RC>(it, 0, lane, o);
+            store_tile<RC>(0, lane, o, lds);
         } else {
 #pragma unroll
             for (int t = 0; t < RC; ++t)
 #pragma unroll
-                for (int i = 0; i < kW; ++i)
-                    lds[(g * RC + t) * kStrip + lds_col(kW * lane + i)] = o[t][i];
+                for (int i = 0; i < kW; ++i) lds[(g * RC + t) * kRow + lds_uniform(i) + lane] = o[t][i];
         }
     }
 
-    // Level D input: rows g of LC at angle tC, skewed by (8s + tC)*g.
-    __host__ __device__ static void load1(const Item &it, int tC, int lane, const T *lds,
-                                          T (&v)[RD][kW]) {
+    // Level D input: LDS rows (g, tC), skewed by (8s + tC)*g.  Strip column of
+    // offset d0 - kLead + 4*lane + i - (8s + tC)*g relative to base0(g):
+    // 4*lane + i - kLead + HD + 7 + (7 - tC)*g, always in [-kPad, kStrip).
+    __host__ __device__ __forceinline__ static void load1(const Item &it, int tC, int lane,
+                                                          const T *lds, T (&v)[RD][kW]) {
         (void)it;
 #pragma unroll
         for (int g = 0; g < RD; ++g) {
-            // strip column of offset (d0 - kLead + 4*lane + i - (8s+tC)*g) - base0(g)
-            const int cb = kW * lane - kLead + HD + (RC - 1) + (RC - 1 - tC) * g;
-            const T *row = lds + (g * RC + tC) * kStrip;
+            const int ub = -kLead + HD + (RC - 1) + (RC - 1 - tC) * g;
+            const T *row = lds + (g * RC + tC) * kRow + lane;
 #pragma unroll
-            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_col(clampi(cb + i, 0, kStrip - 1))];
+            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_uniform(ub + i)];
         }
     }
 
-    __host__ __device__ static void store1(const Item &it, int tC, int lane, const T (&o)[RD][kW]) {
-        store_out<RD>(it, tC, lane, o);
+    __host__ __device__ __forceinline__ static void store1(const Item &it, int tC, int lane,
+                                                           const T (&o)[RD][kW], T *lds) {
+        (void)it;
+        store_tile<RD>(tC * RD, lane, o, lds);
+    }
+
+    // Coalesced store of the tile: thread tid handles (row, group) pairs.
+    __host__ __device__ __forceinline__ static void phase2(const Item &it, const T *lds, int tid) {
+        const T *tile = lds;
+        for (int f = tid; f < kOut * G; f += kThreads) {
+            const int d = f / G, grp = f % G;
+            const int od = it.d0 + d;
+            if (od >= it.C) continue;
+            const T *p = tile + tile_pos(d, 4 * grp);
+            store_vec4(it.out + (size_t)od * it.n + it.abase + 4 * grp, p[0], p[1], p[2], p[3]);
+        }
     }
 };
 
 #ifndef ADRT_HOST_EMULATION
 constexpr int log2_small(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
 
+// Phase 0 strips (tasks0 <= kWaves, one per wave), barrier, phase 1 strips
+// (exactly one per wave) loaded and computed in registers, barrier (every
+// read of phase-0 data done), phase 1 stores, barrier, phase 2 cooperative
+// stores.
 template <typename P, typename T>
 __device__ __forceinline__ void run_block(const typename P::Item &it, T *lds) {
-    const int wave = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
-    for (int task = wave; task < P::tasks0; task += kWaves) {
+    static_assert(P::tasks0 <= kWaves, "at most one phase-0 strip per wave");
+    static_assert(P::tasks1 == 0 || P::tasks1 == kWaves, "one phase-1 strip per wave");
+    const int wave = wave_uniform(threadIdx.x / kLanes);
+    const int lane = threadIdx.x % kLanes;
+    if (wave < P::tasks0) {
         T v[P::R0][kW];
-        P::load0(it, task, lane, v);
+        P::load0(it, wave, lane, v);
         strip_adrt<log2_small(P::R0), T>(v);
-        P::store0(it, task, lane, v, lds);
+        P::store0(it, wave, lane, v, lds);
     }
+    __syncthreads();
     if constexpr (P::tasks1 > 0) {
+        T v[P::R1][kW];
+        P::load1(it, wave, lane, lds, v);
+        strip_adrt<log2_small(P::R1), T>(v);
         __syncthreads();
-        for (int task = wave; task < P::tasks1; task += kWaves) {
-            T v[P::R1][kW];
-            P::load1(it, task, lane, lds, v);
-            strip_adrt<log2_small(P::R1), T>(v);
-            P::store1(it, task, lane, v);
-        }
+        P::store1(it, wave, lane, v, lds);
+        __syncthreads();
     }
+    P::phase2(it, lds, threadIdx.x);
 }
 
 template <typename T, int KB>
@@ -520,7 +611,7 @@ inline size_t x_bytes_per_image(int n, int L, size_t elem) {
 template <typename T, int KB>
 static void launch_pass1(const T *img, T *X, int n, int first, int count, hipStream_t s) {
     using P = Pass1<T, KB>;
-    const unsigned grid = unsigned(count) * 4u * unsigned(n >> P::k1) * unsigned(P::num_chunks(n));
+    const unsigned grid = xcd_grid(P::num_items(n, count));
     constexpr size_t lds = P::lds_elems * sizeof(T);
     auto kern = fwd_pass1<T, KB>;
     static bool attr_set = false;
@@ -536,7 +627,7 @@ template <typename T, int KD>
 static void launch_pass2(const T *X, T *out, int n, int k1, int first, int count,
                          hipStream_t s) {
     using P = Pass2<T, KD>;
-    const unsigned grid = unsigned(count) * 4u * unsigned(1 << k1) * unsigned(P::num_chunks(n));
+    const unsigned grid = xcd_grid(P::num_items(n, k1, count));
     constexpr size_t lds = P::lds_elems * sizeof(T);
     auto kern = fwd_pass2<T, KD>;
     static bool attr_set = false;

@@ -39,11 +39,19 @@ static void emulate_grid(unsigned grid, Args... args) {
             strip_adrt_host<log2i(P::R0), T>(v0);
             for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
         }
+        // Phase 1: every strip loads and computes before any stores (the
+        // stores may overwrite the LDS rows the loads read).
+        std::vector<T> keep(size_t(P::tasks1 > 0 ? P::tasks1 : 1) * buf1.size());
         for (int task = 0; task < P::tasks1; ++task) {
             for (int lane = 0; lane < kLanes; ++lane) P::load1(it, task, lane, l, v1[lane]);
             strip_adrt_host<log2i(P::R1), T>(v1);
-            for (int lane = 0; lane < kLanes; ++lane) P::store1(it, task, lane, v1[lane]);
+            std::memcpy(keep.data() + size_t(task) * buf1.size(), buf1.data(), buf1.size() * sizeof(T));
         }
+        for (int task = 0; task < P::tasks1; ++task) {
+            std::memcpy(buf1.data(), keep.data() + size_t(task) * buf1.size(), buf1.size() * sizeof(T));
+            for (int lane = 0; lane < kLanes; ++lane) P::store1(it, task, lane, v1[lane], l);
+        }
+        for (int tid = 0; tid < kThreads; ++tid) P::phase2(it, l, tid);
     }
 }
 
@@ -51,9 +59,9 @@ template <typename T, int KB, int KD>
 static void emulate_group(const T *img, T *X, T *out, int n, int first, int count) {
     using P1 = Pass1<T, KB>;
     using P2 = Pass2<T, KD>;
-    const unsigned g1 = unsigned(count) * 4u * unsigned(n >> P1::k1) * unsigned(P1::num_chunks(n));
+    const unsigned g1 = xcd_grid(P1::num_items(n, count));
     emulate_grid<P1, T>(g1, img, X, n, first, count);
-    const unsigned g2 = unsigned(count) * 4u * unsigned(1 << P1::k1) * unsigned(P2::num_chunks(n));
+    const unsigned g2 = xcd_grid(P2::num_items(n, P1::k1, count));
     emulate_grid<P2, T>(g2, (const T *)X, out, n, P1::k1, first, count);
 }
 
