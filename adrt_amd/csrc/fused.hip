@@ -71,33 +71,31 @@ __host__ __device__ __forceinline__ int wave_uniform(int v) {
 #endif
 }
 
-// Work-groups are dispatched to the 8 XCDs round-robin; consecutive logical
-// items (which share input rows) are remapped onto one XCD so they share its
-// L2.  grid = 8 * ceil(items / 8); returns -1 for the padding blocks.
+// Work-group -> item.  Work-groups are dispatched to the 8 XCDs
+// round-robin, so work-group w runs on XCD w % 8.  Each XCD owns a
+// contiguous eighth of the items, so the work-groups resident on one XCD at
+// a time work on neighbouring items, which share input rows in its L2.  The
+// mapping only affects speed: with grid = 8 * ceil(items / 8) every item is
+// visited exactly once (and a smaller multiple of 8 makes each work-group
+// step through several).
 constexpr int kXcds = 8;
-__host__ __device__ __forceinline__ int xcd_remap(int block, int items) {
+struct Schedule {
+    int idx, hi, step;
+};
+__host__ __device__ __forceinline__ Schedule schedule(int block, int grid, int items) {
     const int per = (items + kXcds - 1) / kXcds;
-    const int logical = (block % kXcds) * per + block / kXcds;
-    return logical < items ? logical : -1;
-}
-__host__ __device__ __forceinline__ unsigned xcd_grid(int items) {
-    return unsigned((items + kXcds - 1) / kXcds * kXcds);
+    const int lo = (block % kXcds) * per;
+    Schedule sc;
+    sc.idx = lo + block / kXcds;
+    sc.hi = items < lo + per ? items : lo + per;
+    sc.step = grid / kXcds;
+    return sc;
 }
 
-// Quadrant view of the zero-padded init buffer (adrt_cdefs_adrt.hpp:123-186):
+// Quadrant views of the zero-padded init buffer (adrt_cdefs_adrt.hpp:123-186),
+// row r and offset c of quadrant q:
 //   q0: img[r][n-1-c]  q1: img[n-1-c][r]  q2: img[c][r]  q3: img[n-1-r][n-1-c]
-// c < 0 -> -0.0 (reproduces the reference's copies), c >= n -> +0.0.
-template <typename T>
-__host__ __device__ __forceinline__ T quad_at(const T *__restrict__ img, int q, int n, int r, int c) {
-    if (c < 0) return neg_zero<T>();
-    if (c >= n) return T(0);
-    switch (q) {
-        case 0: return img[(size_t)r * n + (n - 1 - c)];
-        case 1: return img[(size_t)(n - 1 - c) * n + r];
-        case 2: return img[(size_t)c * n + r];
-        default: return img[(size_t)(n - 1 - r) * n + (n - 1 - c)];
-    }
-}
+// c < 0 reads -0.0 (reproduces the reference's copies), c >= n reads +0.0.
 
 // 16-byte (fp32) / 32-byte (fp64) vector moves through a native vector type
 // so the temporaries stay in registers; p needs only element alignment.
@@ -131,18 +129,69 @@ __host__ __device__ __forceinline__ void store_vec4(T *p, T a, T b, T c, T d) {
     __builtin_memcpy(p, &t, sizeof(t));
 }
 
-// Four consecutive elements [c, c+4) of a padded row row[0, lim): -0.0
-// below 0 and +0.0 from lim on.  Interior windows use one 16-byte load.
+// Streaming store for data no kernel of this call reads again (the final
+// output): it is not kept in L2, which leaves L2 to the intermediate.
 template <typename T>
-__host__ __device__ __forceinline__ void load4_padded(const T *__restrict__ row, int c, int lim,
-                                                      T (&out)[4]) {
-    if (c >= 0 && c + 4 <= lim) {
-        load_vec4(row + unsigned(c), out[0], out[1], out[2], out[3]);
-    } else {
+__host__ __device__ __forceinline__ void store_vec4_stream(T *p, T a, T b, T c, T d) {
+#ifndef ADRT_HOST_EMULATION
+    vec4_t<T> t;
+    t.x = a;
+    t.y = b;
+    t.z = c;
+    t.w = d;
+    __builtin_nontemporal_store(t, reinterpret_cast<vec4_t<T> *>(p));
+#else
+    store_vec4(p, a, b, c, d);
+#endif
+}
+
+// Padded windows.  A strip row needs four consecutive offsets of a row that
+// has `lim` (>= 4) valid elements and reads -0.0 below offset 0 and +0.0 from
+// `lim` on.  Every window is read with one unconditional 16-byte load from a
+// start clamped into the row; windows that cross an edge are then patched in
+// a separate step.  Keeping the loads free of divergent branches lets all
+// rows of a strip be in flight at once (a branch around a load would make
+// the compiler wait for it before the paths join).
+__host__ __device__ __forceinline__ int clamp_start(int c, int lim) {
+    return c < 0 ? 0 : (c > lim - 4 ? lim - 4 : c);
+}
+
+// Window [c, c+4) of `row`: one load from the clamped start; windows that
+// cross an edge are completed by patch_window once every row is loaded.
+template <typename T>
+__host__ __device__ __forceinline__ void load_window(const T *__restrict__ row, int c, int lim,
+                                                     T (&out)[4]) {
+    load_vec4(row + unsigned(clamp_start(c, lim)), out[0], out[1], out[2], out[3]);
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void patch_window(const T *__restrict__ row, int c, int lim,
+                                                      T (&v)[4]) {
+    if (c < 0 || c > lim - 4) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int cc = c + i;
-            out[i] = cc < 0 ? neg_zero<T>() : (cc >= lim ? T(0) : row[cc]);
+            const int k = c + i;
+            v[i] = k < 0 ? neg_zero<T>() : (k >= lim ? T(0) : row[k]);
+        }
+    }
+}
+
+// Reversed window: out[i] = row[lim - 1 - (c + i)], i.e. offsets c..c+3 of a
+// row read backwards; offsets below 0 read -0.0, offsets from lim on +0.0.
+template <typename T>
+__host__ __device__ __forceinline__ void load_window_rev(const T *__restrict__ row, int c, int lim,
+                                                         T (&out)[4]) {
+    load_vec4(row + unsigned(clamp_start(lim - 4 - c, lim)), out[3], out[2], out[1], out[0]);
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void patch_window_rev(const T *__restrict__ row, int c,
+                                                          int lim, T (&v)[4]) {
+    if (c < 0 || c > lim - 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = c + i;
+            v[i] = k < 0 ? neg_zero<T>() : (k >= lim ? T(0) : row[lim - 1 - k]);
         }
     }
 }
@@ -249,7 +298,6 @@ struct Pass1 {
         const T *img;
         T *X;  // rows of block J: X + s * P1
         int q, n, c0, row0, W1, P1;
-        bool valid;
     };
 
     __host__ __device__ __forceinline__ static int num_chunks(int n) {
@@ -259,23 +307,37 @@ struct Pass1 {
         return nimg * 4 * (n >> k1) * num_chunks(n);
     }
 
+    // Item `block` in [0, num_items) -> (image, quadrant, row block J, chunk),
+    // ordered so that the two quadrants reading the same image data are
+    // neighbours (and so run at the same time in one XCD's L2):
+    //   block = (((b * 2 + pair) * nJ + J') * nchunk + chunk') * 2 + m
+    //   pair 0: m = 0 -> q0, block J', chunk'; m = 1 -> q3, block nJ-1-J'
+    //           (the same image rows, reversed), chunk'
+    //   pair 1: m = 0 -> q2, block J', chunk'; m = 1 -> q1, block J', chunk
+    //           nchunk-1-chunk' (its offsets run over the image rows backwards)
     __host__ __device__ __forceinline__ static Item item(const T *img_base, T *X_base, int n,
-                                                         int first_image, int nimg, int block) {
+                                                         int first_image, int block) {
         Item it;
         it.n = n;
         it.W1 = n + (1 << k1) - 1;  // valid offsets of X
         it.P1 = n + (1 << k1);      // row pitch of X (multiple of 4)
-        const int logical = xcd_remap(block, num_items(n, nimg));
-        it.valid = logical >= 0;
-        block = it.valid ? logical : 0;
         const int nJ = n >> k1;
         const int nchunk = num_chunks(n);
-        const int chunk = block % nchunk;
+        const int m = block & 1;
+        block >>= 1;
+        int chunk = block % nchunk;
         block /= nchunk;
-        const int J = block % nJ;
+        int J = block % nJ;
         block /= nJ;
-        it.q = block & 3;
-        const int b = first_image + (block >> 2);
+        const int pair = block & 1;
+        const int b = first_image + (block >> 1);
+        if (pair == 0) {
+            it.q = m == 0 ? 0 : 3;
+            if (m == 1) J = nJ - 1 - J;
+        } else {
+            it.q = m == 0 ? 2 : 1;
+            if (m == 1) chunk = nchunk - 1 - chunk;
+        }
         it.c0 = chunk * kOut;
         it.img = img_base + (size_t)b * n * n;
         it.X = X_base + ((size_t)(b - first_image) * 4 + it.q) * nJ * ((size_t)1 << k1) * it.P1 +
@@ -288,14 +350,25 @@ struct Pass1 {
         return KB == 0 ? it.c0 - kLead : it.c0 - HB - (RA - 1) * g - (RA - 1);
     }
 
-    // Level A input: 8 quadrant rows at offsets base0 + 4*lane + i.
+    // Level A input: 8 quadrant rows at offsets base0 + 4*lane + i.  All
+    // loads are issued before any edge patching.
     __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
                                                           T (&v)[RA][kW]) {
         const int c = base0(it, g) + kW * lane;
         const int r0 = it.row0 + g * RA;
         const int n = it.n;
         if (it.q == 1 || it.q == 2) {
-            // Rows are image columns r0..r0+7, offsets index image rows.
+            // Rows are image columns r0..r0+7; offset c+i is image row
+            // n-1-(c+i) (q1) or c+i (q2), clamped into the image for the load.
+#pragma unroll
+            for (int i = 0; i < kW; ++i) {
+                const int cc = c + i;
+                int irow = it.q == 1 ? n - 1 - cc : cc;
+                irow = irow < 0 ? 0 : (irow > n - 1 ? n - 1 : irow);
+                const T *src = it.img + unsigned(irow * n + r0);
+                load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
+                load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
+            }
 #pragma unroll
             for (int i = 0; i < kW; ++i) {
                 const int cc = c + i;
@@ -303,26 +376,20 @@ struct Pass1 {
                     const T pad = cc < 0 ? neg_zero<T>() : T(0);
 #pragma unroll
                     for (int j = 0; j < RA; ++j) v[j][i] = pad;
-                } else {
-                    const int irow = it.q == 1 ? n - 1 - cc : cc;
-                    const T *src = it.img + unsigned(irow * n + r0);
-                    load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
-                    load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
                 }
             }
         } else {
             // Rows are image rows (q0: r, q3: n-1-r); offsets c..c+3 are the
-            // columns n-1-c-3 .. n-1-c in reverse order.
+            // columns n-1-c .. n-4-c.
 #pragma unroll
             for (int j = 0; j < RA; ++j) {
                 const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
-                const T *row = it.img + (size_t)irow * n;
-                if (c >= 0 && c + kW <= n) {
-                    load_vec4(row + unsigned(n - kW - c), v[j][3], v[j][2], v[j][1], v[j][0]);
-                } else {
+                load_window_rev(it.img + (size_t)irow * n, c, n, v[j]);
+            }
 #pragma unroll
-                    for (int i = 0; i < kW; ++i) v[j][i] = quad_at(it.img, it.q, n, r0 + j, c + i);
-                }
+            for (int j = 0; j < RA; ++j) {
+                const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
+                patch_window_rev(it.img + (size_t)irow * n, c, n, v[j]);
             }
         }
     }
@@ -409,7 +476,6 @@ struct Pass2 {
         T *out;      // plane (b, q)
         int n, s, d0, C, W1, abase;
         size_t xrow_stride;
-        bool valid;
     };
 
     __host__ __device__ __forceinline__ static int num_chunks(int n) {
@@ -419,17 +485,15 @@ struct Pass2 {
         return nimg * 4 * (1 << k1) * num_chunks(n);
     }
 
+    // Item `block` in [0, num_items): (image, quadrant, angle s, chunk).
     __host__ __device__ __forceinline__ static Item item(const T *X_base, T *out_base, int n, int k1,
-                                                         int first_image, int nimg, int block) {
+                                                         int first_image, int block) {
         Item it;
         it.n = n;
         it.C = 2 * n - 1;
         it.W1 = n + (1 << k1) - 1;
         const int P1 = n + (1 << k1);
         const int nS = 1 << k1;
-        const int logical = xcd_remap(block, num_items(n, k1, nimg));
-        it.valid = logical >= 0;
-        block = it.valid ? logical : 0;
         const int nchunk = num_chunks(n);
         const int chunk = block % nchunk;
         block /= nchunk;
@@ -456,13 +520,17 @@ struct Pass2 {
         return d * A2 + 4 * ((a >> 2) ^ ((d >> 2) & (G - 1))) + (a & 3);
     }
 
-    // Level C input: blocks 8g + j at angle s, row j skewed by s*j.
+    // Level C input: blocks 8g + j at angle s, row j skewed by s*j.  All
+    // loads are issued before any edge patching.
     __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
                                                           T (&v)[RC][kW]) {
         const int c = base0(it, g) + kW * lane;
 #pragma unroll
         for (int j = 0; j < RC; ++j)
-            load4_padded(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
+            load_window(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
+#pragma unroll
+        for (int j = 0; j < RC; ++j)
+            patch_window(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
     }
 
     // Tile rows d = 4*lane + i - kLead (lanes 16..63), angles a0 + t.
@@ -529,7 +597,7 @@ struct Pass2 {
             const int od = it.d0 + d;
             if (od >= it.C) continue;
             const T *p = tile + tile_pos(d, 4 * grp);
-            store_vec4(it.out + (size_t)od * it.n + it.abase + 4 * grp, p[0], p[1], p[2], p[3]);
+            store_vec4_stream(it.out + (size_t)od * it.n + it.abase + 4 * grp, p[0], p[1], p[2], p[3]);
         }
     }
 };
@@ -537,14 +605,17 @@ struct Pass2 {
 #ifndef ADRT_HOST_EMULATION
 constexpr int log2_small(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
 
-// Phase 0 strips (tasks0 <= kWaves, one per wave), barrier, phase 1 strips
-// (exactly one per wave) loaded and computed in registers, barrier (every
-// read of phase-0 data done), phase 1 stores, barrier, phase 2 cooperative
-// stores.
-template <typename P, typename T>
-__device__ __forceinline__ void run_block(const typename P::Item &it, T *lds) {
+// One item per work-group: phase 0 strips (tasks0 <= kWaves, one per
+// wave), barrier, phase 1 strips (exactly one per wave) loaded and computed
+// in registers, barrier (every read of phase-0 data done), phase 1 stores,
+// barrier, phase 2 cooperative stores.
+template <typename P, typename T, typename... A>
+__device__ __forceinline__ void run_item(int items, T *lds, A... args) {
     static_assert(P::tasks0 <= kWaves, "at most one phase-0 strip per wave");
     static_assert(P::tasks1 == 0 || P::tasks1 == kWaves, "one phase-1 strip per wave");
+    const Schedule sc = schedule(blockIdx.x, gridDim.x, items);
+    if (sc.idx >= sc.hi) return;
+    const typename P::Item it = P::item(args..., sc.idx);
     const int wave = wave_uniform(threadIdx.x / kLanes);
     const int lane = threadIdx.x % kLanes;
     if (wave < P::tasks0) {
@@ -555,38 +626,45 @@ __device__ __forceinline__ void run_block(const typename P::Item &it, T *lds) {
     }
     __syncthreads();
     if constexpr (P::tasks1 > 0) {
-        T v[P::R1][kW];
-        P::load1(it, wave, lane, lds, v);
-        strip_adrt<log2_small(P::R1), T>(v);
+        T w[P::R1][kW];
+        P::load1(it, wave, lane, lds, w);
+        strip_adrt<log2_small(P::R1), T>(w);
         __syncthreads();
-        P::store1(it, wave, lane, v, lds);
+        P::store1(it, wave, lane, w, lds);
         __syncthreads();
     }
     P::phase2(it, lds, threadIdx.x);
 }
 
+// Register budget: waves per SIMD the kernel's LDS allows (160 KB per CU,
+// at most 4 work-groups per CU), capped by what the strips' registers need
+// without spilling (`cap` waves: 64, 128 or 256 VGPRs).
+template <typename P, typename T>
+constexpr int lds_waves_per_simd(int cap) {
+    constexpr size_t bytes = P::lds_elems * sizeof(T);
+    constexpr int wgs = bytes == 0 ? 4 : int((size_t(160) << 10) / bytes);
+    const int w = (wgs < 4 ? (wgs < 1 ? 1 : wgs) : 4) * kWaves / 4;
+    return w < cap ? w : cap;
+}
+
 template <typename T, int KB>
-__global__ void __launch_bounds__(kThreads)
-    fwd_pass1(const T *__restrict__ img_base, T *__restrict__ X_base, int n, int first_image,
-              int nimg) {
-    using P = Pass1<T, KB>;
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                               amdgpu_waves_per_eu(lds_waves_per_simd<Pass1<T, KB>, T>(sizeof(T) == 4 ? 8 : 4))))
+fwd_pass1(const T *__restrict__ img_base, T *__restrict__ X_base, int n, int first_image,
+              int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-    const typename P::Item it = P::item(img_base, X_base, n, first_image, nimg, blockIdx.x);
-    if (!it.valid) return;
-    run_block<P, T>(it, lds);
+    run_item<Pass1<T, KB>, T>(items, reinterpret_cast<T *>(smem_raw), img_base, X_base, n,
+                               first_image);
 }
 
 template <typename T, int KD>
-__global__ void __launch_bounds__(kThreads)
-    fwd_pass2(const T *__restrict__ X_base, T *__restrict__ out_base, int n, int k1,
-              int first_image, int nimg) {
-    using P = Pass2<T, KD>;
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                               amdgpu_waves_per_eu(lds_waves_per_simd<Pass2<T, KD>, T>(sizeof(T) == 4 ? 8 : 4))))
+fwd_pass2(const T *__restrict__ X_base, T *__restrict__ out_base, int n, int k1,
+              int first_image, int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-    const typename P::Item it = P::item(X_base, out_base, n, k1, first_image, nimg, blockIdx.x);
-    if (!it.valid) return;
-    run_block<P, T>(it, lds);
+    run_item<Pass2<T, KD>, T>(items, reinterpret_cast<T *>(smem_raw), X_base, out_base, n, k1,
+                               first_image);
 }
 #endif  // ADRT_HOST_EMULATION
 
@@ -594,10 +672,12 @@ __global__ void __launch_bounds__(kThreads)
 // pass 2 the remaining k2 = 3 + KD.
 inline int pass1_stages(int L) { return (L + 1) / 2; }
 
-// Images per group: keep one group's intermediate at or below ~64 MiB so it
-// is written and re-read while resident in the 256 MB Infinity Cache.
+// Images per group: keep one group's intermediate at or below ~136 MiB (two
+// 2048^2 fp32 images) so that pass 2 re-reads it largely from the 256 MB
+// Infinity Cache; the final output bypasses that cache (streaming stores).
+// Fewer, larger groups also mean fewer launches.
 inline size_t group_images(size_t B, size_t per_image_bytes) {
-    const size_t budget = size_t(64) << 20;
+    const size_t budget = size_t(136) << 20;
     size_t g = per_image_bytes >= budget ? 1 : budget / per_image_bytes;
     return g < B ? g : B;
 }
@@ -608,35 +688,41 @@ inline size_t x_bytes_per_image(int n, int L, size_t elem) {
 }
 
 #ifndef ADRT_HOST_EMULATION
+// One work-group per item, rounded up to whole rows of the 8 XCDs.  The LDS
+// limit is raised once per device for kernels above the default 64 KB.
+template <typename K>
+static unsigned launch_grid(K kern, size_t lds, int items) {
+    constexpr int kMaxDevices = 64;
+    static bool attr_set[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    if (lds > 0 && !attr_set[dev]) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        attr_set[dev] = true;
+    }
+    return unsigned((items + kXcds - 1) / kXcds * kXcds);
+}
+
 template <typename T, int KB>
 static void launch_pass1(const T *img, T *X, int n, int first, int count, hipStream_t s) {
     using P = Pass1<T, KB>;
-    const unsigned grid = xcd_grid(P::num_items(n, count));
+    const int items = P::num_items(n, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
     auto kern = fwd_pass1<T, KB>;
-    static bool attr_set = false;
-    if (lds > 0 && !attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, img, X, n, first, count);
+    const unsigned grid = launch_grid(kern, lds, items);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, img, X, n, first, items);
 }
 
 template <typename T, int KD>
 static void launch_pass2(const T *X, T *out, int n, int k1, int first, int count,
                          hipStream_t s) {
     using P = Pass2<T, KD>;
-    const unsigned grid = xcd_grid(P::num_items(n, k1, count));
+    const int items = P::num_items(n, k1, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
     auto kern = fwd_pass2<T, KD>;
-    static bool attr_set = false;
-    if (lds > 0 && !attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, X, out, n, k1, first, count);
+    const unsigned grid = launch_grid(kern, lds, items);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, X, out, n, k1, first, items);
 }
 
 template <typename T>

@@ -8,7 +8,9 @@
 // exactly as the library's workspace, so any out-of-bounds access aborts
 // under AddressSanitizer and any read of an unwritten or invalid slot shows
 // up as NaN in the result.  Usage:
-//   fused_emu f32|f64 B n in.bin out.bin
+//   fused_emu f32|f64 B n in.bin out.bin [slots]
+// `slots` (a multiple of 8) caps the grid so that work-groups run several
+// items each; 0 (default) is the library's one-item-per-work-group grid.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -23,46 +25,74 @@ using namespace adrt_hip::fused;
 
 constexpr int log2i(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
 
+// Runs the persistent schedule of `grid` work-groups over `items` items, one
+// item at a time, exactly as the kernels' loop assigns them.
 template <typename P, typename T, typename... Args>
-static void emulate_grid(unsigned grid, Args... args) {
+static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args... args) {
     std::vector<T> lds(P::lds_elems > 0 ? P::lds_elems : 1);
     std::vector<T> buf0(size_t(kLanes) * P::R0 * kW), buf1(size_t(kLanes) * P::R1 * kW);
     auto *v0 = reinterpret_cast<T(*)[P::R0][kW]>(buf0.data());
     auto *v1 = reinterpret_cast<T(*)[P::R1][kW]>(buf1.data());
+    seen.assign(size_t(items), 0);
     for (unsigned blk = 0; blk < grid; ++blk) {
-        const auto it = P::item(args..., int(blk));
-        if (!it.valid) continue;
-        for (auto &x : lds) x = T(NAN);
-        T *l = P::lds_elems > 0 ? lds.data() : nullptr;
-        for (int task = 0; task < P::tasks0; ++task) {
-            for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
-            strip_adrt_host<log2i(P::R0), T>(v0);
-            for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
+        const Schedule sc = schedule(int(blk), int(grid), items);
+        for (int idx = sc.idx; idx < sc.hi; idx += sc.step) {
+            seen[size_t(idx)]++;
+            const auto it = P::item(args..., idx);
+            for (auto &x : lds) x = T(NAN);
+            T *l = P::lds_elems > 0 ? lds.data() : nullptr;
+            for (int task = 0; task < P::tasks0; ++task) {
+                for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
+                strip_adrt_host<log2i(P::R0), T>(v0);
+                for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
+            }
+            // Phase 1: every strip loads and computes before any stores (the
+            // stores may overwrite the LDS rows the loads read).
+            std::vector<T> keep(size_t(P::tasks1 > 0 ? P::tasks1 : 1) * buf1.size());
+            for (int task = 0; task < P::tasks1; ++task) {
+                for (int lane = 0; lane < kLanes; ++lane) P::load1(it, task, lane, l, v1[lane]);
+                strip_adrt_host<log2i(P::R1), T>(v1);
+                std::memcpy(keep.data() + size_t(task) * buf1.size(), buf1.data(), buf1.size() * sizeof(T));
+            }
+            for (int task = 0; task < P::tasks1; ++task) {
+                std::memcpy(buf1.data(), keep.data() + size_t(task) * buf1.size(), buf1.size() * sizeof(T));
+                for (int lane = 0; lane < kLanes; ++lane) P::store1(it, task, lane, v1[lane], l);
+            }
+            for (int tid = 0; tid < kThreads; ++tid) P::phase2(it, l, tid);
         }
-        // Phase 1: every strip loads and computes before any stores (the
-        // stores may overwrite the LDS rows the loads read).
-        std::vector<T> keep(size_t(P::tasks1 > 0 ? P::tasks1 : 1) * buf1.size());
-        for (int task = 0; task < P::tasks1; ++task) {
-            for (int lane = 0; lane < kLanes; ++lane) P::load1(it, task, lane, l, v1[lane]);
-            strip_adrt_host<log2i(P::R1), T>(v1);
-            std::memcpy(keep.data() + size_t(task) * buf1.size(), buf1.data(), buf1.size() * sizeof(T));
-        }
-        for (int task = 0; task < P::tasks1; ++task) {
-            std::memcpy(buf1.data(), keep.data() + size_t(task) * buf1.size(), buf1.size() * sizeof(T));
-            for (int lane = 0; lane < kLanes; ++lane) P::store1(it, task, lane, v1[lane], l);
-        }
-        for (int tid = 0; tid < kThreads; ++tid) P::phase2(it, l, tid);
     }
 }
 
+// Every item of a launch must be visited exactly once.
+static bool once_each(const std::vector<int> &seen) {
+    for (int c : seen)
+        if (c != 1) return false;
+    return true;
+}
+
+// The library launches one work-group per item (items rounded up to 8).  A
+// nonzero `slots` caps the grid instead, so each work-group steps through
+// several items: the schedule must still visit every item exactly once.
+static unsigned grid_for_slots(int items, unsigned slots) {
+    const unsigned need = unsigned((items + kXcds - 1) / kXcds * kXcds);
+    return slots == 0 || need < slots ? need : slots;
+}
+
+static unsigned g_slots = 0;
+
 template <typename T, int KB, int KD>
-static void emulate_group(const T *img, T *X, T *out, int n, int first, int count) {
+static int emulate_group(const T *img, T *X, T *out, int n, int first, int count) {
     using P1 = Pass1<T, KB>;
     using P2 = Pass2<T, KD>;
-    const unsigned g1 = xcd_grid(P1::num_items(n, count));
-    emulate_grid<P1, T>(g1, img, X, n, first, count);
-    const unsigned g2 = xcd_grid(P2::num_items(n, P1::k1, count));
-    emulate_grid<P2, T>(g2, (const T *)X, out, n, P1::k1, first, count);
+    std::vector<int> seen;
+    const int items1 = P1::num_items(n, count);
+    emulate_grid<P1, T>(items1, grid_for_slots(items1, g_slots), seen, img, X, n, first);
+    if (!once_each(seen)) return 5;
+    const int items2 = P2::num_items(n, P1::k1, count);
+    emulate_grid<P2, T>(items2, grid_for_slots(items2, g_slots), seen, (const T *)X, out, n, P1::k1,
+                        first);
+    if (!once_each(seen)) return 6;
+    return 0;
 }
 
 template <typename T>
@@ -81,16 +111,18 @@ static int run(int B, int n, const char *fin, const char *fout) {
     for (int first = 0; first < B; first += int(G)) {
         const int count = (B - first) < int(G) ? (B - first) : int(G);
         const int code = (k1 - 3) * 4 + (L - k1 - 3);
+        int st = 3;
         switch (code) {
-            case 0: emulate_group<T, 0, 0>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 4: emulate_group<T, 1, 0>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 5: emulate_group<T, 1, 1>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 9: emulate_group<T, 2, 1>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 10: emulate_group<T, 2, 2>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 14: emulate_group<T, 3, 2>(img.data(), X.data(), out.data(), n, first, count); break;
-            case 15: emulate_group<T, 3, 3>(img.data(), X.data(), out.data(), n, first, count); break;
-            default: return 3;
+            case 0: st = emulate_group<T, 0, 0>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 4: st = emulate_group<T, 1, 0>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 5: st = emulate_group<T, 1, 1>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 9: st = emulate_group<T, 2, 1>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 10: st = emulate_group<T, 2, 2>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 14: st = emulate_group<T, 3, 2>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 15: st = emulate_group<T, 3, 3>(img.data(), X.data(), out.data(), n, first, count); break;
+            default: break;
         }
+        if (st != 0) return st;
     }
     f = std::fopen(fout, "wb");
     if (!f || std::fwrite(out.data(), sizeof(T), out_elems, f) != out_elems) return 4;
@@ -99,10 +131,11 @@ static int run(int B, int n, const char *fin, const char *fout) {
 }
 
 int main(int argc, char **argv) {
-    if (argc != 6) {
-        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin\n", argv[0]);
+    if (argc != 6 && argc != 7) {
+        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [slots]\n", argv[0]);
         return 1;
     }
+    if (argc == 7) g_slots = unsigned(std::atoi(argv[6]));
     const int B = std::atoi(argv[2]), n = std::atoi(argv[3]);
     if (std::strcmp(argv[1], "f32") == 0) return run<float>(B, n, argv[4], argv[5]);
     return run<double>(B, n, argv[4], argv[5]);

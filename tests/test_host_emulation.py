@@ -35,14 +35,14 @@ def emulator(tmp_path_factory):
     return exe
 
 
-def run_emu(exe, x, tmp_path):
+def run_emu(exe, x, tmp_path, slots=0):
     b, n, _ = x.shape
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     x.tofile(fin)
     tag = "f32" if x.dtype == np.float32 else "f64"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout)], capture_output=True,
-                         text=True, env=env)
+    res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout), str(slots)],
+                         capture_output=True, text=True, env=env)
     assert res.returncode == 0, res.stderr[-4000:]
     return np.fromfile(fout, dtype=x.dtype).reshape(b, 4, 2 * n - 1, n)
 
@@ -64,4 +64,13 @@ def test_fused_emulation_multi_group(emulator, tmp_path):
     """2048^2 float32: one image per group, so two images take two groups."""
     x = np.random.default_rng(7).random((2, 2048, 2048), dtype=np.float32)
     got = run_emu(emulator, x, tmp_path)
+    assert np.array_equal(got.view(np.uint8), O.adrt(x).view(np.uint8))
+
+
+@pytest.mark.parametrize("slots", [8, 24])
+def test_fused_emulation_capped_grid(emulator, tmp_path, slots):
+    """A grid smaller than the item count: each work-group walks several
+    items and every item must still be computed exactly once."""
+    x = np.random.default_rng(slots).standard_normal((3, 256, 256)).astype(np.float32)
+    got = run_emu(emulator, x, tmp_path, slots)
     assert np.array_equal(got.view(np.uint8), O.adrt(x).view(np.uint8))
