@@ -15,7 +15,13 @@ the reference sources into oracle/_ref/libadrt_ref.so) on a bounded sample
 of the same phantoms, on this host's cores; if that library is absent the
 oracle restatement is timed instead.
 
-Prints exactly one JSON line.
+Measured HBM traffic ("roofline.traffic") comes from the committed rocprofv3
+summary profiles/fwd_adrt_16x2048_f32.md, made by two counter runs of
+`bench.py --profile` (FETCH_SIZE, then WRITE_SIZE) and `bench.py --summarize`.
+Each counter is scaled by what it reads for a plain 256 MiB device copy in
+the same run (FETCH_SIZE reads half the bytes a copy moves, WRITE_SIZE all).
+
+Prints exactly one JSON line (or, with --summarize, the profile summary).
 """
 
 import argparse
@@ -29,6 +35,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0
 METRIC = "fwd ADRT 2048² fp32 imgs/s (%HBM peak)"
+PROFILE = os.path.join(ROOT, "profiles", "fwd_adrt_16x2048_f32.md")
+CALIB_BYTES = 256 << 20  # device-to-device copy timed and counted beside the forward
 
 
 def parse():
@@ -42,6 +50,14 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--extras", action="store_true", help="also time bdrt / iadrt / fp64")
+    p.add_argument("--profile", action="store_true",
+                   help="short run for rocprofv3 (1 warm-up + 3 steps, no CPU leg), followed "
+                        "by a 256 MiB device copy that calibrates the memory counters")
+    p.add_argument("--summarize", nargs="+", metavar="DIR",
+                   help="print a summary of rocprofv3 csv output directories (kernel stats "
+                        "and/or FETCH_SIZE / WRITE_SIZE counter runs of --profile)")
+    p.add_argument("--calls", type=int, default=4,
+                   help="forward calls in the profiled run (--summarize)")
     return p.parse_args()
 
 
@@ -105,12 +121,94 @@ def cpu_baseline(host_imgs, budget_s):
     }
 
 
+def _csv_rows(dirs, suffix):
+    import csv
+    import glob
+
+    for d in dirs:
+        for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+            with open(f, newline="") as fh:
+                yield from csv.DictReader(fh)
+
+
+def _short(name):
+    name = name.split("(")[0].replace("void ", "")
+    return name.replace("adrt_hip::fused::", "")
+
+
+def summarize(dirs, calls):
+    """Kernel durations and calibrated HBM traffic per forward call from
+    rocprofv3 csv output (kernel stats and FETCH_SIZE / WRITE_SIZE runs)."""
+    lines = []
+    stats = [r for r in _csv_rows(dirs, "kernel_stats.csv") if "adrt_hip" in r["Name"]]
+    if stats:
+        lines += ["| kernel | calls | avg µs | min µs | max µs | total µs |",
+                  "|---|---|---|---|---|---|"]
+        total = 0.0
+        for r in stats:
+            t = float(r["TotalDurationNs"]) / 1e3
+            total += t
+            lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                         f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {t:.1f} |")
+        lines.append(f"\nkernel_us_per_call: {total / calls:.1f}")
+    ctr = {}
+    for r in _csv_rows(dirs, "counter_collection.csv"):
+        name = r.get("Kernel_Name", "")
+        key = ("copy" if "copyBuffer" in name else _short(name)) if (
+            "adrt_hip" in name or "copyBuffer" in name) else None
+        if key is None:
+            continue
+        ctr.setdefault(key, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    if ctr:
+        copy = ctr.get("copy", {})
+        kib = float(1 << 10)
+        # the calibration copy is the largest copy dispatch of the run
+        fetch_scale = CALIB_BYTES / (max(copy["FETCH_SIZE"]) * kib) if copy.get("FETCH_SIZE") else None
+        write_scale = CALIB_BYTES / (max(copy["WRITE_SIZE"]) * kib) if copy.get("WRITE_SIZE") else None
+        lines += ["", f"calibration (256 MiB device copy): FETCH_SIZE x {fetch_scale:.3f}, "
+                      f"WRITE_SIZE x {write_scale:.3f}" if fetch_scale and write_scale else
+                  "calibration copy missing", "",
+                  "| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch | "
+                  "HBM read MiB/dispatch | HBM write MiB/dispatch |", "|---|---|---|---|---|---|"]
+        per_call = 0.0
+        for key, d in sorted(ctr.items()):
+            if key == "copy":
+                continue
+            f = d.get("FETCH_SIZE", [])
+            w = d.get("WRITE_SIZE", [])
+            fm = sum(f) / len(f) if f else float("nan")
+            wm = sum(w) / len(w) if w else float("nan")
+            rd = fm * kib * (fetch_scale or float("nan")) / (1 << 20)
+            wr = wm * kib * (write_scale or float("nan")) / (1 << 20)
+            lines.append(f"| `{key}` | {max(len(f), len(w))} | {fm:.0f} | {wm:.0f} | {rd:.1f} | {wr:.1f} |")
+            per_call += (sum(f) * (fetch_scale or 0) + sum(w) * (write_scale or 0)) * kib
+        lines.append(f"\ntraffic_bytes_per_call: {int(per_call / calls)}")
+    return "\n".join(lines)
+
+
+def profiled_traffic():
+    """HBM bytes per forward call from the committed rocprofv3 summary."""
+    try:
+        with open(PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_call:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     args = parse()
+    if args.summarize:
+        print(summarize(args.summarize, args.calls))
+        return
+    if args.profile:
+        args.steps, args.warmup, args.no_cpu, args.extras = 3, 1, True, False
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(every MI355X box here has one GPU; batches shard across GPUs with no "
-              "exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: "
+              "this benchmark runs on a single MI355X (a batch of images would shard "
+              "across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
     import ctypes
 
@@ -174,12 +272,18 @@ def main():
             "peak": PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "adrt_hip_adrt: all kernels of one forward call",
+            "traffic": profiled_traffic() if (n, batch, s) == (2048, 16, 4) else None,
+            "kernel": "adrt_hip_adrt: one forward call (fwd_pass1 + fwd_pass2 per group "
+                      "of images)",
             "algorithmic_bytes_per_call": algo,
             "device_ms_per_call": round(dev_ms, 4),
         },
     }
+    if args.profile:
+        a = torch.empty(CALIB_BYTES, device=device, dtype=torch.uint8)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
     if args.extras:
         extras = {}
         y = out.clone()
