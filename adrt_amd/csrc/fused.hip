@@ -156,24 +156,15 @@ __host__ __device__ __forceinline__ int clamp_start(int c, int lim) {
     return c < 0 ? 0 : (c > lim - 4 ? lim - 4 : c);
 }
 
-// Window [c, c+4) of `row`: one load from the clamped start; windows that
-// cross an edge are completed by patch_window once every row is loaded.
+// t_k for k in [0, 3], `below` for k < 0, `above` for k > 3: a chain of
+// selects on four scalars (an array here would be indexed through scratch).
 template <typename T>
-__host__ __device__ __forceinline__ void load_window(const T *__restrict__ row, int c, int lim,
-                                                     T (&out)[4]) {
-    load_vec4(row + unsigned(clamp_start(c, lim)), out[0], out[1], out[2], out[3]);
-}
-
-template <typename T>
-__host__ __device__ __forceinline__ void patch_window(const T *__restrict__ row, int c, int lim,
-                                                      T (&v)[4]) {
-    if (c < 0 || c > lim - 4) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = c + i;
-            v[i] = k < 0 ? neg_zero<T>() : (k >= lim ? T(0) : row[k]);
-        }
-    }
+__host__ __device__ __forceinline__ T pick4(T t0, T t1, T t2, T t3, int k, T below, T above) {
+    T v = k < 0 ? below : t0;
+    v = k == 1 ? t1 : v;
+    v = k == 2 ? t2 : v;
+    v = k == 3 ? t3 : v;
+    return k > 3 ? above : v;
 }
 
 // Reversed window: out[i] = row[lim - 1 - (c + i)], i.e. offsets c..c+3 of a
@@ -185,14 +176,13 @@ __host__ __device__ __forceinline__ void load_window_rev(const T *__restrict__ r
 }
 
 template <typename T>
-__host__ __device__ __forceinline__ void patch_window_rev(const T *__restrict__ row, int c,
-                                                          int lim, T (&v)[4]) {
+__host__ __device__ __forceinline__ void patch_window_rev(int c, int lim, T (&v)[4]) {
     if (c < 0 || c > lim - 4) {
+        const T t0 = v[3], t1 = v[2], t2 = v[1], t3 = v[0];  // ascending columns
+        const int e = (lim - 4 - c) - clamp_start(lim - 4 - c, lim);
+        // offset c + i is column lim - 1 - c - i = clamped start + e + 3 - i
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = c + i;
-            v[i] = k < 0 ? neg_zero<T>() : (k >= lim ? T(0) : row[lim - 1 - k]);
-        }
+        for (int i = 0; i < 4; ++i) v[i] = pick4(t0, t1, t2, t3, e + 3 - i, T(0), neg_zero<T>());
     }
 }
 
@@ -275,6 +265,28 @@ inline void strip_adrt_host(T (*a)[1 << K][kW]) {
             for (int i = 0; i < kW; ++i) a[l][j][i] = cur[j][l * kW + i];
 }
 
+// Intermediate X rows (element offsets relative to the row's offset 0):
+//   [-4, 0)         -0.0
+//   [0, W1)         the W1 = n + 2^k1 - 1 valid offsets
+//   [W1, W1 + 4)    +0.0 (and +0.0 wherever pass 1's last chunk runs past
+//                   W1: those values come from zero padding only)
+// A window of 4 offsets whose start is clamped into [-4, W1] therefore reads
+// exactly the padded row (-0.0 below 0, +0.0 from W1 on) with one
+// unconditional load.  Pass 1 writes the margins.  The left margin (kXL
+// elements) and the pitch are whole 128-byte lines (256 bytes for fp64), so
+// pass 1's row segments are line-aligned.
+constexpr int kXL = 32;
+constexpr int kXPad = 4;  // margin elements read on each side
+__host__ __device__ __forceinline__ int x_valid(int n, int k1) { return n + (1 << k1) - 1; }
+__host__ __device__ __forceinline__ int x_chunks(int n, int k1) {
+    return (x_valid(n, k1) + kOut - 1) / kOut;
+}
+__host__ __device__ __forceinline__ int x_pitch(int n, int k1) {
+    const int w = x_valid(n, k1) + kXPad;
+    const int span = x_chunks(n, k1) * kOut;
+    return kXL + ((w > span ? w : span) + 31) / 32 * 32;
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1: rows row0 .. row0 + 2^k1 of quadrant q, X offsets [c0, c0 + kOut).
 //   phase 0 (level A): strip g (2^KB of them): rows row0 + 8g + j, base
@@ -300,9 +312,7 @@ struct Pass1 {
         int q, n, c0, row0, W1, P1;
     };
 
-    __host__ __device__ __forceinline__ static int num_chunks(int n) {
-        return (n + (1 << k1) - 1 + kOut - 1) / kOut;
-    }
+    __host__ __device__ __forceinline__ static int num_chunks(int n) { return x_chunks(n, k1); }
     __host__ __device__ __forceinline__ static int num_items(int n, int nimg) {
         return nimg * 4 * (n >> k1) * num_chunks(n);
     }
@@ -319,8 +329,8 @@ struct Pass1 {
                                                          int first_image, int block) {
         Item it;
         it.n = n;
-        it.W1 = n + (1 << k1) - 1;  // valid offsets of X
-        it.P1 = n + (1 << k1);      // row pitch of X (multiple of 4)
+        it.W1 = x_valid(n, k1);  // valid offsets of X
+        it.P1 = x_pitch(n, k1);  // row pitch of X (multiple of 4)
         const int nJ = n >> k1;
         const int nchunk = num_chunks(n);
         const int m = block & 1;
@@ -341,7 +351,7 @@ struct Pass1 {
         it.c0 = chunk * kOut;
         it.img = img_base + (size_t)b * n * n;
         it.X = X_base + ((size_t)(b - first_image) * 4 + it.q) * nJ * ((size_t)1 << k1) * it.P1 +
-               (size_t)J * ((size_t)1 << k1) * it.P1;
+               (size_t)J * ((size_t)1 << k1) * it.P1 + kXL;
         it.row0 = J << k1;
         return it;
     }
@@ -350,61 +360,98 @@ struct Pass1 {
         return KB == 0 ? it.c0 - kLead : it.c0 - HB - (RA - 1) * g - (RA - 1);
     }
 
-    // Level A input: 8 quadrant rows at offsets base0 + 4*lane + i.  All
-    // loads are issued before any edge patching.
+    // Level A input: 8 quadrant rows at offsets base0 + 4*lane + i, in two
+    // steps: every load is issued first (from start offsets clamped into the
+    // image), then the windows that cross an image edge are patched, so all
+    // rows' loads are in flight together.
+    //
+    // q1/q2: rows are image columns r0..r0+7; offset c+i is image row
+    // n-1-(c+i) (q1) or c+i (q2), clamped into the image for the load.
+    __host__ __device__ __forceinline__ static void issue_cols(const Item &it, int c, int r0,
+                                                               T (&v)[RA][kW]) {
+        const int n = it.n;
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int cc = c + i;
+            int irow = it.q == 1 ? n - 1 - cc : cc;
+            irow = irow < 0 ? 0 : (irow > n - 1 ? n - 1 : irow);
+            const T *src = it.img + unsigned(irow * n + r0);
+            load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
+            load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
+        }
+    }
+    __host__ __device__ __forceinline__ static void patch_cols(int c, int n, T (&v)[RA][kW]) {
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int cc = c + i;
+            if (cc < 0 || cc >= n) {
+                const T pad = cc < 0 ? neg_zero<T>() : T(0);
+#pragma unroll
+                for (int j = 0; j < RA; ++j) v[j][i] = pad;
+            }
+        }
+    }
+    // q0/q3: rows are image rows (q0: r, q3: n-1-r); offsets c..c+3 are the
+    // columns n-1-c .. n-4-c.
+    __host__ __device__ __forceinline__ static void issue_rows(const Item &it, int c, int r0,
+                                                               T (&v)[RA][kW]) {
+        const int n = it.n;
+#pragma unroll
+        for (int j = 0; j < RA; ++j) {
+            const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
+            load_window_rev(it.img + (size_t)irow * n, c, n, v[j]);
+        }
+    }
+    __host__ __device__ __forceinline__ static void patch_rows(int c, int n, T (&v)[RA][kW]) {
+#pragma unroll
+        for (int j = 0; j < RA; ++j) patch_window_rev(c, n, v[j]);
+    }
+
     __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
                                                           T (&v)[RA][kW]) {
         const int c = base0(it, g) + kW * lane;
-        const int r0 = it.row0 + g * RA;
-        const int n = it.n;
         if (it.q == 1 || it.q == 2) {
-            // Rows are image columns r0..r0+7; offset c+i is image row
-            // n-1-(c+i) (q1) or c+i (q2), clamped into the image for the load.
-#pragma unroll
-            for (int i = 0; i < kW; ++i) {
-                const int cc = c + i;
-                int irow = it.q == 1 ? n - 1 - cc : cc;
-                irow = irow < 0 ? 0 : (irow > n - 1 ? n - 1 : irow);
-                const T *src = it.img + unsigned(irow * n + r0);
-                load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
-                load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
-            }
-#pragma unroll
-            for (int i = 0; i < kW; ++i) {
-                const int cc = c + i;
-                if (cc < 0 || cc >= n) {
-                    const T pad = cc < 0 ? neg_zero<T>() : T(0);
-#pragma unroll
-                    for (int j = 0; j < RA; ++j) v[j][i] = pad;
-                }
-            }
+            issue_cols(it, c, it.row0 + g * RA, v);
+            patch_cols(c, it.n, v);
         } else {
-            // Rows are image rows (q0: r, q3: n-1-r); offsets c..c+3 are the
-            // columns n-1-c .. n-4-c.
-#pragma unroll
-            for (int j = 0; j < RA; ++j) {
-                const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
-                load_window_rev(it.img + (size_t)irow * n, c, n, v[j]);
-            }
-#pragma unroll
-            for (int j = 0; j < RA; ++j) {
-                const int irow = it.q == 0 ? r0 + j : n - 1 - (r0 + j);
-                patch_window_rev(it.img + (size_t)irow * n, c, n, v[j]);
-            }
+            issue_rows(it, c, it.row0 + g * RA, v);
+            patch_rows(c, it.n, v);
         }
     }
 
     // X rows mA*R + t at offsets c0 - kLead + 4*lane + i; lanes 16..63 deliver
-    // [c0, c0 + kOut).  A window may run past W1 up to P1 - 1: harmless padding.
+    // [c0, c0 + kOut) (a last chunk running past W1 delivers +0.0 there,
+    // computed from zero padding only).  Lane 0, which delivers nothing,
+    // writes the row margins: -0.0 before offset 0 (first chunk) and +0.0
+    // from the end of the last chunk up to W1 + 4, if it ends before that.
     template <int R>
     __host__ __device__ __forceinline__ static void store_x(const Item &it, int mA, int lane,
                                                             const T (&o)[R][kW]) {
         const int c = it.c0 - kLead + kW * lane;
-        if (lane < kLead / kW || c >= it.W1) return;
+        if (lane >= kLead / kW) {
 #pragma unroll
-        for (int t = 0; t < R; ++t)
-            store_vec4(it.X + (size_t)(mA * R + t) * it.P1 + unsigned(c), o[t][0], o[t][1], o[t][2],
-                       o[t][3]);
+            for (int t = 0; t < R; ++t)
+                store_vec4(it.X + (size_t)(mA * R + t) * it.P1 + unsigned(c), o[t][0], o[t][1],
+                           o[t][2], o[t][3]);
+        }
+        if (lane == 0) {
+            if (it.c0 == 0) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    T *row = it.X + (size_t)(mA * R + t) * it.P1;
+                    store_vec4(row - kXPad, neg_zero<T>(), neg_zero<T>(), neg_zero<T>(),
+                               neg_zero<T>());
+                }
+            }
+            if (it.c0 + kOut > it.W1) {  // the last chunk (W1 is odd, c0 + kOut even)
+                for (int z = it.c0 + kOut; z < it.W1 + kXPad; z += 4) {
+#pragma unroll
+                    for (int t = 0; t < R; ++t)
+                        store_vec4(it.X + (size_t)(mA * R + t) * it.P1 + unsigned(z), T(0), T(0),
+                                   T(0), T(0));
+                }
+            }
+        }
     }
 
     __host__ __device__ __forceinline__ static void store0(const Item &it, int g, int lane,
@@ -491,8 +538,8 @@ struct Pass2 {
         Item it;
         it.n = n;
         it.C = 2 * n - 1;
-        it.W1 = n + (1 << k1) - 1;
-        const int P1 = n + (1 << k1);
+        it.W1 = x_valid(n, k1);
+        const int P1 = x_pitch(n, k1);
         const int nS = 1 << k1;
         const int nchunk = num_chunks(n);
         const int chunk = block % nchunk;
@@ -504,7 +551,7 @@ struct Pass2 {
         it.d0 = chunk * kOut;
         it.xrow_stride = (size_t)nS * P1;  // between blocks J at fixed s
         it.X = X_base + ((size_t)(b - first_image) * 4 + q) * ((size_t)1 << k2) * it.xrow_stride +
-               (size_t)it.s * P1;
+               (size_t)it.s * P1 + kXL;
         it.out = out_base + ((size_t)b * 4 + q) * it.C * (size_t)n;
         it.abase = it.s << k2;
         return it;
@@ -520,17 +567,20 @@ struct Pass2 {
         return d * A2 + 4 * ((a >> 2) ^ ((d >> 2) & (G - 1))) + (a & 3);
     }
 
-    // Level C input: blocks 8g + j at angle s, row j skewed by s*j.  All
-    // loads are issued before any edge patching.
+    // Level C input: blocks 8g + j at angle s, row j skewed by s*j.  The
+    // row margins make every window one load with its start clamped into
+    // [-4, W1] (a window reaching past either edge reads the margin).
     __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
                                                           T (&v)[RC][kW]) {
         const int c = base0(it, g) + kW * lane;
 #pragma unroll
-        for (int j = 0; j < RC; ++j)
-            load_window(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
-#pragma unroll
-        for (int j = 0; j < RC; ++j)
-            patch_window(it.X + (size_t)(g * RC + j) * it.xrow_stride, c - it.s * j, it.W1, v[j]);
+        for (int j = 0; j < RC; ++j) {
+            // start clamped into [-4, W1], taken relative to offset -4
+            int cj = c - it.s * j + kXPad;
+            cj = cj < 0 ? 0 : (cj > it.W1 + kXPad ? it.W1 + kXPad : cj);
+            load_vec4(it.X - kXPad + (size_t)(g * RC + j) * it.xrow_stride + unsigned(cj), v[j][0],
+                      v[j][1], v[j][2], v[j][3]);
+        }
     }
 
     // Tile rows d = 4*lane + i - kLead (lanes 16..63), angles a0 + t.
@@ -649,19 +699,21 @@ constexpr int lds_waves_per_simd(int cap) {
 
 template <typename T, int KB>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
-                               amdgpu_waves_per_eu(lds_waves_per_simd<Pass1<T, KB>, T>(sizeof(T) == 4 ? 8 : 4))))
+                               amdgpu_waves_per_eu(lds_waves_per_simd<Pass1<T, KB>, T>(
+                                   sizeof(T) == 4 ? 8 : 4))))
 fwd_pass1(const T *__restrict__ img_base, T *__restrict__ X_base, int n, int first_image,
-              int items) {
+          int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     run_item<Pass1<T, KB>, T>(items, reinterpret_cast<T *>(smem_raw), img_base, X_base, n,
-                               first_image);
+                              first_image);
 }
 
 template <typename T, int KD>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
-                               amdgpu_waves_per_eu(lds_waves_per_simd<Pass2<T, KD>, T>(sizeof(T) == 4 ? 8 : 4))))
+                               amdgpu_waves_per_eu(lds_waves_per_simd<Pass2<T, KD>, T>(
+                                   sizeof(T) == 4 ? 8 : 4))))
 fwd_pass2(const T *__restrict__ X_base, T *__restrict__ out_base, int n, int k1,
-              int first_image, int items) {
+          int first_image, int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     run_item<Pass2<T, KD>, T>(items, reinterpret_cast<T *>(smem_raw), X_base, out_base, n, k1,
                                first_image);
@@ -683,8 +735,7 @@ inline size_t group_images(size_t B, size_t per_image_bytes) {
 }
 
 inline size_t x_bytes_per_image(int n, int L, size_t elem) {
-    const int k1 = pass1_stages(L);
-    return size_t(4) * n * size_t(n + (1 << k1)) * elem;
+    return size_t(4) * n * size_t(x_pitch(n, pass1_stages(L))) * elem;
 }
 
 #ifndef ADRT_HOST_EMULATION
