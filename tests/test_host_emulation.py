@@ -74,3 +74,38 @@ def test_fused_emulation_capped_grid(emulator, tmp_path, slots):
     x = np.random.default_rng(slots).standard_normal((3, 256, 256)).astype(np.float32)
     got = run_emu(emulator, x, tmp_path, slots)
     assert np.array_equal(got.view(np.uint8), O.adrt(x).view(np.uint8))
+
+
+IADRT_SRC = os.path.join(HERE, "host_emulation", "iadrt_emu.cpp")
+
+
+@pytest.fixture(scope="module")
+def iadrt_emulator(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None or not os.path.isdir(ROCM_INCLUDE):
+        pytest.skip("g++ or ROCm headers not available")
+    exe = str(tmp_path_factory.mktemp("emu") / "iadrt_emu")
+    cmd = [gxx, "-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fsanitize=address",
+           "-D__HIP_PLATFORM_AMD__", f"-I{ROCM_INCLUDE}", f"-I{os.path.join(ROOT, 'include')}",
+           "-x", "c++", IADRT_SRC, "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+@pytest.mark.parametrize("n", [1, 2, 4, 16, 64, 128, 256])
+def test_iadrt_emulation_bytes_equal(iadrt_emulator, tmp_path, dt, n):
+    """The iadrt stage kernel's lane logic (lagged odd chains, 64-row ring,
+    row flushes) run on the CPU must equal the oracle byte for byte."""
+    rng = np.random.default_rng(n)
+    b = 2 if n <= 16 else 1
+    x = (rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)
+    x.flat[:3] = [-0.0, 0.0, -0.0]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(fin)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([iadrt_emulator, "f32" if dt == "float32" else "f64", str(b), str(n),
+                          str(fin), str(fout)], capture_output=True, text=True, env=env)
+    assert res.returncode == 0, res.stderr[-4000:]
+    got = np.fromfile(fout, dtype=dt).reshape(x.shape)
+    assert np.array_equal(got.view(np.uint8), O.iadrt(x).view(np.uint8))
