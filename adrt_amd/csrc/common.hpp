@@ -75,6 +75,9 @@ size_t adrt_workspace(int dtype, size_t B, size_t N);
 bool fused_supported(size_t n);
 size_t fused_workspace(int dtype, size_t B, size_t n);
 int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, hipStream_t s);
+size_t fused_bdrt_workspace(int dtype, size_t B, size_t n);
+int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
+                   hipStream_t s);
 int launch_adrt_step(const void *in, void *out, int dtype, size_t B, size_t N, int step,
                      hipStream_t s);
 int launch_bdrt(const void *in, void *out, void *ws, size_t ws_bytes, int dtype, size_t B,

@@ -265,6 +265,95 @@ inline void strip_adrt_host(T (*a)[1 << K][kW]) {
             for (int i = 0; i < kW; ++i) a[l][j][i] = cur[j][l * kW + i];
 }
 
+// ---------------------------------------------------------------------------
+// Back-projection strip: K bdrt stages (adrt_cdefs_bdrt.hpp:55-116) on
+// R = 2^K indices held as a[index * S + section][i] (S sections per index,
+// strip offset 4*lane + i).  Stage u pairs indices (2j, 2j+1) into index j
+// with sections 2*sec (left: x + y) and 2*sec + 1 (right: x at offset +j plus
+// y at offset +j+1); the level's common shift (the pair index of the
+// enclosing group) is applied outside, as a skew.  Operands above the lane
+// come from the next lane (DPP wave_shl:1).  After K stages a[s] holds
+// section s.  Valid for strip offsets below 256 - (2^K - 1).
+// ---------------------------------------------------------------------------
+#ifndef ADRT_HOST_EMULATION
+template <typename T>
+__device__ __forceinline__ T lane_shl1(T v);
+
+template <>
+__device__ __forceinline__ float lane_shl1<float>(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
+template <>
+__device__ __forceinline__ double lane_shl1<double>(double v) {
+    const long long bits = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(bits & 0xffffffff), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(bits >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                  (unsigned long long)(unsigned)lo));
+}
+
+template <int K, typename T>
+__device__ __forceinline__ void strip_bdrt(T (&a)[1 << K][kW]) {
+    constexpr int R0 = 1 << K;
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const int S = 1 << u;   // sections per index before this stage
+        const int I = R0 >> u;  // indices before this stage
+        T b[R0][kW];
+#pragma unroll
+        for (int j = 0; j < I / 2; ++j) {
+#pragma unroll
+            for (int sec = 0; sec < S; ++sec) {
+                const int xa = (2 * j) * S + sec, ya = (2 * j + 1) * S + sec;
+#pragma unroll
+                for (int i = 0; i < kW; ++i) b[j * 2 * S + 2 * sec][i] = a[xa][i] + a[ya][i];
+#pragma unroll
+                for (int i = 0; i < kW; ++i) {
+                    const T x = (i + j < kW) ? a[xa][i + j] : lane_shl1(a[xa][i + j - kW]);
+                    const T y = (i + j + 1 < kW) ? a[ya][i + j + 1]
+                                                 : lane_shl1(a[ya][i + j + 1 - kW]);
+                    b[j * 2 * S + 2 * sec + 1][i] = x + y;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+#pragma unroll
+            for (int i = 0; i < kW; ++i) a[r][i] = b[r][i];
+    }
+}
+#endif
+
+// Whole-strip restatement for host emulation; offsets whose operands fall
+// past the strip are poisoned with NaN.
+template <int K, typename T>
+inline void strip_bdrt_host(T (*a)[1 << K][kW]) {
+    constexpr int R0 = 1 << K;
+    static T cur[R0][kStrip], nxt[R0][kStrip];
+    for (int l = 0; l < kLanes; ++l)
+        for (int r = 0; r < R0; ++r)
+            for (int i = 0; i < kW; ++i) cur[r][l * kW + i] = a[l][r][i];
+    for (int u = 0; u < K; ++u) {
+        const int S = 1 << u, I = R0 >> u;
+        for (int j = 0; j < I / 2; ++j)
+            for (int sec = 0; sec < S; ++sec) {
+                const int xa = (2 * j) * S + sec, ya = (2 * j + 1) * S + sec;
+                for (int o = 0; o < kStrip; ++o) {
+                    nxt[j * 2 * S + 2 * sec][o] = cur[xa][o] + cur[ya][o];
+                    nxt[j * 2 * S + 2 * sec + 1][o] =
+                        o + j + 1 < kStrip ? cur[xa][o + j] + cur[ya][o + j + 1] : T(NAN);
+                }
+            }
+        for (int r = 0; r < R0; ++r)
+            for (int o = 0; o < kStrip; ++o) cur[r][o] = nxt[r][o];
+    }
+    for (int l = 0; l < kLanes; ++l)
+        for (int r = 0; r < R0; ++r)
+            for (int i = 0; i < kW; ++i) a[l][r][i] = cur[r][l * kW + i];
+}
+
 // Intermediate X rows (element offsets relative to the row's offset 0):
 //   [-4, 0)         -0.0
 //   [0, W1)         the W1 = n + 2^k1 - 1 valid offsets
@@ -305,6 +394,13 @@ struct Pass1 {
     static constexpr int tasks0 = RB;
     static constexpr int tasks1 = KB == 0 ? 0 : RA;
     static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) * kRow;
+
+#ifndef ADRT_HOST_EMULATION
+    __device__ __forceinline__ static void strip0(T (&v)[R0][kW]) { strip_adrt<KA, T>(v); }
+    __device__ __forceinline__ static void strip1(T (&v)[R1][kW]) { strip_adrt<KB, T>(v); }
+#endif
+    static void strip0_host(T (*v)[R0][kW]) { strip_adrt_host<KA, T>(v); }
+    static void strip1_host(T (*v)[R1][kW]) { strip_adrt_host<KB, T>(v); }
 
     struct Item {
         const T *img;
@@ -518,6 +614,13 @@ struct Pass2 {
     // The tile reuses the level-C rows once every level-D strip has loaded.
     static constexpr size_t lds_elems = lc_elems > tile_elems ? lc_elems : tile_elems;
 
+#ifndef ADRT_HOST_EMULATION
+    __device__ __forceinline__ static void strip0(T (&v)[R0][kW]) { strip_adrt<KC, T>(v); }
+    __device__ __forceinline__ static void strip1(T (&v)[R1][kW]) { strip_adrt<KD, T>(v); }
+#endif
+    static void strip0_host(T (*v)[R0][kW]) { strip_adrt_host<KC, T>(v); }
+    static void strip1_host(T (*v)[R1][kW]) { strip_adrt_host<KD, T>(v); }
+
     struct Item {
         const T *X;  // row (J, s) at X + J * xrow_stride
         T *out;      // plane (b, q)
@@ -652,8 +755,312 @@ struct Pass2 {
     }
 };
 
+// ===========================================================================
+// Fused two-pass back-projection (bdrt, adrt_cdefs_bdrt.hpp:55-187).
+//
+// Stage i pairs angle groups (2si, 2si+1) of the previous stage into group
+// si; section s of the pair becomes sections 2s (x + y, no shift) and 2s+1
+// (x at row r+si plus y at row r+si+1).  Unrolled, a group m of R = 2^K
+// consecutive groups at one level goes through K stages whose shifts are
+// m * (R / 2^(u+1)) + (local pair index) at stage u, so the K-stage result
+// for output section s_loc is the local K-stage strip (strip_bdrt) shifted
+// by the common m * s_loc: computed at strip offset rho, it belongs to row
+// rho - m * s_loc.  The L stages split into
+//   pass A (kA = floor(L/2) stages): for every group p of 2^kA input angles
+//     and chunk of input rows, a 3-stage level over 8 angles and a
+//     (kA-3)-stage level over the level-A groups, stored to Z[p][s][r] with
+//     the per-section skew p * s (store side);
+//   pass B (kB = ceil(L/2) stages): for every section s of Z and chunk of
+//     output rows, a 3-stage level over 8 groups p and a (kB-3)-stage level
+//     whose inputs are skewed by g' * s3 (load side), written through an
+//     LDS tile as rows of 2^kB consecutive output columns.
+// Rows past the last real row C-1 read as -0.0 at row C and +0.0 above, the
+// values that make the reference's boundary rule (right[C-1-si] = x[C-1],
+// zeros after) fall out of the same adds; the result is bit-identical to
+// the reference up to the sign of zero results (canonical -0.0 == +0.0).
+// ===========================================================================
+
+// Z rows: kZL elements of scratch, rows [0, C), margins C: -0.0 and
+// C+1..C+4: +0.0, then scratch up to the pitch (whole 128-byte lines).
+constexpr int kZL = 32;
+__host__ __device__ __forceinline__ int z_pitch(int n) { return kZL + (2 * n - 1 + 8 + 31) / 32 * 32; }
+
+// LDS column c = 4*lane + u of a row (u >= 0, c < kRow): interleaved by
+// residue so that one wave-uniform u is bank-consecutive across lanes.
+__host__ __device__ __forceinline__ int lds_col(int u) { return (u & (kW - 1)) * kSlice + (u >> 2); }
+
+template <typename T, int K2>
+struct BdrtA {
+    static constexpr int kA = 3 + K2;
+    static constexpr int R0 = 8, R1 = 1 << K2;
+    static constexpr int tasks0 = R1;                // level-1 strips g
+    static constexpr int tasks1 = K2 == 0 ? 0 : 8;  // level-2 strips s1
+    static constexpr size_t lds_elems = K2 == 0 ? 0 : size_t(R1) * 8 * kRow;
+
 #ifndef ADRT_HOST_EMULATION
-constexpr int log2_small(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
+    __device__ __forceinline__ static void strip0(T (&v)[R0][kW]) { strip_bdrt<3, T>(v); }
+    __device__ __forceinline__ static void strip1(T (&v)[R1][kW]) { strip_bdrt<K2, T>(v); }
+#endif
+    static void strip0_host(T (*v)[R0][kW]) { strip_bdrt_host<3, T>(v); }
+    static void strip1_host(T (*v)[R1][kW]) { strip_bdrt_host<K2, T>(v); }
+
+    struct Item {
+        const T *y;  // input plane (C, n)
+        T *Z;        // row 0 of section 0 of group p; section s at + s * PZ
+        int n, C, p, c0, PZ;
+    };
+
+    // Chunks of input rows: outputs reach row C + 4 (margins) after the
+    // largest skew (groups - 1) * (sections - 1).
+    __host__ __device__ __forceinline__ static int num_chunks(int n) {
+        const int C = 2 * n - 1;
+        const int maxskew = ((n >> kA) - 1) * ((1 << kA) - 1);
+        return (C + 5 + maxskew + kOut - 1) / kOut;
+    }
+    __host__ __device__ __forceinline__ static int num_items(int n, int nimg) {
+        return nimg * 4 * (n >> kA) * num_chunks(n);
+    }
+
+    // Item `block` -> (image, quadrant, group p, chunk), chunk fastest.
+    __host__ __device__ __forceinline__ static Item item(const T *y_base, T *Z_base, int n,
+                                                         int first_image, int block) {
+        Item it;
+        it.n = n;
+        it.C = 2 * n - 1;
+        it.PZ = z_pitch(n);
+        const int nP = n >> kA, nchunk = num_chunks(n);
+        const int chunk = block % nchunk;
+        block /= nchunk;
+        it.p = block % nP;
+        block /= nP;
+        const int q = block & 3;
+        const int b = first_image + (block >> 2);
+        it.c0 = chunk * kOut;
+        it.y = y_base + ((size_t)b * 4 + q) * it.C * (size_t)n;
+        it.Z = Z_base + (((size_t)(b - first_image) * 4 + q) * nP + it.p) * ((size_t)1 << kA) * it.PZ +
+               kZL;
+        return it;
+    }
+
+    // Level-1 strip g: input rows c0 + 4*lane + i (row C reads -0.0, rows
+    // above +0.0), angles p * 2^kA + 8g + v for v in [0, 8).  All loads are
+    // issued before the padding is applied.
+    __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
+                                                          T (&v)[R0][kW]) {
+        const int a0 = (it.p << kA) + 8 * g;
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int r = it.c0 + kW * lane + i;
+            const int rc = r < it.C ? r : it.C - 1;
+            const T *src = it.y + unsigned(rc * it.n + a0);
+            load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
+            load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
+        }
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int r = it.c0 + kW * lane + i;
+            if (r >= it.C) {
+                const T pad = r == it.C ? neg_zero<T>() : T(0);
+#pragma unroll
+                for (int j = 0; j < R0; ++j) v[j][i] = pad;
+            }
+        }
+    }
+
+    // Section s at strip offsets 4*lane + i (lanes 0..47, the chunk's kOut
+    // offsets) -> Z rows c0 + 4*lane + i - p*s; rows from C on get the
+    // margin values.  Windows holding no row of [0, C + 5) are skipped; the
+    // others stay inside the row's scratch ends.
+    __host__ __device__ __forceinline__ static void store_z(const Item &it, int s, int lane,
+                                                            const T (&o)[kW]) {
+        if (lane >= kOut / kW) return;
+        const int r0 = it.c0 + kW * lane - it.p * s;
+        if (r0 < -3 || r0 > it.C + 4) return;
+        T v[kW];
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int r = r0 + i;
+            v[i] = r < it.C ? o[i] : (r == it.C ? neg_zero<T>() : T(0));
+        }
+        store_vec4(it.Z + (size_t)s * it.PZ + r0, v[0], v[1], v[2], v[3]);
+    }
+
+    __host__ __device__ __forceinline__ static void store0(const Item &it, int g, int lane,
+                                                           const T (&o)[R0][kW], T *lds) {
+        if constexpr (K2 == 0) {
+            (void)lds;
+            (void)g;
+#pragma unroll
+            for (int t = 0; t < R0; ++t) store_z(it, t, lane, o[t]);
+        } else {
+            (void)it;
+#pragma unroll
+            for (int t = 0; t < R0; ++t)
+#pragma unroll
+                for (int i = 0; i < kW; ++i) lds[(g * R0 + t) * kRow + lds_col(i) + lane] = o[t][i];
+        }
+    }
+
+    // Level-2 strip s1: input g is level-1 section s1 of strip g at strip
+    // offset + g*s1 (the relative skew of the groups R1*p + g).
+    __host__ __device__ __forceinline__ static void load1(const Item &it, int s1, int lane,
+                                                          const T *lds, T (&v)[R1][kW]) {
+        (void)it;
+#pragma unroll
+        for (int g = 0; g < R1; ++g) {
+            const T *row = lds + (g * R0 + s1) * kRow + lane;
+#pragma unroll
+            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_col(i + g * s1)];
+        }
+    }
+
+    // Level-2 output s2 of strip s1 is section R1*s1 + s2; the strip's
+    // offsets are rows c0 + offset - R1*p*s1 before the section skew p*s2.
+    __host__ __device__ __forceinline__ static void store1(const Item &it, int s1, int lane,
+                                                           const T (&o)[R1][kW], T *lds) {
+        (void)lds;
+#pragma unroll
+        for (int s2 = 0; s2 < R1; ++s2) store_z(it, R1 * s1 + s2, lane, o[s2]);
+    }
+
+    __host__ __device__ __forceinline__ static void phase2(const Item &, const T *, int) {}
+};
+
+template <typename T, int K4>
+struct BdrtB {
+    static constexpr int kB = 3 + K4;
+    static constexpr int R0 = 8, R1 = 1 << K4;
+    static constexpr int A2 = 1 << kB;  // output columns per tile row
+    static constexpr int G = A2 / 4;    // 4-element groups per tile row
+    static constexpr int tasks0 = R1;                // level-3 strips g'
+    static constexpr int tasks1 = K4 == 0 ? 0 : 8;  // level-4 strips s3
+    static constexpr size_t lc_elems = K4 == 0 ? 0 : size_t(R1) * 8 * kRow;
+    static constexpr size_t tile_elems = size_t(kOut) * A2;
+    static constexpr size_t lds_elems = lc_elems > tile_elems ? lc_elems : tile_elems;
+
+#ifndef ADRT_HOST_EMULATION
+    __device__ __forceinline__ static void strip0(T (&v)[R0][kW]) { strip_bdrt<3, T>(v); }
+    __device__ __forceinline__ static void strip1(T (&v)[R1][kW]) { strip_bdrt<K4, T>(v); }
+#endif
+    static void strip0_host(T (*v)[R0][kW]) { strip_bdrt_host<3, T>(v); }
+    static void strip1_host(T (*v)[R1][kW]) { strip_bdrt_host<K4, T>(v); }
+
+    struct Item {
+        const T *Z;  // row 0 of (group 0, section s); group p at + p * pstride
+        T *out;      // output plane (C, n)
+        int n, C, d0, abase;
+        size_t pstride;
+    };
+
+    __host__ __device__ __forceinline__ static int num_chunks(int n) {
+        return (2 * n - 1 + kOut - 1) / kOut;
+    }
+    __host__ __device__ __forceinline__ static int num_items(int n, int kA, int nimg) {
+        return nimg * 4 * (1 << kA) * num_chunks(n);
+    }
+
+    // Item `block` -> (image, quadrant, section s, chunk), chunk fastest.
+    __host__ __device__ __forceinline__ static Item item(const T *Z_base, T *out_base, int n, int kA,
+                                                         int first_image, int block) {
+        Item it;
+        it.n = n;
+        it.C = 2 * n - 1;
+        const int PZ = z_pitch(n);
+        const int nS = 1 << kA, nchunk = num_chunks(n);
+        const int chunk = block % nchunk;
+        block /= nchunk;
+        const int s = block % nS;
+        block /= nS;
+        const int q = block & 3;
+        const int b = first_image + (block >> 2);
+        it.d0 = chunk * kOut;
+        it.pstride = (size_t)nS * PZ;
+        it.Z = Z_base + ((size_t)(b - first_image) * 4 + q) * (size_t)n * PZ + (size_t)s * PZ + kZL;
+        it.out = out_base + ((size_t)b * 4 + q) * it.C * (size_t)n;
+        it.abase = s << kB;
+        return it;
+    }
+
+    // Level-3 strip g': Z rows d0 + 4*lane + i of groups 8g' + v.  A window
+    // starting past C + 1 is read from C + 1 (all +0.0, as every row past C).
+    __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
+                                                          T (&v)[R0][kW]) {
+        int c = it.d0 + kW * lane;
+        c = c > it.C + 1 ? it.C + 1 : c;
+#pragma unroll
+        for (int j = 0; j < R0; ++j)
+            load_vec4(it.Z + (size_t)(8 * g + j) * it.pstride + unsigned(c), v[j][0], v[j][1], v[j][2],
+                      v[j][3]);
+    }
+
+    // Swizzled tile position of (row d, column a), as in the forward's tile.
+    __host__ __device__ __forceinline__ static int tile_pos(int d, int a) {
+        return d * A2 + 4 * ((a >> 2) ^ ((d >> 2) & (G - 1))) + (a & 3);
+    }
+
+    template <int R>
+    __host__ __device__ __forceinline__ static void store_tile(int a0, int lane, const T (&o)[R][kW],
+                                                               T *tile) {
+        if (lane >= kOut / kW) return;
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int d = kW * lane + i;
+#pragma unroll
+            for (int t = 0; t < R; ++t) tile[tile_pos(d, a0 + t)] = o[t][i];
+        }
+    }
+
+    __host__ __device__ __forceinline__ static void store0(const Item &it, int g, int lane,
+                                                           const T (&o)[R0][kW], T *lds) {
+        (void)it;
+        if constexpr (K4 == 0) {
+            (void)g;
+            store_tile<R0>(0, lane, o, lds);
+        } else {
+#pragma unroll
+            for (int t = 0; t < R0; ++t)
+#pragma unroll
+                for (int i = 0; i < kW; ++i) lds[(g * R0 + t) * kRow + lds_col(i) + lane] = o[t][i];
+        }
+    }
+
+    // Level-4 strip s3: input g' is level-3 section s3 of strip g' at strip
+    // offset + g'*s3.
+    __host__ __device__ __forceinline__ static void load1(const Item &it, int s3, int lane,
+                                                          const T *lds, T (&v)[R1][kW]) {
+        (void)it;
+#pragma unroll
+        for (int g = 0; g < R1; ++g) {
+            const T *row = lds + (g * R0 + s3) * kRow + lane;
+#pragma unroll
+            for (int i = 0; i < kW; ++i) v[g][i] = row[lds_col(i + g * s3)];
+        }
+    }
+
+    __host__ __device__ __forceinline__ static void store1(const Item &it, int s3, int lane,
+                                                           const T (&o)[R1][kW], T *lds) {
+        (void)it;
+        store_tile<R1>(s3 * R1, lane, o, lds);
+    }
+
+    // Tile rows d -> out[d0 + d][abase .. abase + A2), coalesced.
+    __host__ __device__ __forceinline__ static void phase2(const Item &it, const T *lds, int tid) {
+        const T *tile = lds;
+        for (int f = tid; f < kOut * G; f += kThreads) {
+            const int d = f / G, grp = f % G;
+            const int od = it.d0 + d;
+            if (od >= it.C) continue;
+            const T *p = tile + tile_pos(d, 4 * grp);
+            store_vec4_stream(it.out + (size_t)od * it.n + it.abase + 4 * grp, p[0], p[1], p[2], p[3]);
+        }
+    }
+};
+
+inline size_t z_bytes_per_image(int n, size_t elem) {
+    return size_t(4) * n * size_t(z_pitch(n)) * elem;
+}
+
+#ifndef ADRT_HOST_EMULATION
 
 // One item per work-group: phase 0 strips (tasks0 <= kWaves, one per
 // wave), barrier, phase 1 strips (exactly one per wave) loaded and computed
@@ -671,14 +1078,14 @@ __device__ __forceinline__ void run_item(int items, T *lds, A... args) {
     if (wave < P::tasks0) {
         T v[P::R0][kW];
         P::load0(it, wave, lane, v);
-        strip_adrt<log2_small(P::R0), T>(v);
+        P::strip0(v);
         P::store0(it, wave, lane, v, lds);
     }
     __syncthreads();
     if constexpr (P::tasks1 > 0) {
         T w[P::R1][kW];
         P::load1(it, wave, lane, lds, w);
-        strip_adrt<log2_small(P::R1), T>(w);
+        P::strip1(w);
         __syncthreads();
         P::store1(it, wave, lane, w, lds);
         __syncthreads();
@@ -717,6 +1124,27 @@ fwd_pass2(const T *__restrict__ X_base, T *__restrict__ out_base, int n, int k1,
     extern __shared__ __align__(16) unsigned char smem_raw[];
     run_item<Pass2<T, KD>, T>(items, reinterpret_cast<T *>(smem_raw), X_base, out_base, n, k1,
                                first_image);
+}
+template <typename T, int K2>
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                               amdgpu_waves_per_eu(lds_waves_per_simd<BdrtA<T, K2>, T>(
+                                   sizeof(T) == 4 ? 8 : 4))))
+bdrt_passA(const T *__restrict__ y_base, T *__restrict__ Z_base, int n, int first_image,
+           int items) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    run_item<BdrtA<T, K2>, T>(items, reinterpret_cast<T *>(smem_raw), y_base, Z_base, n,
+                              first_image);
+}
+
+template <typename T, int K4>
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                               amdgpu_waves_per_eu(lds_waves_per_simd<BdrtB<T, K4>, T>(
+                                   sizeof(T) == 4 ? 8 : 4))))
+bdrt_passB(const T *__restrict__ Z_base, T *__restrict__ out_base, int n, int kA,
+           int first_image, int items) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    run_item<BdrtB<T, K4>, T>(items, reinterpret_cast<T *>(smem_raw), Z_base, out_base, n, kA,
+                              first_image);
 }
 #endif  // ADRT_HOST_EMULATION
 
@@ -793,6 +1221,45 @@ static void run_group(const T *img, T *X, T *out, int n, int L, int first, int c
         default: launch_pass2<T, 3>(X, out, n, k1, first, count, s); break;
     }
 }
+
+template <typename T, int K2>
+static void launch_bdrt_a(const T *y, T *Z, int n, int first, int count, hipStream_t s) {
+    using P = BdrtA<T, K2>;
+    const int items = P::num_items(n, count);
+    constexpr size_t lds = P::lds_elems * sizeof(T);
+    auto kern = bdrt_passA<T, K2>;
+    const unsigned grid = launch_grid(kern, lds, items);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, y, Z, n, first, items);
+}
+
+template <typename T, int K4>
+static void launch_bdrt_b(const T *Z, T *out, int n, int kA, int first, int count, hipStream_t s) {
+    using P = BdrtB<T, K4>;
+    const int items = P::num_items(n, kA, count);
+    constexpr size_t lds = P::lds_elems * sizeof(T);
+    auto kern = bdrt_passB<T, K4>;
+    const unsigned grid = launch_grid(kern, lds, items);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, Z, out, n, kA, first, items);
+}
+
+// bdrt split: pass A takes kA = floor(L/2) stages, pass B kB = ceil(L/2).
+template <typename T>
+static void run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int first, int count,
+                           hipStream_t s) {
+    const int kA = L / 2;
+    switch (kA - 3) {
+        case 0: launch_bdrt_a<T, 0>(y, Z, n, first, count, s); break;
+        case 1: launch_bdrt_a<T, 1>(y, Z, n, first, count, s); break;
+        case 2: launch_bdrt_a<T, 2>(y, Z, n, first, count, s); break;
+        default: launch_bdrt_a<T, 3>(y, Z, n, first, count, s); break;
+    }
+    switch (L - kA - 3) {
+        case 0: launch_bdrt_b<T, 0>(Z, out, n, kA, first, count, s); break;
+        case 1: launch_bdrt_b<T, 1>(Z, out, n, kA, first, count, s); break;
+        case 2: launch_bdrt_b<T, 2>(Z, out, n, kA, first, count, s); break;
+        default: launch_bdrt_b<T, 3>(Z, out, n, kA, first, count, s); break;
+    }
+}
 #endif  // ADRT_HOST_EMULATION
 
 }  // namespace fused
@@ -828,6 +1295,33 @@ int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n
                            static_cast<float *>(ws), B, n, s);
     return run_fused_t(static_cast<const double *>(in), static_cast<double *>(out),
                        static_cast<double *>(ws), B, n, s);
+}
+
+size_t fused_bdrt_workspace(int dtype, size_t B, size_t n) {
+    const size_t elem = dtype == ADRT_HIP_F64 ? 8 : 4;
+    const size_t per = fused::z_bytes_per_image(int(n), elem);
+    return fused::group_images(B, per) * per;
+}
+
+template <typename T>
+static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
+    const int L = num_iters(n);
+    const size_t per = fused::z_bytes_per_image(int(n), sizeof(T));
+    const size_t G = fused::group_images(B, per);
+    for (size_t first = 0; first < B; first += G) {
+        const size_t count = (B - first) < G ? (B - first) : G;
+        fused::run_bdrt_group<T>(in, ws, out, int(n), L, int(first), int(count), s);
+    }
+    return launch_status();
+}
+
+int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
+                   hipStream_t s) {
+    if (dtype == ADRT_HIP_F32)
+        return run_fused_bdrt_t(static_cast<const float *>(in), static_cast<float *>(out),
+                                static_cast<float *>(ws), B, n, s);
+    return run_fused_bdrt_t(static_cast<const double *>(in), static_cast<double *>(out),
+                            static_cast<double *>(ws), B, n, s);
 }
 #endif  // ADRT_HOST_EMULATION
 

@@ -8,7 +8,7 @@
 // exactly as the library's workspace, so any out-of-bounds access aborts
 // under AddressSanitizer and any read of an unwritten or invalid slot shows
 // up as NaN in the result.  Usage:
-//   fused_emu f32|f64 B n in.bin out.bin [slots]
+//   fused_emu f32|f64 B n in.bin out.bin [slots] [fwd|bdrt]
 // `slots` (a multiple of 8) caps the grid so that work-groups run several
 // items each; 0 (default) is the library's one-item-per-work-group grid.
 #include <cmath>
@@ -23,7 +23,6 @@
 using namespace adrt_hip;
 using namespace adrt_hip::fused;
 
-constexpr int log2i(int r) { return r == 8 ? 3 : (r == 4 ? 2 : (r == 2 ? 1 : 0)); }
 
 // Runs the persistent schedule of `grid` work-groups over `items` items, one
 // item at a time, exactly as the kernels' loop assigns them.
@@ -43,7 +42,7 @@ static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args.
             T *l = P::lds_elems > 0 ? lds.data() : nullptr;
             for (int task = 0; task < P::tasks0; ++task) {
                 for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
-                strip_adrt_host<log2i(P::R0), T>(v0);
+                P::strip0_host(v0);
                 for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
             }
             // Phase 1: every strip loads and computes before any stores (the
@@ -51,7 +50,7 @@ static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args.
             std::vector<T> keep(size_t(P::tasks1 > 0 ? P::tasks1 : 1) * buf1.size());
             for (int task = 0; task < P::tasks1; ++task) {
                 for (int lane = 0; lane < kLanes; ++lane) P::load1(it, task, lane, l, v1[lane]);
-                strip_adrt_host<log2i(P::R1), T>(v1);
+                P::strip1_host(v1);
                 std::memcpy(keep.data() + size_t(task) * buf1.size(), buf1.data(), buf1.size() * sizeof(T));
             }
             for (int task = 0; task < P::tasks1; ++task) {
@@ -95,6 +94,56 @@ static int emulate_group(const T *img, T *X, T *out, int n, int first, int count
     return 0;
 }
 
+template <typename T, int K2, int K4>
+static int emulate_bdrt_group(const T *y, T *Z, T *out, int n, int first, int count) {
+    using PA = BdrtA<T, K2>;
+    using PB = BdrtB<T, K4>;
+    std::vector<int> seen;
+    const int itemsA = PA::num_items(n, count);
+    emulate_grid<PA, T>(itemsA, grid_for_slots(itemsA, g_slots), seen, y, Z, n, first);
+    if (!once_each(seen)) return 5;
+    const int itemsB = PB::num_items(n, PA::kA, count);
+    emulate_grid<PB, T>(itemsB, grid_for_slots(itemsB, g_slots), seen, (const T *)Z, out, n, PA::kA,
+                        first);
+    if (!once_each(seen)) return 6;
+    return 0;
+}
+
+// Back-projection: the input is (B, 4, 2n-1, n), as the output.
+template <typename T>
+static int run_bdrt(int B, int n, const char *fin, const char *fout) {
+    const size_t elems = size_t(B) * 4 * (2 * n - 1) * n;
+    std::vector<T> y(elems), out(elems, T(NAN));
+    FILE *f = std::fopen(fin, "rb");
+    if (!f || std::fread(y.data(), sizeof(T), elems, f) != elems) return 2;
+    std::fclose(f);
+    const int L = num_iters(size_t(n));
+    const size_t per = z_bytes_per_image(n, sizeof(T));
+    const size_t G = group_images(size_t(B), per);
+    std::vector<T> Z(G * per / sizeof(T), T(NAN));
+    const int kA = L / 2;
+    for (int first = 0; first < B; first += int(G)) {
+        const int count = (B - first) < int(G) ? (B - first) : int(G);
+        const int code = (kA - 3) * 4 + (L - kA - 3);
+        int st = 3;
+        switch (code) {
+            case 0: st = emulate_bdrt_group<T, 0, 0>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 1: st = emulate_bdrt_group<T, 0, 1>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 5: st = emulate_bdrt_group<T, 1, 1>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 6: st = emulate_bdrt_group<T, 1, 2>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 10: st = emulate_bdrt_group<T, 2, 2>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 11: st = emulate_bdrt_group<T, 2, 3>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 15: st = emulate_bdrt_group<T, 3, 3>(y.data(), Z.data(), out.data(), n, first, count); break;
+            default: break;
+        }
+        if (st != 0) return st;
+    }
+    f = std::fopen(fout, "wb");
+    if (!f || std::fwrite(out.data(), sizeof(T), elems, f) != elems) return 4;
+    std::fclose(f);
+    return 0;
+}
+
 template <typename T>
 static int run(int B, int n, const char *fin, const char *fout) {
     const size_t in_elems = size_t(B) * n * n;
@@ -131,12 +180,14 @@ static int run(int B, int n, const char *fin, const char *fout) {
 }
 
 int main(int argc, char **argv) {
-    if (argc != 6 && argc != 7) {
-        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [slots]\n", argv[0]);
+    if (argc < 6 || argc > 8) {
+        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [slots] [fwd|bdrt]\n", argv[0]);
         return 1;
     }
-    if (argc == 7) g_slots = unsigned(std::atoi(argv[6]));
+    if (argc >= 7) g_slots = unsigned(std::atoi(argv[6]));
+    const bool bdrt = argc == 8 && std::strcmp(argv[7], "bdrt") == 0;
     const int B = std::atoi(argv[2]), n = std::atoi(argv[3]);
-    if (std::strcmp(argv[1], "f32") == 0) return run<float>(B, n, argv[4], argv[5]);
-    return run<double>(B, n, argv[4], argv[5]);
+    if (std::strcmp(argv[1], "f32") == 0)
+        return bdrt ? run_bdrt<float>(B, n, argv[4], argv[5]) : run<float>(B, n, argv[4], argv[5]);
+    return bdrt ? run_bdrt<double>(B, n, argv[4], argv[5]) : run<double>(B, n, argv[4], argv[5]);
 }

@@ -14,6 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from helpers import canonical
 from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -35,16 +36,36 @@ def emulator(tmp_path_factory):
     return exe
 
 
-def run_emu(exe, x, tmp_path, slots=0):
-    b, n, _ = x.shape
+def run_emu(exe, x, tmp_path, slots=0, op="fwd"):
+    b, n = x.shape[0], x.shape[-1]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     x.tofile(fin)
     tag = "f32" if x.dtype == np.float32 else "f64"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout), str(slots)],
+    res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout), str(slots), op],
                          capture_output=True, text=True, env=env)
     assert res.returncode == 0, res.stderr[-4000:]
     return np.fromfile(fout, dtype=x.dtype).reshape(b, 4, 2 * n - 1, n)
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024])
+def test_fused_bdrt_emulation(emulator, tmp_path, dt, n):
+    """Fused back-projection (both passes) on the CPU: equal to the oracle
+    after canonicalising the sign of zero, including Inf/NaN propagation."""
+    rng = np.random.default_rng(n + 1)
+    y = (rng.standard_normal((1, 4, 2 * n - 1, n)) * 10).astype(dt)
+    y[0, 0, 0, :4] = [-0.0, np.inf, -np.inf, np.nan]
+    y[0, 1, -1, :3] = [-0.0, -0.0, 0.0]
+    got = run_emu(emulator, y, tmp_path, op="bdrt")
+    assert np.array_equal(canonical(got).view(np.uint8), canonical(O.bdrt(y)).view(np.uint8))
+
+
+@pytest.mark.parametrize("slots", [8, 24])
+def test_fused_bdrt_emulation_capped_grid(emulator, tmp_path, slots):
+    y = np.random.default_rng(slots).standard_normal((3, 4, 127, 64))
+    got = run_emu(emulator, y, tmp_path, slots, op="bdrt")
+    assert np.array_equal(canonical(got).view(np.uint8), canonical(O.bdrt(y)).view(np.uint8))
 
 
 @pytest.mark.parametrize("dt", ["float32", "float64"])
