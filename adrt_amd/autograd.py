@@ -1,0 +1,139 @@
+"""Differentiable ADRT for torch device tensors, and least squares through it.
+
+* ``ADRTFunction`` (``adrt``): forward = the fused forward ADRT; backward =
+  the adjoint A^T g = sum_q truncate(bdrt(g))_q, computed by the fused
+  back-projection and the quadrant-sum kernel (the reference's adjoint
+  identity, tests/test_bdrt.py:310-325).
+* ``BDRTFunction`` (``bdrt``): forward = bdrt; backward = the forward stage
+  chain adrt_step(0), ..., adrt_step(L-1) on the ADRT-layout buffer, since
+  bdrt_step(k) is the transpose of adrt_step(L-1-k)
+  (tests/test_bdrt_step.py:76-92).
+* ``lsq_cg``: minimises 0.5 * ||adrt(x) - b||^2 by conjugate gradients on the
+  normal equations (CGLS); each A^T application is the autograd backward of
+  ``adrt``, i.e. the hand-written back-projection kernel.  Inner products
+  and the CG scalars are float64 on the device, per image of a batch.
+
+All of it runs on the GPU; there is no CPU path.
+"""
+
+from . import _device as dev
+from . import _kernels
+from .core import num_iters
+
+
+def _torch():
+    return dev.torch()
+
+
+def _adjoint(g):
+    """A^T g = sum over quadrants of truncate(bdrt(g)): (..., 4, 2n-1, n) -> (..., n, n)."""
+    return _kernels.truncate_combine(_kernels.bdrt(g.contiguous()), 1.0, _kernels.SUM)
+
+
+def _stage_chain(h):
+    """adrt_step(0) then ... adrt_step(L-1) on an ADRT-layout buffer."""
+    out = h.contiguous()
+    for k in range(num_iters(int(h.shape[-1]))):
+        out = _kernels.adrt_step(out, k)
+    return out
+
+
+def _function_classes():
+    torch = _torch()
+
+    class ADRTFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x):
+            return _kernels.adrt(x.contiguous())
+
+        @staticmethod
+        def backward(ctx, g):
+            return _adjoint(g)
+
+    class BDRTFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, y):
+            return _kernels.bdrt(y.contiguous())
+
+        @staticmethod
+        def backward(ctx, g):
+            return _stage_chain(g)
+
+    return ADRTFunction, BDRTFunction
+
+
+_classes = None
+
+
+def _get_classes():
+    global _classes
+    if _classes is None:
+        _classes = _function_classes()
+    return _classes
+
+
+def adrt(x):
+    """Differentiable forward ADRT of a device tensor (..., n, n)."""
+    dev.require_gpu()
+    return _get_classes()[0].apply(x)
+
+
+def bdrt(y):
+    """Differentiable back-projection of a device tensor (..., 4, 2n-1, n)."""
+    dev.require_gpu()
+    return _get_classes()[1].apply(y)
+
+
+def _bdot(u, v):
+    """Per-image float64 inner product of two (B, ...) tensors."""
+    return (u.double() * v.double()).reshape(u.shape[0], -1).sum(dim=1)
+
+
+def _apply_adjoint(x, r):
+    """A^T r via the autograd backward of adrt at x (hand-written bdrt)."""
+    torch = _torch()
+    xv = x.detach().requires_grad_(True)
+    with torch.enable_grad():
+        y = adrt(xv)
+        (g,) = torch.autograd.grad(y, xv, grad_outputs=r)
+    return g
+
+
+def lsq_cg(b, x0=None, *, iters=200, callback=None):
+    """Least squares min_x 0.5 * ||adrt(x) - b||^2 by CGLS for a batch of
+    ADRT data b (B, 4, 2n-1, n) (or unbatched).  Runs exactly ``iters``
+    iterations (stopping early only if an image's gradient vanishes).
+    Returns ``(x, loss_history)``; the history holds per-image losses (float64
+    tensors) before the first and after every iteration."""
+    torch = _torch()
+    dev.require_gpu()
+    single = b.dim() == 3
+    bb = (b.unsqueeze(0) if single else b).contiguous()
+    n = int(bb.shape[-1])
+    x = torch.zeros(bb.shape[0], n, n, dtype=bb.dtype, device=bb.device) if x0 is None else (
+        (x0.unsqueeze(0) if x0.dim() == 2 else x0).to(bb.dtype).clone())
+    r = bb - adrt(x).detach()
+    s = _apply_adjoint(x, r)
+    p = s.clone()
+    gamma = _bdot(s, s)
+    history = [0.5 * _bdot(r, r)]
+    for _ in range(iters):
+        if not bool((gamma > 0).any()):
+            break
+        q = adrt(p).detach()
+        qq = _bdot(q, q)
+        alpha = torch.where(qq > 0, gamma / torch.where(qq > 0, qq, torch.ones_like(qq)),
+                            torch.zeros_like(qq))
+        a = alpha.to(bb.dtype)
+        x = x + a[:, None, None] * p
+        r = r - a[:, None, None, None] * q
+        s = _apply_adjoint(x, r)
+        gamma_new = _bdot(s, s)
+        beta = torch.where(gamma > 0, gamma_new / torch.where(gamma > 0, gamma, torch.ones_like(gamma)),
+                           torch.zeros_like(gamma))
+        p = s + beta.to(bb.dtype)[:, None, None] * p
+        gamma = gamma_new
+        history.append(0.5 * _bdot(r, r))
+        if callback is not None:
+            callback(x[0] if single else x)
+    return (x[0] if single else x), history
