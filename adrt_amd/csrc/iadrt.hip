@@ -136,33 +136,41 @@ __global__ void __launch_bounds__(kLanes)
         return *reinterpret_cast<const P2 *>(src + size_t(r) * n + (ln.live ? ln.pc : 0));
     };
 
+    // Row s of the wave's columns is flushed at step s.  Every lane stores at
+    // every step, so each block issues exactly kSteps stores after its
+    // prefetch and the wait for the prefetched pairs leaves them in flight:
+    // steps above the last row store junk into row R-1, which its real value
+    // overwrites at step R-1 (same lane, same address, later); lanes past n
+    // repeat the last live lane's store (same value, same address).
+    const int flush_lane = store_lane ? lane : n - 1 - w.ch0;
+    T *flush_col = dst + w.ch0 + flush_lane;
+    auto run_step = [&](int sk, const P2 &p, T &acc, T &prev1) {
+        T val;
+        int row;
+        if (step<T>(w, ln, sk, p.x, p.y, prev1, acc, val, row)) ring[row % kRing][lane] = val;
+        prev1 = p.y;
+        const int fr = sk < R ? sk : R - 1;
+        flush_col[size_t(fr) * n] = ring[sk % kRing][flush_lane];
+    };
+
     T acc = T(0);
     T prev1 = T(0);
-    P2 cur[kSteps], nxt[kSteps];
+    P2 cur[kSteps];
     int s = w.top;
 #pragma unroll
     for (int k = 0; k < kSteps; ++k) cur[k] = load(s - k);
-    for (; s >= 0; s -= kSteps) {
-        const bool more = s - kSteps >= 0;
-        if (more) {
+    for (; s >= kSteps - 1; s -= kSteps) {  // whole blocks
+        P2 nxt[kSteps];
 #pragma unroll
-            for (int k = 0; k < kSteps; ++k) nxt[k] = load(s - kSteps - k);
-        }
+        for (int k = 0; k < kSteps; ++k) nxt[k] = load(s - kSteps - k);
 #pragma unroll
-        for (int k = 0; k < kSteps; ++k) {
-            const int sk = s - k;
-            if (sk < 0) break;
-            T val;
-            int row;
-            if (step<T>(w, ln, sk, cur[k].x, cur[k].y, prev1, acc, val, row))
-                ring[row % kRing][lane] = val;
-            prev1 = cur[k].y;
-            if (sk < R && store_lane) dst[size_t(sk) * n + w.ch0 + lane] = ring[sk % kRing][lane];
-        }
-        if (!more) break;
+        for (int k = 0; k < kSteps; ++k) run_step(s - k, cur[k], acc, prev1);
 #pragma unroll
         for (int k = 0; k < kSteps; ++k) cur[k] = nxt[k];
     }
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k)  // tail: steps s..0
+        if (k <= s) run_step(s - k, cur[k], acc, prev1);
 }
 #endif  // ADRT_HOST_EMULATION
 

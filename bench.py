@@ -57,7 +57,11 @@ def parse():
                    help="print a summary of rocprofv3 csv output directories (kernel stats "
                         "and/or FETCH_SIZE / WRITE_SIZE counter runs of --profile)")
     p.add_argument("--calls", type=int, default=4,
-                   help="forward calls in the profiled run (--summarize)")
+                   help="calls in the profiled run (--summarize)")
+    p.add_argument("--match", default="adrt_hip",
+                   help="count only kernels whose name contains this (--summarize)")
+    p.add_argument("--op", default="adrt", choices=["adrt", "bdrt", "iadrt"],
+                   help="operation a --profile run calls (bdrt/iadrt on the forward output)")
     return p.parse_args()
 
 
@@ -136,11 +140,11 @@ def _short(name):
     return name.replace("adrt_hip::fused::", "")
 
 
-def summarize(dirs, calls):
+def summarize(dirs, calls, match="adrt_hip"):
     """Kernel durations and calibrated HBM traffic per forward call from
     rocprofv3 csv output (kernel stats and FETCH_SIZE / WRITE_SIZE runs)."""
     lines = []
-    stats = [r for r in _csv_rows(dirs, "kernel_stats.csv") if "adrt_hip" in r["Name"]]
+    stats = [r for r in _csv_rows(dirs, "kernel_stats.csv") if match in r["Name"]]
     if stats:
         lines += ["| kernel | calls | avg µs | min µs | max µs | total µs |",
                   "|---|---|---|---|---|---|"]
@@ -155,7 +159,7 @@ def summarize(dirs, calls):
     for r in _csv_rows(dirs, "counter_collection.csv"):
         name = r.get("Kernel_Name", "")
         key = ("copy" if "copyBuffer" in name else _short(name)) if (
-            "adrt_hip" in name or "copyBuffer" in name) else None
+            match in name or "copyBuffer" in name) else None
         if key is None:
             continue
         ctr.setdefault(key, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
@@ -201,7 +205,7 @@ def profiled_traffic():
 def main():
     args = parse()
     if args.summarize:
-        print(summarize(args.summarize, args.calls))
+        print(summarize(args.summarize, args.calls, args.match))
         return
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.extras = 3, 1, True, False
@@ -238,6 +242,26 @@ def main():
         st = lib.adrt_hip_adrt(pin, pout, pws, ws_bytes, code, batch, n, sp)
         if st != 0:
             _lib.check(st, "adrt_hip_adrt")
+
+    if args.profile and args.op != "adrt":
+        # Profile bdrt or iadrt on the forward's output: 1 warm-up + 3 calls,
+        # then the calibration copy.
+        fwd()
+        op_id = {"bdrt": _lib.OP_BDRT, "iadrt": _lib.OP_IADRT}[args.op]
+        y = out
+        z = torch.empty_like(y)
+        wb = lib.adrt_hip_workspace_bytes(op_id, code, batch, n)
+        wst = torch.empty(max(wb, 1), device=device, dtype=torch.uint8)
+        fn_c = getattr(lib, f"adrt_hip_{args.op}")
+        call = (ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(z.data_ptr()),
+                ctypes.c_void_p(wst.data_ptr()), wb, code, batch, n, sp)
+        _, ms = time_calls(lambda: fn_c(*call), 3, 1, stream)
+        a = torch.empty(CALIB_BYTES, device=device, dtype=torch.uint8)
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"op": args.op, "ms_per_call": round(ms, 4),
+                          "imgs_per_s": round(batch / ms * 1e3, 2)}), flush=True)
+        return
 
     wall, dev_ms = time_calls(fwd, args.steps, args.warmup, stream)
     value = batch * args.steps / wall

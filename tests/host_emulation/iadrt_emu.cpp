@@ -3,7 +3,7 @@
 // Runs every wavefront of every stage on the CPU with the kernel's own
 // per-lane step function: lanes advance in lockstep, step by step, through a
 // 64-row ring exactly sized as the kernel's LDS ring (poisoned with NaN), and
-// row s is flushed at step s.  Pair loads use the kernel's clamped rows.
+// row s is flushed at step s with the kernel's unconditional stores.  Pair loads use the kernel's clamped rows.
 // Buffers are exactly sized, so an out-of-bounds access aborts under
 // AddressSanitizer.  Usage:  iadrt_emu f32|f64 B n in.bin out.bin
 #include <cmath>
@@ -48,9 +48,14 @@ static void emulate_stage(const std::vector<T> &in, std::vector<T> &out, int pla
                         ring[size_t(row % kRing) * (kLanes + 1) + l] = val;
                     prev1[l] = p1;
                 }
-                for (int l = 0; l < kLanes; ++l)
-                    if (s < R && w.ch0 + l < n)
-                        dst[size_t(s) * n + w.ch0 + l] = ring[size_t(s % kRing) * (kLanes + 1) + l];
+                // the kernel's unconditional flush: steps above the last row
+                // write row R-1 (overwritten at step R-1), lanes past n repeat
+                // the last live lane
+                for (int l = 0; l < kLanes; ++l) {
+                    const int fl = w.ch0 + l < n ? l : n - 1 - w.ch0;
+                    const int fr = s < R ? s : R - 1;
+                    dst[size_t(fr) * n + w.ch0 + fl] = ring[size_t(s % kRing) * (kLanes + 1) + fl];
+                }
             }
         }
     }
