@@ -28,6 +28,10 @@ ERR_WORKSPACE = 6
 ERR_DEVICE = 7
 ERR_ARG = 8
 
+VARIANT_DEFAULT = 0
+VARIANT_ALT_SPLIT = 1
+VARIANT_STREAMING = 2
+
 _SZ = ctypes.c_size_t
 _VP = ctypes.c_void_p
 _INT = ctypes.c_int
@@ -59,6 +63,8 @@ SIGNATURES = {
     "adrt_hip_adrt_init": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_truncate": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_truncate_combine": (_INT, [_VP, _VP, _INT, _SZ, _SZ, ctypes.c_double, _INT, _VP]),
+    "adrt_hip_set_config": (_INT, [_INT, _SZ, _INT, _SZ]),
+    "adrt_hip_get_config": (_INT, [_INT, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_SZ)]),
 }
 
 _lib = None

@@ -61,7 +61,7 @@ const char *adrt_hip_status_string(int status) {
 
 const char *adrt_hip_last_error(void) { return g_last_error; }
 
-int adrt_hip_abi_version(void) { return 100; }
+int adrt_hip_abi_version(void) { return 101; }
 
 int adrt_hip_device_count(void) {
     int n = 0;
@@ -241,6 +241,14 @@ int adrt_hip_truncate_combine(const void *in, void *out, int dtype, size_t batch
     if (!in || !out) return ADRT_HIP_ERR_NULL;
     return launch_truncate_combine(in, out, dtype, batch, n, divisor, mode,
                                    static_cast<hipStream_t>(stream));
+}
+
+int adrt_hip_set_config(int op, size_t n, int variant, size_t group_bytes) {
+    return set_fused_config(op, n, variant, group_bytes);
+}
+
+int adrt_hip_get_config(int op, size_t n, int *variant, size_t *group_bytes) {
+    return get_fused_config(op, n, variant, group_bytes);
 }
 
 }  // extern "C"

@@ -73,6 +73,11 @@ int launch_adrt(const void *in, void *out, void *ws, size_t ws_bytes, int dtype,
                 size_t N, hipStream_t s);
 size_t adrt_workspace(int dtype, size_t B, size_t N);
 bool fused_supported(size_t n);
+// Tuned configuration of the fused paths (op = ADRT_HIP_OP_ADRT or _BDRT).
+int fused_variant(int op, size_t n);
+bool use_fused(int op, size_t n);
+int set_fused_config(int op, size_t n, int variant, size_t group_bytes);
+int get_fused_config(int op, size_t n, int *variant, size_t *group_bytes);
 size_t fused_workspace(int dtype, size_t B, size_t n);
 int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, hipStream_t s);
 size_t fused_bdrt_workspace(int dtype, size_t B, size_t n);

@@ -32,6 +32,8 @@
 // Host emulation.  The per-lane load/store functions are __host__ __device__;
 // with ADRT_HOST_EMULATION defined only those and the whole-strip compute are
 // compiled, so tests/host_emulation can run every block on the CPU.
+#include <atomic>
+
 #include "common.hpp"
 
 namespace adrt_hip {
@@ -1148,16 +1150,24 @@ bdrt_passB(const T *__restrict__ Z_base, T *__restrict__ out_base, int n, int kA
 }
 #endif  // ADRT_HOST_EMULATION
 
-// Split of L = log2 n stages: pass 1 takes k1 = ceil(L/2) = 3 + KB stages,
-// pass 2 the remaining k2 = 3 + KD.
+// Split of L = log2 n stages.  Forward: pass 1 takes k1 = 3 + KB stages,
+// pass 2 the remaining k2 = 3 + KD; variant 0 has k1 = ceil(L/2), variant 1
+// k1 = floor(L/2) (the same for even L).  Back-projection: pass A takes
+// kA = floor(L/2) (variant 0) or ceil(L/2) (variant 1).  For L in [6, 12]
+// every pass keeps 3..6 stages.
+inline int fwd_pass1_stages(int L, int variant) { return variant == 1 ? L / 2 : (L + 1) / 2; }
+inline int bdrt_passA_stages(int L, int variant) { return variant == 1 ? (L + 1) / 2 : L / 2; }
+
+// Largest k1 of either variant: sizes X for both.
 inline int pass1_stages(int L) { return (L + 1) / 2; }
 
-// Images per group: keep one group's intermediate at or below ~136 MiB (two
-// 2048^2 fp32 images) so that pass 2 re-reads it largely from the 256 MB
-// Infinity Cache; the final output bypasses that cache (streaming stores).
-// Fewer, larger groups also mean fewer launches.
-inline size_t group_images(size_t B, size_t per_image_bytes) {
-    const size_t budget = size_t(136) << 20;
+// Images per group: by default keep one group's intermediate at or below
+// ~136 MiB (two 2048^2 fp32 images) so that pass 2 re-reads it largely from
+// the 256 MB Infinity Cache; the final output bypasses that cache (streaming
+// stores).  Fewer, larger groups also mean fewer launches.
+constexpr size_t kDefaultGroupBytes = size_t(136) << 20;
+inline size_t group_images(size_t B, size_t per_image_bytes,
+                           size_t budget = kDefaultGroupBytes) {
     size_t g = per_image_bytes >= budget ? 1 : budget / per_image_bytes;
     return g < B ? g : B;
 }
@@ -1205,9 +1215,8 @@ static void launch_pass2(const T *X, T *out, int n, int k1, int first, int count
 }
 
 template <typename T>
-static void run_group(const T *img, T *X, T *out, int n, int L, int first, int count,
+static void run_group(const T *img, T *X, T *out, int n, int L, int k1, int first, int count,
                       hipStream_t s) {
-    const int k1 = pass1_stages(L);
     switch (k1 - 3) {
         case 0: launch_pass1<T, 0>(img, X, n, first, count, s); break;
         case 1: launch_pass1<T, 1>(img, X, n, first, count, s); break;
@@ -1242,11 +1251,10 @@ static void launch_bdrt_b(const T *Z, T *out, int n, int kA, int first, int coun
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, Z, out, n, kA, first, items);
 }
 
-// bdrt split: pass A takes kA = floor(L/2) stages, pass B kB = ceil(L/2).
+// bdrt split: pass A takes kA stages, pass B kB = L - kA.
 template <typename T>
-static void run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int first, int count,
+static void run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int kA, int first, int count,
                            hipStream_t s) {
-    const int kA = L / 2;
     switch (kA - 3) {
         case 0: launch_bdrt_a<T, 0>(y, Z, n, first, count, s); break;
         case 1: launch_bdrt_a<T, 1>(y, Z, n, first, count, s); break;
@@ -1270,21 +1278,70 @@ bool fused_supported(size_t n) {
     return is_pow2(n) && L >= 6 && L <= 12;
 }
 
+// Per-(op, log2 n) configuration, set by the autotuner through the C ABI.
+namespace {
+struct Tuned {
+    std::atomic<int> variant{ADRT_HIP_VARIANT_DEFAULT};
+    std::atomic<size_t> group_bytes{0};
+};
+Tuned g_tuned[2][64];
+
+Tuned *tuned(int op, size_t n) {
+    if ((op != ADRT_HIP_OP_ADRT && op != ADRT_HIP_OP_BDRT) || !is_pow2(n)) return nullptr;
+    return &g_tuned[op][num_iters(n)];
+}
+
+size_t group_budget(int op, size_t n) {
+    const Tuned *t = tuned(op, n);
+    const size_t b = t ? t->group_bytes.load(std::memory_order_relaxed) : 0;
+    return b ? b : fused::kDefaultGroupBytes;
+}
+}  // namespace
+
+int fused_variant(int op, size_t n) {
+    const Tuned *t = tuned(op, n);
+    return t ? t->variant.load(std::memory_order_relaxed) : ADRT_HIP_VARIANT_DEFAULT;
+}
+
+bool use_fused(int op, size_t n) {
+    return fused_supported(n) && fused_variant(op, n) != ADRT_HIP_VARIANT_STREAMING;
+}
+
+int set_fused_config(int op, size_t n, int variant, size_t group_bytes) {
+    Tuned *t = tuned(op, n);
+    if (!t) return ADRT_HIP_ERR_ARG;
+    if (variant < ADRT_HIP_VARIANT_DEFAULT || variant > ADRT_HIP_VARIANT_STREAMING)
+        return ADRT_HIP_ERR_ARG;
+    if (variant == ADRT_HIP_VARIANT_ALT_SPLIT && !fused_supported(n)) return ADRT_HIP_ERR_ARG;
+    t->variant.store(variant, std::memory_order_relaxed);
+    t->group_bytes.store(group_bytes, std::memory_order_relaxed);
+    return ADRT_HIP_OK;
+}
+
+int get_fused_config(int op, size_t n, int *variant, size_t *group_bytes) {
+    const Tuned *t = tuned(op, n);
+    if (!t) return ADRT_HIP_ERR_ARG;
+    if (variant) *variant = t->variant.load(std::memory_order_relaxed);
+    if (group_bytes) *group_bytes = group_budget(op, n);
+    return ADRT_HIP_OK;
+}
+
 size_t fused_workspace(int dtype, size_t B, size_t n) {
     const size_t elem = dtype == ADRT_HIP_F64 ? 8 : 4;
     const int L = num_iters(n);
     const size_t per = fused::x_bytes_per_image(int(n), L, elem);
-    return fused::group_images(B, per) * per;
+    return fused::group_images(B, per, group_budget(ADRT_HIP_OP_ADRT, n)) * per;
 }
 
 template <typename T>
 static int run_fused_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
     const int L = num_iters(n);
+    const int k1 = fused::fwd_pass1_stages(L, fused_variant(ADRT_HIP_OP_ADRT, n));
     const size_t per = fused::x_bytes_per_image(int(n), L, sizeof(T));
-    const size_t G = fused::group_images(B, per);
+    const size_t G = fused::group_images(B, per, group_budget(ADRT_HIP_OP_ADRT, n));
     for (size_t first = 0; first < B; first += G) {
         const size_t count = (B - first) < G ? (B - first) : G;
-        fused::run_group<T>(in, ws, out, int(n), L, int(first), int(count), s);
+        fused::run_group<T>(in, ws, out, int(n), L, k1, int(first), int(count), s);
     }
     return launch_status();
 }
@@ -1300,17 +1357,18 @@ int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n
 size_t fused_bdrt_workspace(int dtype, size_t B, size_t n) {
     const size_t elem = dtype == ADRT_HIP_F64 ? 8 : 4;
     const size_t per = fused::z_bytes_per_image(int(n), elem);
-    return fused::group_images(B, per) * per;
+    return fused::group_images(B, per, group_budget(ADRT_HIP_OP_BDRT, n)) * per;
 }
 
 template <typename T>
 static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
     const int L = num_iters(n);
+    const int kA = fused::bdrt_passA_stages(L, fused_variant(ADRT_HIP_OP_BDRT, n));
     const size_t per = fused::z_bytes_per_image(int(n), sizeof(T));
-    const size_t G = fused::group_images(B, per);
+    const size_t G = fused::group_images(B, per, group_budget(ADRT_HIP_OP_BDRT, n));
     for (size_t first = 0; first < B; first += G) {
         const size_t count = (B - first) < G ? (B - first) : G;
-        fused::run_bdrt_group<T>(in, ws, out, int(n), L, int(first), int(count), s);
+        fused::run_bdrt_group<T>(in, ws, out, int(n), L, kA, int(first), int(count), s);
     }
     return launch_status();
 }

@@ -92,6 +92,52 @@ def time_calls(fn, steps, warmup, stream):
     return wall, dev_ms
 
 
+UNEVEN_SIDES = (32, 64, 128, 256, 512, 1024)
+
+
+def uneven_extra(steps, warmup):
+    """1000 float32 images, sides drawn from UNEVEN_SIDES with probability
+    proportional to 1/n (numpy default_rng(0)), pixels U[0, 1) (torch
+    generator seed 0): forward ADRT + back-projection of all of them,
+    autotuned per side and captured as one hipGraph; K graph replays timed."""
+    import numpy as np
+    import torch
+
+    from adrt_amd import batched
+
+    sides = np.array(UNEVEN_SIDES)
+    p = 1.0 / sides
+    sizes = np.random.default_rng(0).choice(sides, 1000, p=p / p.sum()).tolist()
+    t0 = time.perf_counter()
+    plan = batched.UnevenBatch(sizes)
+    tune_s = time.perf_counter() - t0
+    g = torch.Generator(device=plan.device)
+    g.manual_seed(0)
+    for n in plan.classes:
+        plan.images[n].uniform_(0.0, 1.0, generator=g)
+    plan.capture()
+    stream = torch.cuda.current_stream(plan.device)
+    wall, graph_ms = time_calls(plan.replay, steps, warmup, stream)
+    _, eager_ms = time_calls(plan.run, steps, warmup, stream)
+    algo = plan.algorithmic_bytes()
+    gbs = algo / (graph_ms * 1e-3) / 1e9
+    choice = {f"{t['op']}{t['n']}": [t["choice"]["variant"], t["choice"]["group_bytes"] >> 20]
+              for t in plan.tuning}
+    return {
+        "workload": "forward ADRT + bdrt of 1000 float32 images, n in {32..1024} with "
+                    "p ~ 1/n (seed 0), one hipGraph",
+        "counts": {str(n): plan.count[n] for n in plan.classes},
+        "graph_ms_per_pipeline": round(graph_ms, 3),
+        "eager_ms_per_pipeline": round(eager_ms, 3),
+        "imgs_per_s": round(1000 / graph_ms * 1e3, 1),
+        "algorithmic_bytes": algo,
+        "GBps_algorithmic": round(gbs, 1),
+        "frac_of_peak": round(gbs / PEAK_GBS, 4),
+        "autotune_s": round(tune_s, 2),
+        "tuned_variant_groupMiB": choice,
+    }
+
+
 def cpu_baseline(host_imgs, budget_s):
     """Time the reference's CPU path on a bounded sample; returns a dict."""
     import numpy as np
@@ -322,6 +368,7 @@ def main():
             gbs = batch * bytes_bdrt(n, s) / (ms * 1e-3) / 1e9
             extras[name] = {"ms_per_call": round(ms, 3), "imgs_per_s": round(batch / ms * 1e3, 2),
                             "GBps_algorithmic": round(gbs, 1)}
+        extras["uneven"] = uneven_extra(max(3, args.steps // 4), 2)
         result["extras"] = extras
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(host, args.cpu_seconds)

@@ -162,6 +162,33 @@ int adrt_hip_truncate(const void *in, void *out, size_t elem_bytes, size_t outer
 int adrt_hip_truncate_combine(const void *in, void *out, int dtype, size_t batch, size_t n,
                               double divisor, int mode, void *stream);
 
+/* ---- kernel configuration (no reference counterpart) --------------------- */
+
+/* Variants of the forward and back-projection paths for one size:
+ *   DEFAULT    log2 n in [6, 12]: the fused two-pass kernels with the default
+ *              stage split (forward pass 1 takes ceil(L/2) of the L = log2 n
+ *              stages, back-projection pass A floor(L/2)); other n: streaming;
+ *   ALT_SPLIT  the fused kernels with the other split (differs for odd L);
+ *   STREAMING  one kernel per stage.
+ * Every variant gives bit-identical results. */
+#define ADRT_HIP_VARIANT_DEFAULT 0
+#define ADRT_HIP_VARIANT_ALT_SPLIT 1
+#define ADRT_HIP_VARIANT_STREAMING 2
+
+/* Selects the variant and the per-launch-group workspace budget (bytes of
+ * intermediate per group of images; 0 = the default 136 MiB; a group holds
+ * at least one image) used from now on for `op` (ADRT_HIP_OP_ADRT or
+ * ADRT_HIP_OP_BDRT) at side n.  The setting is process-wide (all devices
+ * and threads) and changes adrt_hip_workspace_bytes for that (op, n):
+ * query the workspace after configuring.  A launch that meets a workspace
+ * sized for another configuration returns ADRT_HIP_ERR_WORKSPACE.  Used by
+ * the autotuner; ALT_SPLIT is rejected (ADRT_HIP_ERR_ARG) where n has no
+ * fused path. */
+int adrt_hip_set_config(int op, size_t n, int variant, size_t group_bytes);
+
+/* Reads the configuration of (op, n) (group_bytes: the budget in effect). */
+int adrt_hip_get_config(int op, size_t n, int *variant, size_t *group_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,8 @@ static unsigned grid_for_slots(int items, unsigned slots) {
 }
 
 static unsigned g_slots = 0;
+static int g_variant = 0;  // stage split (ADRT_HIP_VARIANT_DEFAULT / _ALT_SPLIT)
+static size_t g_budget = kDefaultGroupBytes;  // intermediate bytes per launch group
 
 template <typename T, int KB, int KD>
 static int emulate_group(const T *img, T *X, T *out, int n, int first, int count) {
@@ -119,9 +121,9 @@ static int run_bdrt(int B, int n, const char *fin, const char *fout) {
     std::fclose(f);
     const int L = num_iters(size_t(n));
     const size_t per = z_bytes_per_image(n, sizeof(T));
-    const size_t G = group_images(size_t(B), per);
+    const size_t G = group_images(size_t(B), per, g_budget);
     std::vector<T> Z(G * per / sizeof(T), T(NAN));
-    const int kA = L / 2;
+    const int kA = bdrt_passA_stages(L, g_variant);
     for (int first = 0; first < B; first += int(G)) {
         const int count = (B - first) < int(G) ? (B - first) : int(G);
         const int code = (kA - 3) * 4 + (L - kA - 3);
@@ -129,10 +131,13 @@ static int run_bdrt(int B, int n, const char *fin, const char *fout) {
         switch (code) {
             case 0: st = emulate_bdrt_group<T, 0, 0>(y.data(), Z.data(), out.data(), n, first, count); break;
             case 1: st = emulate_bdrt_group<T, 0, 1>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 4: st = emulate_bdrt_group<T, 1, 0>(y.data(), Z.data(), out.data(), n, first, count); break;
             case 5: st = emulate_bdrt_group<T, 1, 1>(y.data(), Z.data(), out.data(), n, first, count); break;
             case 6: st = emulate_bdrt_group<T, 1, 2>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 9: st = emulate_bdrt_group<T, 2, 1>(y.data(), Z.data(), out.data(), n, first, count); break;
             case 10: st = emulate_bdrt_group<T, 2, 2>(y.data(), Z.data(), out.data(), n, first, count); break;
             case 11: st = emulate_bdrt_group<T, 2, 3>(y.data(), Z.data(), out.data(), n, first, count); break;
+            case 14: st = emulate_bdrt_group<T, 3, 2>(y.data(), Z.data(), out.data(), n, first, count); break;
             case 15: st = emulate_bdrt_group<T, 3, 3>(y.data(), Z.data(), out.data(), n, first, count); break;
             default: break;
         }
@@ -154,19 +159,22 @@ static int run(int B, int n, const char *fin, const char *fout) {
     std::fclose(f);
     const int L = num_iters(size_t(n));
     const size_t per = x_bytes_per_image(n, L, sizeof(T));
-    const size_t G = group_images(size_t(B), per);
+    const size_t G = group_images(size_t(B), per, g_budget);
     std::vector<T> X(G * per / sizeof(T), T(NAN));
-    const int k1 = pass1_stages(L);
+    const int k1 = fwd_pass1_stages(L, g_variant);
     for (int first = 0; first < B; first += int(G)) {
         const int count = (B - first) < int(G) ? (B - first) : int(G);
         const int code = (k1 - 3) * 4 + (L - k1 - 3);
         int st = 3;
         switch (code) {
             case 0: st = emulate_group<T, 0, 0>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 1: st = emulate_group<T, 0, 1>(img.data(), X.data(), out.data(), n, first, count); break;
             case 4: st = emulate_group<T, 1, 0>(img.data(), X.data(), out.data(), n, first, count); break;
             case 5: st = emulate_group<T, 1, 1>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 6: st = emulate_group<T, 1, 2>(img.data(), X.data(), out.data(), n, first, count); break;
             case 9: st = emulate_group<T, 2, 1>(img.data(), X.data(), out.data(), n, first, count); break;
             case 10: st = emulate_group<T, 2, 2>(img.data(), X.data(), out.data(), n, first, count); break;
+            case 11: st = emulate_group<T, 2, 3>(img.data(), X.data(), out.data(), n, first, count); break;
             case 14: st = emulate_group<T, 3, 2>(img.data(), X.data(), out.data(), n, first, count); break;
             case 15: st = emulate_group<T, 3, 3>(img.data(), X.data(), out.data(), n, first, count); break;
             default: break;
@@ -180,12 +188,16 @@ static int run(int B, int n, const char *fin, const char *fout) {
 }
 
 int main(int argc, char **argv) {
-    if (argc < 6 || argc > 8) {
-        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [slots] [fwd|bdrt]\n", argv[0]);
+    if (argc < 6 || argc > 10) {
+        std::fprintf(stderr,
+                     "usage: %s f32|f64 B n in.bin out.bin [slots] [fwd|bdrt] [variant] [group_bytes]\n",
+                     argv[0]);
         return 1;
     }
     if (argc >= 7) g_slots = unsigned(std::atoi(argv[6]));
-    const bool bdrt = argc == 8 && std::strcmp(argv[7], "bdrt") == 0;
+    const bool bdrt = argc >= 8 && std::strcmp(argv[7], "bdrt") == 0;
+    if (argc >= 9) g_variant = std::atoi(argv[8]);
+    if (argc == 10 && std::atoll(argv[9]) > 0) g_budget = size_t(std::atoll(argv[9]));
     const int B = std::atoi(argv[2]), n = std::atoi(argv[3]);
     if (std::strcmp(argv[1], "f32") == 0)
         return bdrt ? run_bdrt<float>(B, n, argv[4], argv[5]) : run<float>(B, n, argv[4], argv[5]);

@@ -36,13 +36,14 @@ def emulator(tmp_path_factory):
     return exe
 
 
-def run_emu(exe, x, tmp_path, slots=0, op="fwd"):
+def run_emu(exe, x, tmp_path, slots=0, op="fwd", variant=0, group_bytes=0):
     b, n = x.shape[0], x.shape[-1]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     x.tofile(fin)
     tag = "f32" if x.dtype == np.float32 else "f64"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout), str(slots), op],
+    res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout), str(slots), op,
+                          str(variant), str(group_bytes)],
                          capture_output=True, text=True, env=env)
     assert res.returncode == 0, res.stderr[-4000:]
     return np.fromfile(fout, dtype=x.dtype).reshape(b, 4, 2 * n - 1, n)
@@ -95,6 +96,40 @@ def test_fused_emulation_capped_grid(emulator, tmp_path, slots):
     x = np.random.default_rng(slots).standard_normal((3, 256, 256)).astype(np.float32)
     got = run_emu(emulator, x, tmp_path, slots)
     assert np.array_equal(got.view(np.uint8), O.adrt(x).view(np.uint8))
+
+
+@pytest.mark.parametrize("n", [128, 512, 2048])
+def test_fused_alt_split_emulation(emulator, tmp_path, n):
+    """The other stage split (k1 = floor(L/2) forward, kA = ceil(L/2)
+    back-projection) that the autotuner may pick for odd log2 n."""
+    rng = np.random.default_rng(n + 5)
+    x = (rng.standard_normal((1, n, n)) * 10).astype(np.float32)
+    x[0, 0, :3] = [-0.0, np.inf, np.nan]
+    got = run_emu(emulator, x, tmp_path, variant=1)
+    want = O.adrt(x)
+    same = np.isnan(got) & np.isnan(want)
+    assert np.array_equal(np.where(same, 0, got).view(np.uint8), np.where(same, 0, want).view(np.uint8))
+    y = (rng.standard_normal((1, 4, 2 * n - 1, n)) * 10).astype(np.float32)
+    got = run_emu(emulator, y, tmp_path, op="bdrt", variant=1)
+    assert np.array_equal(canonical(got).view(np.uint8), canonical(O.bdrt(y)).view(np.uint8))
+
+
+@pytest.mark.parametrize("op", ["fwd", "bdrt"])
+def test_fused_small_groups_emulation(emulator, tmp_path, op):
+    """A group budget of two images' intermediate: five images run as
+    groups of 2, 2 and 1 (the tuner's smaller launch groups)."""
+    n = 128
+    rng = np.random.default_rng(9)
+    if op == "fwd":
+        x = rng.standard_normal((5, n, n)).astype(np.float32)
+        per = 4 * n * 224 * 4  # X bytes per image at n = 128 (row pitch 224)
+        got = run_emu(emulator, x, tmp_path, op=op, variant=1, group_bytes=2 * per)
+        assert np.array_equal(got.view(np.uint8), O.adrt(x).view(np.uint8))
+    else:
+        y = rng.standard_normal((5, 4, 2 * n - 1, n)).astype(np.float32)
+        per = 4 * n * 320 * 4  # Z bytes per image at n = 128 (row pitch 320)
+        got = run_emu(emulator, y, tmp_path, op=op, group_bytes=2 * per)
+        assert np.array_equal(canonical(got).view(np.uint8), canonical(O.bdrt(y)).view(np.uint8))
 
 
 IADRT_SRC = os.path.join(HERE, "host_emulation", "iadrt_emu.cpp")
