@@ -65,6 +65,26 @@ SIGNATURES = {
     "adrt_hip_truncate_combine": (_INT, [_VP, _VP, _INT, _SZ, _SZ, ctypes.c_double, _INT, _VP]),
     "adrt_hip_set_config": (_INT, [_INT, _SZ, _INT, _SZ]),
     "adrt_hip_get_config": (_INT, [_INT, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_SZ)]),
+    "adrt_hip_nn_conv3x3": (_INT, [_VP, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _INT, _INT,
+                                   _INT, _INT, _INT, _INT, _INT, _VP]),
+    "adrt_hip_nn_wgrad_workspace": (_SZ, [_INT, _INT, _INT, _INT, _INT]),
+    "adrt_hip_nn_wgrad": (_INT, [_VP, _INT, _VP, _INT, _VP, _VP, _VP, _SZ, _INT, _INT, _INT,
+                                 _INT, _INT, _VP]),
+    "adrt_hip_nn_transpose_weights": (_INT, [_VP, _VP, _INT, _INT, _VP]),
+    "adrt_hip_nn_pool": (_INT, [_VP, _INT, _VP, _INT, _INT, _INT, _INT, _INT, _VP]),
+    "adrt_hip_nn_pool_backward": (_INT, [_VP, _INT, _VP, _INT, _VP, _INT, _VP, _INT, _INT, _INT,
+                                         _INT, _INT, _VP]),
+    "adrt_hip_nn_upsample": (_INT, [_VP, _INT, _VP, _INT, _INT, _INT, _INT, _INT, _VP]),
+    "adrt_hip_nn_upsample_backward": (_INT, [_VP, _INT, _VP, _INT, _INT, _INT, _INT, _INT, _VP]),
+    "adrt_hip_nn_pack_input": (_INT, [_VP, _VP, _INT, _INT, _INT, ctypes.c_float, _VP]),
+    "adrt_hip_nn_loss_grad": (_INT, [_VP, _VP, _VP, _INT, _INT, _INT, ctypes.c_float, _VP]),
+    "adrt_hip_nn_adam": (_INT, [_VP, _VP, _VP, _VP, _VP, _SZ, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                _VP]),
+    "adrt_hip_pyr_pool": (_INT, [_VP, _VP, _INT, _INT, _INT, _VP]),
+    "adrt_hip_pyr_stencil_up": (_INT, [_VP, _VP, _VP, _VP, _INT, _INT, _INT, _VP]),
+    "adrt_hip_pyr_correlate": (_INT, [_VP, _VP, _VP, _INT, _INT, _INT, _INT, _VP]),
+    "adrt_hip_phantom_raster": (_INT, [_VP, _VP, _INT, _VP, _INT, _INT, _VP]),
 }
 
 _lib = None

@@ -1,0 +1,801 @@
+// Kernels of the learned initial guess / preconditioner for CG inversion:
+// a 4-level U-Net (3x3 convolutions, BF16 weights and activations, FP32
+// accumulation on the matrix cores) with its training kernels, a
+// multiscale linear preconditioner (fp32), and a phantom rasteriser that
+// generates training data on the device.  None of this has a counterpart in
+// the reference package; it builds on its transforms.
+//
+// Activations are NHWC bf16 with a pixel stride (elements) that may exceed
+// the channel count, so a U-Net skip connection and the upsampled coarse
+// features can share one buffer (zero-copy concatenation).  Channel counts
+// and strides are multiples of 8, so 8 channels are one 16-byte load.
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16:
+//   forward / input gradient: D[pixel][co] = sum_k X[pixel + tap(k)][ci(k)] * W[co][k],
+//     k = tap * Cin + ci (weights stored [Cout][9][Cin]);
+//   weight gradient: D[co][ci] = sum_pixel dZ[pixel][co] * X[pixel + tap][ci],
+//     per tap, reduced over pixel partitions in a second pass.
+// Operand fragments of one MFMA: A lane l holds row l%16 with k-group l/16
+// (8 consecutive k), B lane l holds column l%16 with the same k-group, and
+// the result lane l holds column l%16, rows 4*(l/16) + 0..3.
+#include "common.hpp"
+
+namespace adrt_hip {
+namespace nn {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaveLanes = 64;
+constexpr int kWgThreads = 256;
+constexpr int kMT = 4;  // 16-pixel row tiles per wave (forward / input gradient)
+
+__device__ __forceinline__ bf16x8 load8(const bf16 *p) {
+    return *reinterpret_cast<const bf16x8 *>(p);
+}
+
+// v if keep, else all zeros (branch-free: the load above is always issued).
+__device__ __forceinline__ bf16x8 keep_or_zero(bf16x8 v, bool keep) {
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    const unsigned m = keep ? 0xffffffffu : 0u;
+    u.x &= m;
+    u.y &= m;
+    u.z &= m;
+    u.w &= m;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, zero padding, stride 1 (forward and input gradient).
+// ---------------------------------------------------------------------------
+enum ConvMode : int {
+    kReluBias = 0,   // y = relu(conv + bias)                       (bf16)
+    kResidual = 1,   // y32[p] = resid[p] + conv[p][0] + bias[0]     (fp32, channel 0 only)
+    kMaskInput = 2,  // y = conv * (mask > 0)                       (bf16)
+    kLinear = 3,     // y = conv                                    (bf16)
+};
+
+struct ConvArgs {
+    const bf16 *x;
+    const bf16 *w;  // [Cout][9][Cin]
+    const float *bias;
+    bf16 *y;
+    float *y32;
+    const float *resid;
+    const bf16 *mask;
+    long long P;  // pixels = B * H * W
+    int x_stride, y_stride, mask_stride;
+    int H, W, Cin, Cout;
+};
+
+// Wave = 64 consecutive pixels (4 tiles of 16) x 16*NT output channels.  A
+// work-group's 4 waves share the output-channel block (their weight
+// fragments hit the same cache lines).
+template <int NT, int MODE>
+__global__ void __launch_bounds__(kWgThreads) conv3x3_kernel(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int n_co_blocks = a.Cout / (16 * NT);
+    const int co0 = int(blockIdx.x % n_co_blocks) * 16 * NT;
+    const long long px0 = ((long long)(blockIdx.x / n_co_blocks) * 4 + wave) * 64;
+    if (px0 >= a.P) return;  // no barriers below
+
+    // A rows: pixel px0 + 16*mt + r
+    int py[kMT], pxx[kMT];
+    long long pp[kMT];
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt) {
+        const long long p = px0 + 16 * mt + r;
+        pp[mt] = p;
+        pxx[mt] = int(p % a.W);
+        py[mt] = int((p / a.W) % a.H);
+    }
+    f32x4 acc[kMT][NT];
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int K = 9 * a.Cin;
+    int k = 8 * g;  // this lane group's first k
+    int t = k / a.Cin, ci = k - t * a.Cin;
+    const bf16 *wrow[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) wrow[nt] = a.w + (size_t)(co0 + 16 * nt + r) * K;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bool kok = k < K;
+        const int tc = kok ? t : 4;
+        const int dy = tc / 3 - 1, dx = tc - (tc / 3) * 3 - 1;
+        const long long delta = ((long long)dy * a.W + dx) * a.x_stride + (kok ? ci : 0);
+        bf16x8 A[kMT], Bf[NT];
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt) {
+            const bool ok = kok && unsigned(py[mt] + dy) < unsigned(a.H) &&
+                            unsigned(pxx[mt] + dx) < unsigned(a.W);
+            const long long off = pp[mt] * a.x_stride + (ok ? delta : 0);
+            A[mt] = keep_or_zero(load8(a.x + off), ok);
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) Bf[nt] = keep_or_zero(load8(wrow[nt] + (kok ? k : 0)), kok);
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(A[mt], Bf[nt], acc[mt][nt]);
+        k += 32;
+        ci += 32;
+        while (ci >= a.Cin) {
+            ci -= a.Cin;
+            ++t;
+        }
+    }
+
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = co0 + 16 * nt + r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long p = px0 + 16 * mt + 4 * g + j;
+                float v = acc[mt][nt][j];
+                if (MODE == kReluBias) {
+                    v += a.bias[co];
+                    v = v > 0.f ? v : 0.f;
+                    a.y[p * a.y_stride + co] = bf16(v);
+                } else if (MODE == kResidual) {
+                    if (co == 0) a.y32[p] = a.resid[p] + (v + a.bias[0]);
+                } else if (MODE == kMaskInput) {
+                    const float m = float(a.mask[p * a.mask_stride + co]);
+                    a.y[p * a.y_stride + co] = bf16(m > 0.f ? v : 0.f);
+                } else {
+                    a.y[p * a.y_stride + co] = bf16(v);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: partial[part][co][t][ci] = sum over the part's pixels of
+// dZ[p][co] * X[p + tap t][ci]; partial_bias[part][co] = sum dZ[p][co].
+// Wave = one (16 co) x (16 ci) tile for all 9 taps over one pixel part.
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+    const bf16 *x;
+    const bf16 *dz;
+    float *part;       // [parts][Cout][9][Cin]
+    float *part_bias;  // [parts][Cout]
+    long long P;
+    long long px_per_part;
+    int parts;
+    int x_stride, dz_stride;
+    int H, W, Cin, Cout;
+};
+
+__device__ __forceinline__ bf16x8 pack8(const bf16 (&v)[8]) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = v[j];
+    return r;
+}
+
+__global__ void __launch_bounds__(kWgThreads) wgrad_kernel(WgradArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int n_ci = (a.Cin + 15) / 16, n_co = a.Cout / 16;
+    const long long wv = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long tiles = (long long)n_ci * n_co;
+    if (wv >= tiles * a.parts) return;  // no barriers below
+    const int part = int(wv / tiles);
+    const int tile = int(wv % tiles);
+    const int co0 = (tile / n_ci) * 16, ci0 = (tile % n_ci) * 16;
+    const int ci = ci0 + r;
+    const bool ci_ok = ci < a.Cin;
+    const int cic = ci_ok ? ci : 0;
+    const bool do_bias = ci0 == 0;
+
+    f32x4 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 accb = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 ones;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ones[j] = bf16(1.0f);
+
+    const long long pa = (long long)part * a.px_per_part;
+    long long pb = pa + a.px_per_part;
+    if (pb > a.P) pb = a.P;
+    for (long long k0 = pa; k0 < pb; k0 += 32) {
+        // this lane group's 8 pixels: k0 + 8g .. +7, one image row (W % 8 == 0)
+        const long long p0 = k0 + 8 * g;
+        const bool pok = p0 < pb;
+        const long long p0c = pok ? p0 : pa;
+        const int x0 = int(p0c % a.W);
+        const int y0 = int((p0c / a.W) % a.H);
+        bf16 av[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bf16 v = a.dz[(p0c + j) * a.dz_stride + co0 + r];
+            av[j] = pok ? v : bf16(0.0f);
+        }
+        const bf16x8 A = pack8(av);
+#pragma unroll
+        for (int dyi = 0; dyi < 3; ++dyi) {
+            const int yy = y0 + dyi - 1;
+            const bool yok = pok && ci_ok && unsigned(yy) < unsigned(a.H);
+            // window of 10 pixels x0-1 .. x0+8 in row yy
+            bf16 win[10];
+#pragma unroll
+            for (int u = 0; u < 10; ++u) {
+                const int xx = x0 + u - 1;
+                const bool ok = yok && unsigned(xx) < unsigned(a.W);
+                const long long q = ok ? p0c + (long long)(dyi - 1) * a.W + (u - 1) : p0c;
+                const bf16 v = a.x[q * a.x_stride + cic];
+                win[u] = ok ? v : bf16(0.0f);
+            }
+#pragma unroll
+            for (int dxi = 0; dxi < 3; ++dxi) {
+                bf16 bv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bv[j] = win[j + dxi];
+                acc[dyi * 3 + dxi] = mfma(A, pack8(bv), acc[dyi * 3 + dxi]);
+            }
+        }
+        if (do_bias) accb = mfma(A, ones, accb);
+    }
+    // result lane: column ci0 + r, rows co0 + 4g + j
+    float *dst = a.part + (size_t)part * a.Cout * 9 * a.Cin;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (ci_ok) dst[((size_t)(co0 + 4 * g + j) * 9 + t) * a.Cin + ci] = acc[t][j];
+    if (do_bias && r == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a.part_bias[(size_t)part * a.Cout + co0 + 4 * g + j] = accb[j];
+    }
+}
+
+// out[i] = sum_p part[p][i], i < count (weights), then the bias rows.
+__global__ void __launch_bounds__(kWgThreads)
+    reduce_parts_kernel(const float *part, float *out, long long count, int parts) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.f;
+    for (int p = 0; p < parts; ++p) s += part[(size_t)p * count + i];
+    out[i] = s;
+}
+
+// W'[ci][t'][co] = W[co][8 - t'][ci]: the input-gradient convolution's weights.
+__global__ void __launch_bounds__(kWgThreads)
+    transpose_weights_kernel(const bf16 *w, bf16 *wt, int Cin, int Cout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = Cout * 9 * Cin;
+    if (i >= total) return;
+    const int ci = i % Cin, t = (i / Cin) % 9, co = i / (9 * Cin);
+    wt[((size_t)ci * 9 + (8 - t)) * Cout + co] = w[i];
+}
+
+// ---------------------------------------------------------------------------
+// 2x2 average pooling, nearest 2x upsampling and their gradients (NHWC, 8
+// channels per thread).
+// ---------------------------------------------------------------------------
+struct Vec8 {
+    float v[8];
+};
+__device__ __forceinline__ Vec8 to_f32(bf16x8 b) {
+    Vec8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r.v[j] = float(b[j]);
+    return r;
+}
+__device__ __forceinline__ bf16x8 to_bf16(const Vec8 &f) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = bf16(f.v[j]);
+    return r;
+}
+__device__ __forceinline__ void store8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
+
+// y (B, H/2, W/2) = average of 2x2 blocks of x (B, H, W); C channels.
+__global__ void __launch_bounds__(kWgThreads)
+    pool_kernel(const bf16 *x, int xs, bf16 *y, int ys, long long Pc, int H, int W, int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c8 = C / 8;
+    if (i >= Pc * c8) return;
+    const int c = int(i % c8) * 8;
+    const long long q = i / c8;
+    const int Wc = W / 2, Hc = H / 2;
+    const int xq = int(q % Wc), yq = int((q / Wc) % Hc);
+    const long long b = q / ((long long)Wc * Hc);
+    const long long p00 = (b * H + 2 * yq) * W + 2 * xq;
+    const Vec8 a0 = to_f32(load8(x + p00 * xs + c)), a1 = to_f32(load8(x + (p00 + 1) * xs + c));
+    const Vec8 a2 = to_f32(load8(x + (p00 + W) * xs + c)),
+               a3 = to_f32(load8(x + (p00 + W + 1) * xs + c));
+    Vec8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.v[j] = 0.25f * ((a0.v[j] + a1.v[j]) + (a2.v[j] + a3.v[j]));
+    store8(y + q * ys + c, to_bf16(o));
+}
+
+// ds (B, H, W) = (dp[parent] / 4 + dskip) * (s > 0)   (H, W: the fine size)
+__global__ void __launch_bounds__(kWgThreads)
+    pool_backward_kernel(const bf16 *dp, int dps, const bf16 *dskip, int dks, const bf16 *s,
+                         int ss, bf16 *ds, int dss, long long P, int H, int W, int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c8 = C / 8;
+    if (i >= P * c8) return;
+    const int c = int(i % c8) * 8;
+    const long long p = i / c8;
+    const int xx = int(p % W), yy = int((p / W) % H);
+    const long long b = p / ((long long)W * H);
+    const long long q = (b * (H / 2) + yy / 2) * (W / 2) + xx / 2;
+    const Vec8 g = to_f32(load8(dp + q * dps + c));
+    const Vec8 k = to_f32(load8(dskip + p * dks + c));
+    const Vec8 m = to_f32(load8(s + p * ss + c));
+    Vec8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.v[j] = m.v[j] > 0.f ? 0.25f * g.v[j] + k.v[j] : 0.f;
+    store8(ds + p * dss + c, to_bf16(o));
+}
+
+// y (B, 2H, 2W) = x[parent]   (H, W: the coarse size)
+__global__ void __launch_bounds__(kWgThreads)
+    upsample_kernel(const bf16 *x, int xs, bf16 *y, int ys, long long Pf, int H, int W, int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c8 = C / 8;
+    if (i >= Pf * c8) return;
+    const int c = int(i % c8) * 8;
+    const long long p = i / c8;
+    const int W2 = 2 * W, H2 = 2 * H;
+    const int xx = int(p % W2), yy = int((p / W2) % H2);
+    const long long b = p / ((long long)W2 * H2);
+    const long long q = (b * H + yy / 2) * W + xx / 2;
+    store8(y + p * ys + c, load8(x + q * xs + c));
+}
+
+// dx (B, H, W) = sum of the 2x2 children of dy (B, 2H, 2W)   (H, W coarse)
+__global__ void __launch_bounds__(kWgThreads)
+    upsample_backward_kernel(const bf16 *dy, int dys, bf16 *dx, int dxs, long long Pc, int H,
+                             int W, int C) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c8 = C / 8;
+    if (i >= Pc * c8) return;
+    const int c = int(i % c8) * 8;
+    const long long q = i / c8;
+    const int xq = int(q % W), yq = int((q / W) % H);
+    const long long b = q / ((long long)W * H);
+    const long long p00 = (b * 2 * H + 2 * yq) * (2 * W) + 2 * xq;
+    const Vec8 a0 = to_f32(load8(dy + p00 * dys + c)), a1 = to_f32(load8(dy + (p00 + 1) * dys + c));
+    const Vec8 a2 = to_f32(load8(dy + (p00 + 2 * W) * dys + c)),
+               a3 = to_f32(load8(dy + (p00 + 2 * W + 1) * dys + c));
+    Vec8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.v[j] = (a0.v[j] + a1.v[j]) + (a2.v[j] + a3.v[j]);
+    store8(dx + q * dxs + c, to_bf16(o));
+}
+
+// y (P, 8) bf16: channel 0 = scale * x[p], channels 1..7 = 0.
+__global__ void __launch_bounds__(kWgThreads)
+    pack_input_kernel(const float *x, bf16 *y, long long P, float scale) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = bf16(0.0f);
+    v[0] = bf16(scale * x[p]);
+    store8(y + p * 8, v);
+}
+
+// dz (P, 16) bf16: channel 0 = scale * (pred - target), others 0.
+__global__ void __launch_bounds__(kWgThreads)
+    loss_grad_kernel(const float *pred, const float *target, bf16 *dz, long long P, float scale) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    bf16x8 v0, v1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        v0[j] = bf16(0.0f);
+        v1[j] = bf16(0.0f);
+    }
+    v0[0] = bf16(scale * (pred[p] - target[p]));
+    store8(dz + p * 16, v0);
+    store8(dz + p * 16 + 8, v1);
+}
+
+// Adam (Kingma & Ba 2015, with bias correction c1 = 1 - beta1^t,
+// c2 = 1 - beta2^t) on fp32 master weights; w16 receives the bf16 copy.
+__global__ void __launch_bounds__(kWgThreads)
+    adam_kernel(float *w, const float *g, float *m, float *v, bf16 *w16, long long count, float lr,
+                float b1, float b2, float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (w16) w16[i] = bf16(wi);
+}
+
+// ---------------------------------------------------------------------------
+// Multiscale preconditioner P r = sum_l D_l^T (h_l * (D_l r)), fp32:
+// D = 2x2 averaging, D^T = nearest upsampling / 4, h_l a 5x5 stencil with
+// zero padding (symmetric coefficients, so P is symmetric).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWgThreads)
+    pyr_pool_kernel(const float *x, float *y, long long Pc, int H, int W) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Pc) return;
+    const int Wc = W / 2, Hc = H / 2;
+    const int xq = int(q % Wc), yq = int((q / Wc) % Hc);
+    const long long b = q / ((long long)Wc * Hc);
+    const long long p = (b * H + 2 * yq) * W + 2 * xq;
+    y[q] = 0.25f * ((x[p] + x[p + 1]) + (x[p + W] + x[p + W + 1]));
+}
+
+// y = h * x (5x5, zero padding) + coarse[parent] / 4 (coarse may be null).
+__global__ void __launch_bounds__(kWgThreads)
+    pyr_stencil_up_kernel(const float *x, const float *coarse, float *y, const float *coef,
+                          long long P, int H, int W) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int xx = int(p % W), yy = int((p / W) % H);
+    const long long b = p / ((long long)W * H);
+    const float *img = x + b * (long long)H * W;
+    float s = 0.f;
+#pragma unroll
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int y2 = yy + dy;
+        if (unsigned(y2) >= unsigned(H)) continue;
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int x2 = xx + dx;
+            if (unsigned(x2) >= unsigned(W)) continue;
+            s += coef[(dy + 2) * 5 + (dx + 2)] * img[(long long)y2 * W + x2];
+        }
+    }
+    if (coarse) s += 0.25f * coarse[(b * (H / 2) + yy / 2) * (W / 2) + xx / 2];
+    y[p] = s;
+}
+
+// partial[block][o] = sum over the block's pixels p of e[p] * w[p + o],
+// o in the 5x5 window (zero outside each image).
+__global__ void __launch_bounds__(kWgThreads)
+    pyr_correlate_kernel(const float *e, const float *w, float *partial, long long P, int H, int W) {
+    __shared__ float red[25][kWgThreads / kWaveLanes];
+    float s[25];
+#pragma unroll
+    for (int o = 0; o < 25; ++o) s[o] = 0.f;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (long long)gridDim.x * blockDim.x) {
+        const int xx = int(p % W), yy = int((p / W) % H);
+        const float ev = e[p];
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx) {
+                const bool ok = unsigned(yy + dy) < unsigned(H) && unsigned(xx + dx) < unsigned(W);
+                const float wv = ok ? w[p + (long long)dy * W + dx] : 0.f;
+                s[(dy + 2) * 5 + dx + 2] += ev * wv;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 0; o < 25; ++o) {
+        float v = s[o];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[o][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 25) {
+        float v = 0.f;
+        for (int k = 0; k < kWgThreads / kWaveLanes; ++k) v += red[threadIdx.x][k];
+        partial[(size_t)blockIdx.x * 25 + threadIdx.x] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phantoms: out[b][i][j] = sum of val over the ellipses of image b whose
+// closed interior contains the pixel centre (x, y) = (-1 + (2j+1)/n,
+// 1 - (2i+1)/n); params[b][e] = (cx, cy, cos, sin, ax, ay, val), float64.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kWgThreads)
+    phantom_kernel(const double *params, const int *counts, int max_e, float *out, long long P,
+                   int n) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int j = int(p % n), i = int((p / n) % n);
+    const long long b = p / ((long long)n * n);
+    const double xs = -1.0 + (2.0 * j + 1.0) / n;
+    const double ys = 1.0 - (2.0 * i + 1.0) / n;
+    const double *e = params + b * max_e * 7;
+    double acc = 0.0;
+    for (int k = 0; k < counts[b]; ++k, e += 7) {
+        const double dx = xs - e[0], dy = ys - e[1];
+        const double u = (dx * e[2] + dy * e[3]) / e[4];
+        const double v = (-dx * e[3] + dy * e[2]) / e[5];
+        if (u * u + v * v <= 1.0) acc += e[6];
+    }
+    out[p] = float(acc);
+}
+
+inline unsigned blocks_for(long long work, int block = kWgThreads) {
+    long long g = (work + block - 1) / block;
+    return unsigned(g < 1 ? 1 : g);
+}
+
+template <int NT>
+static void launch_conv_nt(const ConvArgs &a, int mode, hipStream_t s) {
+    const long long px_blocks = a.P / 64;
+    const unsigned grid = unsigned((px_blocks + 3) / 4 * (a.Cout / (16 * NT)));
+    switch (mode) {
+        case kReluBias: hipLaunchKernelGGL((conv3x3_kernel<NT, kReluBias>), dim3(grid), dim3(kWgThreads), 0, s, a); break;
+        case kResidual: hipLaunchKernelGGL((conv3x3_kernel<NT, kResidual>), dim3(grid), dim3(kWgThreads), 0, s, a); break;
+        case kMaskInput: hipLaunchKernelGGL((conv3x3_kernel<NT, kMaskInput>), dim3(grid), dim3(kWgThreads), 0, s, a); break;
+        default: hipLaunchKernelGGL((conv3x3_kernel<NT, kLinear>), dim3(grid), dim3(kWgThreads), 0, s, a); break;
+    }
+}
+
+struct WgradPlan {
+    int parts;
+    long long px_per_part;
+};
+
+inline WgradPlan wgrad_plan(long long P, int Cin, int Cout) {
+    const long long tiles = (long long)((Cin + 15) / 16) * (Cout / 16);
+    long long parts = 8192 / tiles;
+    if (parts < 1) parts = 1;
+    const long long max_parts = (P + 31) / 32;
+    if (parts > max_parts) parts = max_parts;
+    long long per = (P + parts - 1) / parts;
+    per = (per + 31) / 32 * 32;
+    WgradPlan w;
+    w.px_per_part = per;
+    w.parts = int((P + per - 1) / per);
+    return w;
+}
+
+}  // namespace nn
+
+using namespace nn;
+
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+extern "C" {
+
+int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float *bias, void *y,
+                        int y_stride, float *y32, const float *resid, const void *mask,
+                        int mask_stride, int batch, int h, int w_, int cin, int cout, int mode,
+                        void *stream) {
+    if (batch < 1 || h < 1 || w_ < 1 || cin < 8 || cin % 8 != 0 || cout < 16 ||
+        cout % 16 != 0 || x_stride < cin || x_stride % 8 != 0 || mode < 0 || mode > 3)
+        return ADRT_HIP_ERR_ARG;
+    const long long P = (long long)batch * h * w_;
+    if (P % 64 != 0) return ADRT_HIP_ERR_ARG;
+    if (!x || !w || !aligned16(x) || !aligned16(w)) return ADRT_HIP_ERR_NULL;
+    if (mode == kResidual) {
+        if (!y32 || !resid || !bias) return ADRT_HIP_ERR_NULL;
+    } else {
+        if (!y || y_stride < cout) return ADRT_HIP_ERR_ARG;
+        if (mode == kReluBias && !bias) return ADRT_HIP_ERR_NULL;
+        if (mode == kMaskInput && (!mask || mask_stride < cout)) return ADRT_HIP_ERR_NULL;
+    }
+    ConvArgs a;
+    a.x = static_cast<const bf16 *>(x);
+    a.w = static_cast<const bf16 *>(w);
+    a.bias = bias;
+    a.y = static_cast<bf16 *>(y);
+    a.y32 = y32;
+    a.resid = resid;
+    a.mask = static_cast<const bf16 *>(mask);
+    a.P = P;
+    a.x_stride = x_stride;
+    a.y_stride = y_stride;
+    a.mask_stride = mask_stride;
+    a.H = h;
+    a.W = w_;
+    a.Cin = cin;
+    a.Cout = cout;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (cout % 64 == 0)
+        launch_conv_nt<4>(a, mode, s);
+    else if (cout % 32 == 0)
+        launch_conv_nt<2>(a, mode, s);
+    else
+        launch_conv_nt<1>(a, mode, s);
+    return launch_status();
+}
+
+size_t adrt_hip_nn_wgrad_workspace(int batch, int h, int w, int cin, int cout) {
+    if (batch < 1 || h < 1 || w < 8 || cin < 8 || cout < 16) return 0;
+    const long long P = (long long)batch * h * w;
+    const WgradPlan pl = wgrad_plan(P, cin, cout);
+    return size_t(pl.parts) * (size_t(cout) * 9 * cin + cout) * sizeof(float);
+}
+
+int adrt_hip_nn_wgrad(const void *x, int x_stride, const void *dz, int dz_stride, float *dw,
+                      float *dbias, void *ws, size_t ws_bytes, int batch, int h, int w, int cin,
+                      int cout, void *stream) {
+    if (batch < 1 || h < 1 || w < 8 || w % 8 != 0 || cin < 8 || cin % 8 != 0 || cout < 16 ||
+        cout % 16 != 0 || x_stride < cin || dz_stride < cout)
+        return ADRT_HIP_ERR_ARG;
+    if (!x || !dz || !dw || !dbias || !ws) return ADRT_HIP_ERR_NULL;
+    if (ws_bytes < adrt_hip_nn_wgrad_workspace(batch, h, w, cin, cout)) return ADRT_HIP_ERR_WORKSPACE;
+    const long long P = (long long)batch * h * w;
+    const WgradPlan pl = wgrad_plan(P, cin, cout);
+    WgradArgs a;
+    a.x = static_cast<const bf16 *>(x);
+    a.dz = static_cast<const bf16 *>(dz);
+    a.part = static_cast<float *>(ws);
+    a.part_bias = a.part + (size_t)pl.parts * cout * 9 * cin;
+    a.P = P;
+    a.px_per_part = pl.px_per_part;
+    a.parts = pl.parts;
+    a.x_stride = x_stride;
+    a.dz_stride = dz_stride;
+    a.H = h;
+    a.W = w;
+    a.Cin = cin;
+    a.Cout = cout;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long waves = (long long)((cin + 15) / 16) * (cout / 16) * pl.parts;
+    hipLaunchKernelGGL(wgrad_kernel, dim3(unsigned((waves + 3) / 4)), dim3(kWgThreads), 0, s, a);
+    const long long nw = (long long)cout * 9 * cin;
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(nw)), dim3(kWgThreads), 0, s,
+                       (const float *)a.part, dw, nw, pl.parts);
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3(blocks_for(cout)), dim3(kWgThreads), 0, s,
+                       (const float *)a.part_bias, dbias, (long long)cout, pl.parts);
+    return launch_status();
+}
+
+int adrt_hip_nn_transpose_weights(const void *w, void *wt, int cin, int cout, void *stream) {
+    if (cin < 1 || cout < 1) return ADRT_HIP_ERR_ARG;
+    if (!w || !wt) return ADRT_HIP_ERR_NULL;
+    const int total = cout * 9 * cin;
+    hipLaunchKernelGGL(transpose_weights_kernel, dim3(blocks_for(total)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(w),
+                       static_cast<bf16 *>(wt), cin, cout);
+    return launch_status();
+}
+
+static bool nhwc_ok(int h, int w, int c, int stride) {
+    return h >= 2 && w >= 2 && h % 2 == 0 && w % 2 == 0 && c >= 8 && c % 8 == 0 && stride >= c &&
+           stride % 8 == 0;
+}
+
+int adrt_hip_nn_pool(const void *x, int x_stride, void *y, int y_stride, int batch, int h, int w,
+                     int c, void *stream) {
+    if (batch < 1 || !nhwc_ok(h, w, c, x_stride) || y_stride < c || y_stride % 8) return ADRT_HIP_ERR_ARG;
+    if (!x || !y) return ADRT_HIP_ERR_NULL;
+    const long long Pc = (long long)batch * (h / 2) * (w / 2);
+    hipLaunchKernelGGL(pool_kernel, dim3(blocks_for(Pc * (c / 8))), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(x), x_stride,
+                       static_cast<bf16 *>(y), y_stride, Pc, h, w, c);
+    return launch_status();
+}
+
+int adrt_hip_nn_pool_backward(const void *dp, int dp_stride, const void *dskip, int dskip_stride,
+                              const void *s, int s_stride, void *ds, int ds_stride, int batch,
+                              int h, int w, int c, void *stream) {
+    if (batch < 1 || !nhwc_ok(h, w, c, s_stride) || dp_stride < c || dskip_stride < c ||
+        ds_stride < c || dp_stride % 8 || dskip_stride % 8 || ds_stride % 8)
+        return ADRT_HIP_ERR_ARG;
+    if (!dp || !dskip || !s || !ds) return ADRT_HIP_ERR_NULL;
+    const long long P = (long long)batch * h * w;
+    hipLaunchKernelGGL(pool_backward_kernel, dim3(blocks_for(P * (c / 8))), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(dp), dp_stride,
+                       static_cast<const bf16 *>(dskip), dskip_stride, static_cast<const bf16 *>(s),
+                       s_stride, static_cast<bf16 *>(ds), ds_stride, P, h, w, c);
+    return launch_status();
+}
+
+int adrt_hip_nn_upsample(const void *x, int x_stride, void *y, int y_stride, int batch, int h,
+                         int w, int c, void *stream) {
+    if (batch < 1 || h < 1 || w < 1 || c < 8 || c % 8 || x_stride < c || x_stride % 8 ||
+        y_stride < c || y_stride % 8)
+        return ADRT_HIP_ERR_ARG;
+    if (!x || !y) return ADRT_HIP_ERR_NULL;
+    const long long Pf = (long long)batch * (2 * h) * (2 * w);
+    hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for(Pf * (c / 8))), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(x), x_stride,
+                       static_cast<bf16 *>(y), y_stride, Pf, h, w, c);
+    return launch_status();
+}
+
+int adrt_hip_nn_upsample_backward(const void *dy, int dy_stride, void *dx, int dx_stride,
+                                  int batch, int h, int w, int c, void *stream) {
+    if (batch < 1 || h < 1 || w < 1 || c < 8 || c % 8 || dy_stride < c || dy_stride % 8 ||
+        dx_stride < c || dx_stride % 8)
+        return ADRT_HIP_ERR_ARG;
+    if (!dy || !dx) return ADRT_HIP_ERR_NULL;
+    const long long Pc = (long long)batch * h * w;
+    hipLaunchKernelGGL(upsample_backward_kernel, dim3(blocks_for(Pc * (c / 8))), dim3(kWgThreads),
+                       0, static_cast<hipStream_t>(stream), static_cast<const bf16 *>(dy),
+                       dy_stride, static_cast<bf16 *>(dx), dx_stride, Pc, h, w, c);
+    return launch_status();
+}
+
+int adrt_hip_nn_pack_input(const float *x, void *y, int batch, int h, int w, float scale,
+                           void *stream) {
+    if (batch < 1 || h < 1 || w < 1) return ADRT_HIP_ERR_ARG;
+    if (!x || !y) return ADRT_HIP_ERR_NULL;
+    const long long P = (long long)batch * h * w;
+    hipLaunchKernelGGL(pack_input_kernel, dim3(blocks_for(P)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), x, static_cast<bf16 *>(y), P, scale);
+    return launch_status();
+}
+
+int adrt_hip_nn_loss_grad(const float *pred, const float *target, void *dz, int batch, int h,
+                          int w, float scale, void *stream) {
+    if (batch < 1 || h < 1 || w < 1) return ADRT_HIP_ERR_ARG;
+    if (!pred || !target || !dz) return ADRT_HIP_ERR_NULL;
+    const long long P = (long long)batch * h * w;
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks_for(P)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), pred, target, static_cast<bf16 *>(dz), P,
+                       scale);
+    return launch_status();
+}
+
+int adrt_hip_nn_adam(float *w, const float *g, float *m, float *v, void *w16, size_t count,
+                     float lr, float beta1, float beta2, float eps, float c1, float c2,
+                     void *stream) {
+    if (!w || !g || !m || !v) return ADRT_HIP_ERR_NULL;
+    if (count == 0) return ADRT_HIP_OK;
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for((long long)count)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), w, g, m, v, static_cast<bf16 *>(w16),
+                       (long long)count, lr, beta1, beta2, eps, c1, c2);
+    return launch_status();
+}
+
+int adrt_hip_pyr_pool(const float *x, float *y, int batch, int h, int w, void *stream) {
+    if (batch < 1 || h < 2 || w < 2 || h % 2 || w % 2) return ADRT_HIP_ERR_ARG;
+    if (!x || !y) return ADRT_HIP_ERR_NULL;
+    const long long Pc = (long long)batch * (h / 2) * (w / 2);
+    hipLaunchKernelGGL(pyr_pool_kernel, dim3(blocks_for(Pc)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), x, y, Pc, h, w);
+    return launch_status();
+}
+
+int adrt_hip_pyr_stencil_up(const float *x, const float *coarse, float *y, const float *coef25,
+                            int batch, int h, int w, void *stream) {
+    if (batch < 1 || h < 1 || w < 1 || (coarse && (h % 2 || w % 2))) return ADRT_HIP_ERR_ARG;
+    if (!x || !y || !coef25) return ADRT_HIP_ERR_NULL;
+    const long long P = (long long)batch * h * w;
+    hipLaunchKernelGGL(pyr_stencil_up_kernel, dim3(blocks_for(P)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), x, coarse, y, coef25, P, h, w);
+    return launch_status();
+}
+
+int adrt_hip_pyr_correlate(const float *e, const float *w, float *partial, int parts, int batch,
+                           int h, int wd, void *stream) {
+    if (batch < 1 || h < 1 || wd < 1 || parts < 1) return ADRT_HIP_ERR_ARG;
+    if (!e || !w || !partial) return ADRT_HIP_ERR_NULL;
+    const long long P = (long long)batch * h * wd;
+    hipLaunchKernelGGL(pyr_correlate_kernel, dim3(unsigned(parts)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), e, w, partial, P, h, wd);
+    return launch_status();
+}
+
+int adrt_hip_phantom_raster(const double *params, const int *counts, int max_ellipses, float *out,
+                            int batch, int n, void *stream) {
+    if (batch < 1 || n < 1 || max_ellipses < 1) return ADRT_HIP_ERR_ARG;
+    if (!params || !counts || !out) return ADRT_HIP_ERR_NULL;
+    const long long P = (long long)batch * n * n;
+    hipLaunchKernelGGL(phantom_kernel, dim3(blocks_for(P)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), params, counts, max_ellipses, out, P, n);
+    return launch_status();
+}
+
+}  // extern "C"
+
+}  // namespace adrt_hip

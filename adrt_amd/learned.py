@@ -1,0 +1,608 @@
+"""Learned initial guess and preconditioner for CG inversion of noisy ADRT data.
+
+Problem: given d = adrt(x) + noise, solve the normal equations M x = b with
+M = mean_q truncate(bdrt(adrt(.))) (the CG recipe's operator) and
+b = mean_q truncate(bdrt(d)), to a relative residual ||b - M x|| / ||b||.
+
+Two learned parts, both applied on the GPU by this package's own kernels
+(``csrc/nn.hip``):
+
+* ``Pyramid``: a linear multiscale preconditioner
+  P r = sum_l D_l^T (h_l * (D_l r)), D_l = l-fold 2x2 averaging, h_l a 5x5
+  stencil with the symmetries of the square (6 coefficients per level, 8
+  levels).  P is symmetric by construction.  Its 48 coefficients are fitted by
+  least squares, min sum_v ||P(M v) - v||^2 over phantoms and white noise v,
+  with the gradient/Gram entries reduced on the device.
+* ``UNet``: a 4-level U-Net (channels 16/32/64/128, two 3x3 convolutions
+  per level, ReLU, 2x2 average pooling, nearest upsampling, skip
+  concatenation), BF16 weights and activations with FP32 accumulation on the
+  matrix cores.  Input z = P b; output x0 = z + net(z), trained with Adam on
+  ||x0 - x||^2 against the clean phantom x.  Forward and backward are
+  hand-written kernels; the gradients are checked against a float32 CPU
+  autograd model in the tests.
+
+``solve`` runs preconditioned CG from x0 with the float64 scalars of
+``adrt_amd.cg``; an image whose preconditioned step is not a descent step
+(P not positive definite there) or that has not converged by ``maxiter``
+continues with plain CG from its current iterate (the fallback).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _device as dev
+from . import _kernels
+from . import _lib
+from .phantom import ellipse_params
+
+# ---------------------------------------------------------------------------
+# helpers
+# ---------------------------------------------------------------------------
+
+
+def _torch():
+    return dev.torch()
+
+
+def _p(t):
+    return None if t is None else t.data_ptr()
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _check(status, what):
+    _lib.check(status, what)
+
+
+def normal_op(x):
+    """M x = mean_q truncate(bdrt(adrt(x))) for (B, n, n) float32 device images."""
+    return _kernels.truncate_combine(_kernels.bdrt(_kernels.adrt(x)), 1.0, _kernels.MEAN)
+
+
+def normal_rhs(d):
+    return _kernels.truncate_combine(_kernels.bdrt(d), 1.0, _kernels.MEAN)
+
+
+def bdot(u, v):
+    """Per-image float64 inner products of (B, n, n) tensors."""
+    return (u.double() * v.double()).sum(dim=(-2, -1))
+
+
+# ---------------------------------------------------------------------------
+# data: phantoms on the device, noisy ADRT data
+# ---------------------------------------------------------------------------
+MAX_ELLIPSES = 20
+
+
+def phantom_batch(n, seed, start, count, device=None):
+    """(count, n, n) float32 phantoms number start..start+count-1 of a seeded
+    set (the parameters of adrt_amd.phantom.ellipse_params), rasterised by the
+    device kernel."""
+    torch = _torch()
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    params = np.zeros((count, MAX_ELLIPSES, 7), dtype=np.float64)
+    counts = np.zeros(count, dtype=np.int32)
+    for i in range(count):
+        p = ellipse_params(seed, start + i)
+        k = p.shape[0]
+        counts[i] = k
+        params[i, :k, 0] = p[:, 0]
+        params[i, :k, 1] = p[:, 1]
+        params[i, :k, 2] = np.cos(p[:, 4])
+        params[i, :k, 3] = np.sin(p[:, 4])
+        params[i, :k, 4] = p[:, 2]
+        params[i, :k, 5] = p[:, 3]
+        params[i, :k, 6] = p[:, 5]
+    tp = torch.from_numpy(params).to(device)
+    tc = torch.from_numpy(counts).to(device)
+    out = torch.empty((count, n, n), dtype=torch.float32, device=device)
+    _check(_lib.load().adrt_hip_phantom_raster(_p(tp), _p(tc), MAX_ELLIPSES, _p(out), count, n,
+                                               _stream()), "phantom_raster")
+    return out
+
+
+def noisy_data(x, sigma_frac, generator):
+    """d = adrt(x) + N(0, (sigma_frac * max|adrt(x)|)^2), per image."""
+    torch = _torch()
+    a = _kernels.adrt(x)
+    amax = a.abs().amax(dim=(-3, -2, -1), keepdim=True)
+    noise = torch.randn(a.shape, generator=generator, device=a.device, dtype=a.dtype)
+    return a + sigma_frac * amax * noise
+
+
+# ---------------------------------------------------------------------------
+# multiscale preconditioner
+# ---------------------------------------------------------------------------
+_CLASSES = [(0, 0), (0, 1), (1, 1), (0, 2), (1, 2), (2, 2)]
+
+
+def _expansion():
+    """E (25 x 6): stencil entry (dy+2)*5 + dx+2 <- symmetry class."""
+    e = np.zeros((25, len(_CLASSES)))
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            key = (min(abs(dy), abs(dx)), max(abs(dy), abs(dx)))
+            e[(dy + 2) * 5 + dx + 2, _CLASSES.index(key)] = 1.0
+    return e
+
+
+class Pyramid:
+    """P r = sum_l D_l^T (h_l * D_l r) on (B, n, n) float32 device images."""
+
+    def __init__(self, levels=8, theta=None, device=None):
+        torch = _torch()
+        self.levels = levels
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.E = _expansion()
+        if theta is None:
+            theta = np.zeros((levels, len(_CLASSES)))
+            theta[0, 0] = 1.0
+        self.set_theta(theta)
+
+    def set_theta(self, theta):
+        torch = _torch()
+        self.theta = np.asarray(theta, dtype=np.float64).reshape(self.levels, len(_CLASSES))
+        coef = self.theta @ self.E.T  # (levels, 25)
+        self.coef = torch.from_numpy(coef.astype(np.float32)).to(self.device)
+
+    def _pools(self, r):
+        torch = _torch()
+        lib = _lib.load()
+        out = [r]
+        for _ in range(1, self.levels):
+            x = out[-1]
+            b, h, w = x.shape
+            y = torch.empty((b, h // 2, w // 2), dtype=x.dtype, device=x.device)
+            _check(lib.adrt_hip_pyr_pool(_p(x), _p(y), b, h, w, _stream()), "pyr_pool")
+            out.append(y)
+        return out
+
+    def apply(self, r, coef=None):
+        torch = _torch()
+        lib = _lib.load()
+        coef = self.coef if coef is None else coef
+        pyr = self._pools(r.contiguous())
+        acc = None
+        for lvl in range(self.levels - 1, -1, -1):
+            x = pyr[lvl]
+            b, h, w = x.shape
+            y = torch.empty_like(x)
+            _check(lib.adrt_hip_pyr_stencil_up(_p(x), _p(acc), _p(y), _p(coef[lvl]), b, h, w,
+                                               _stream()), "pyr_stencil_up")
+            acc = y
+        return acc
+
+    __call__ = apply
+
+    def _correlations(self, e, w, parts=256):
+        """(levels, 25) sums over the batch of sum_p (D_l e)[p] (D_l w)[p + o]."""
+        torch = _torch()
+        lib = _lib.load()
+        pe, pw = self._pools(e), self._pools(w)
+        out = torch.empty((self.levels, parts, 25), dtype=torch.float32, device=e.device)
+        for lvl in range(self.levels):
+            b, h, wd = pe[lvl].shape
+            _check(lib.adrt_hip_pyr_correlate(_p(pe[lvl]), _p(pw[lvl]), _p(out[lvl]), parts, b, h,
+                                              wd, _stream()), "pyr_correlate")
+        return out.double().sum(dim=1).cpu().numpy()
+
+    def fit(self, vectors, op=normal_op, ridge=1e-8):
+        """Least-squares coefficients for min ||P(op(v)) - v||^2 over the
+        batch ``vectors`` (B, n, n).  The gradient of the objective is
+        (sum_p D_l e * shifted D_l w) tied over the symmetry classes, with
+        e = P w - v, w = op(v); it is affine in the coefficients, so 49
+        evaluations give the normal equations exactly."""
+        torch = _torch()
+        w = op(vectors)
+        nparam = self.levels * len(_CLASSES)
+        zero = torch.zeros((self.levels, 25), dtype=torch.float32, device=self.device)
+
+        def grad(coef):
+            e = self.apply(w, coef) - vectors
+            return (self._correlations(e, w) @ self.E).reshape(-1)
+
+        c = -grad(zero)
+        G = np.empty((nparam, nparam))
+        for k in range(nparam):
+            th = np.zeros(nparam)
+            th[k] = 1.0
+            coef = torch.from_numpy((th.reshape(self.levels, -1) @ self.E.T).astype(np.float32))
+            G[:, k] = grad(coef.to(self.device)) + c
+        G = 0.5 * (G + G.T)
+        G += ridge * np.trace(G) / nparam * np.eye(nparam)
+        self.set_theta(np.linalg.solve(G, c))
+        return self.theta
+
+
+# ---------------------------------------------------------------------------
+# U-Net
+# ---------------------------------------------------------------------------
+# (name, cin, cout): cin/cout as stored (multiples of 8 / 16); the first
+# layer's real input is 1 channel (padded to 8), the last layer's real
+# output 1 channel (padded to 16).
+LAYERS = [
+    ("e0a", 8, 16), ("e0b", 16, 16),
+    ("e1a", 16, 32), ("e1b", 32, 32),
+    ("e2a", 32, 64), ("e2b", 64, 64),
+    ("b3a", 64, 128), ("b3b", 128, 128),
+    ("d2a", 192, 64), ("d2b", 64, 64),
+    ("d1a", 96, 32), ("d1b", 32, 32),
+    ("d0a", 48, 16), ("d0b", 16, 16),
+    ("out", 16, 16),
+]
+REAL_CIN = {"e0a": 1}
+REAL_COUT = {"out": 1}
+
+
+class UNet:
+    """Parameters (fp32 master + bf16 copy), Adam state, and the per-batch
+    activation / gradient buffers of the U-Net at side n."""
+
+    def __init__(self, seed=0, device=None):
+        torch = _torch()
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.offsets = {}
+        off = 0
+        for name, cin, cout in LAYERS:
+            nw = cout * 9 * cin
+            self.offsets[name] = (off, off + nw, cin, cout)
+            off += nw + cout
+        self.count = off
+        rng = np.random.default_rng(seed)
+        flat = np.zeros(self.count, dtype=np.float32)
+        for name, cin, cout in LAYERS:
+            w0, b0, _, _ = self.offsets[name]
+            rcin, rcout = REAL_CIN.get(name, cin), REAL_COUT.get(name, cout)
+            std = math.sqrt(2.0 / (9 * rcin)) if name != "out" else 1e-3
+            w = np.zeros((cout, 9, cin), dtype=np.float32)
+            w[:rcout, :, :rcin] = rng.standard_normal((rcout, 9, rcin)) * std
+            flat[w0:w0 + w.size] = w.reshape(-1)
+        self.w = torch.from_numpy(flat).to(self.device)
+        self.w16 = self.w.to(torch.bfloat16)
+        self.grad = torch.zeros_like(self.w)
+        self.m = torch.zeros_like(self.w)
+        self.v = torch.zeros_like(self.w)
+        self.step_count = 0
+        self.wt = {}
+        self._buffers = {}
+        self.refresh_transposed()
+
+    # -- parameters ----------------------------------------------------------
+    def weight(self, name):
+        w0, _, cin, cout = self.offsets[name]
+        return self.w16[w0:w0 + cout * 9 * cin]
+
+    def bias(self, name):
+        _, b0, _, cout = self.offsets[name]
+        return self.w[b0:b0 + cout]
+
+    def grads(self, name):
+        w0, b0, cin, cout = self.offsets[name]
+        return self.grad[w0:w0 + cout * 9 * cin], self.grad[b0:b0 + cout]
+
+    def refresh_transposed(self):
+        torch = _torch()
+        lib = _lib.load()
+        for name, cin, cout in LAYERS:
+            if name == "e0a":
+                continue
+            if name not in self.wt:
+                self.wt[name] = torch.empty(cout * 9 * cin, dtype=torch.bfloat16,
+                                            device=self.device)
+            _check(lib.adrt_hip_nn_transpose_weights(_p(self.weight(name)), _p(self.wt[name]),
+                                                     cin, cout, _stream()), "transpose_weights")
+
+    def state_dict(self):
+        return {"w": self.w.cpu().numpy(), "m": self.m.cpu().numpy(), "v": self.v.cpu().numpy(),
+                "step": self.step_count}
+
+    def load_state(self, w, m=None, v=None, step=0):
+        torch = _torch()
+        self.w.copy_(torch.as_tensor(np.asarray(w, dtype=np.float32)))
+        self.w16.copy_(self.w.to(torch.bfloat16))
+        if m is not None:
+            self.m.copy_(torch.as_tensor(np.asarray(m, dtype=np.float32)))
+            self.v.copy_(torch.as_tensor(np.asarray(v, dtype=np.float32)))
+        self.step_count = int(step)
+        self.refresh_transposed()
+
+    # -- buffers -------------------------------------------------------------
+    def buffers(self, B, n):
+        key = (B, n)
+        if key in self._buffers:
+            return self._buffers[key]
+        torch = _torch()
+        bf = torch.bfloat16
+        d = self.device
+
+        def t(hw, c):
+            return torch.empty((B, hw, hw, c), dtype=bf, device=d)
+
+        n1, n2, n3 = n // 2, n // 4, n // 8
+        buf = {
+            "x8": t(n, 8), "t0": t(n, 16), "cat0": t(n, 48), "p0": t(n1, 16),
+            "t1": t(n1, 32), "cat1": t(n1, 96), "p1": t(n2, 32),
+            "t2": t(n2, 64), "cat2": t(n2, 192), "p2": t(n3, 64),
+            "t3": t(n3, 128), "b3": t(n3, 128),
+            "t2d": t(n2, 64), "d2": t(n2, 64), "t1d": t(n1, 32), "d1": t(n1, 32),
+            "t0d": t(n, 16), "d0": t(n, 16),
+            "y": torch.empty((B, n, n), dtype=torch.float32, device=d),
+            # gradients
+            "g_out": t(n, 16), "g16a": t(n, 16), "g16b": t(n, 16), "g_cat0": t(n, 48),
+            "g32a": t(n1, 32), "g32b": t(n1, 32), "g_cat1": t(n1, 96),
+            "g64a": t(n2, 64), "g64b": t(n2, 64), "g_cat2": t(n2, 192),
+            "g128a": t(n3, 128), "g128b": t(n3, 128), "g_p2": t(n3, 64),
+            "g_p1": t(n2, 32), "g_p0": t(n1, 16),
+        }
+        lib = _lib.load()
+        ws = 0
+        for name, cin, cout in LAYERS:
+            side = {"e0": n, "d0": n, "ou": n, "e1": n1, "d1": n1, "e2": n2, "d2": n2,
+                    "b3": n3}[name[:2]]
+            ws = max(ws, int(lib.adrt_hip_nn_wgrad_workspace(B, side, side, cin, cout)))
+        buf["wg_ws"] = torch.empty(max(ws, 16), dtype=torch.uint8, device=d)
+        self._buffers[key] = buf
+        return buf
+
+    # -- kernels -------------------------------------------------------------
+    def _conv(self, x, xs, name, y, ys, B, h, mode, mask=None, ms=0, y32=None, resid=None,
+              transposed=False):
+        _, _, cin, cout = self.offsets[name]
+        if transposed:
+            w, bias, cin, cout = self.wt[name], None, cout, cin
+        else:
+            w, bias = self.weight(name), self.bias(name)
+        _check(_lib.load().adrt_hip_nn_conv3x3(_p(x), xs, _p(w), _p(bias), _p(y), ys, _p(y32),
+                                               _p(resid), _p(mask), ms, B, h, h, cin, cout, mode,
+                                               _stream()), f"conv {name}")
+
+    def _wgrad(self, x, xs, dz, dzs, name, B, h, ws):
+        _, _, cin, cout = self.offsets[name]
+        dw, db = self.grads(name)
+        _check(_lib.load().adrt_hip_nn_wgrad(_p(x), xs, _p(dz), dzs, _p(dw), _p(db), _p(ws),
+                                             ws.numel(), B, h, h, cin, cout, _stream()),
+               f"wgrad {name}")
+
+    def forward(self, z):
+        """x0 = z + net(z) for (B, n, n) float32 device images z; keeps the
+        activations for backward()."""
+        lib = _lib.load()
+        B, n, _ = z.shape
+        z = z.contiguous()
+        f = self.buffers(B, n)
+        n1, n2, n3 = n // 2, n // 4, n // 8
+        s = _stream()
+        cat0, cat1, cat2 = f["cat0"], f["cat1"], f["cat2"]
+        _check(lib.adrt_hip_nn_pack_input(_p(z), _p(f["x8"]), B, n, n, 1.0, s), "pack_input")
+        self._conv(f["x8"], 8, "e0a", f["t0"], 16, B, n, 0)
+        self._conv(f["t0"], 16, "e0b", cat0, 48, B, n, 0)
+        _check(lib.adrt_hip_nn_pool(_p(cat0), 48, _p(f["p0"]), 16, B, n, n, 16, s), "pool")
+        self._conv(f["p0"], 16, "e1a", f["t1"], 32, B, n1, 0)
+        self._conv(f["t1"], 32, "e1b", cat1, 96, B, n1, 0)
+        _check(lib.adrt_hip_nn_pool(_p(cat1), 96, _p(f["p1"]), 32, B, n1, n1, 32, s), "pool")
+        self._conv(f["p1"], 32, "e2a", f["t2"], 64, B, n2, 0)
+        self._conv(f["t2"], 64, "e2b", cat2, 192, B, n2, 0)
+        _check(lib.adrt_hip_nn_pool(_p(cat2), 192, _p(f["p2"]), 64, B, n2, n2, 64, s), "pool")
+        self._conv(f["p2"], 64, "b3a", f["t3"], 128, B, n3, 0)
+        self._conv(f["t3"], 128, "b3b", f["b3"], 128, B, n3, 0)
+        _check(lib.adrt_hip_nn_upsample(_p(f["b3"]), 128, _p(cat2[..., 64:]), 192, B, n3, n3, 128,
+                                        s), "upsample")
+        self._conv(cat2, 192, "d2a", f["t2d"], 64, B, n2, 0)
+        self._conv(f["t2d"], 64, "d2b", f["d2"], 64, B, n2, 0)
+        _check(lib.adrt_hip_nn_upsample(_p(f["d2"]), 64, _p(cat1[..., 32:]), 96, B, n2, n2, 64, s),
+               "upsample")
+        self._conv(cat1, 96, "d1a", f["t1d"], 32, B, n1, 0)
+        self._conv(f["t1d"], 32, "d1b", f["d1"], 32, B, n1, 0)
+        _check(lib.adrt_hip_nn_upsample(_p(f["d1"]), 32, _p(cat0[..., 16:]), 48, B, n1, n1, 32, s),
+               "upsample")
+        self._conv(cat0, 48, "d0a", f["t0d"], 16, B, n, 0)
+        self._conv(f["t0d"], 16, "d0b", f["d0"], 16, B, n, 0)
+        self._conv(f["d0"], 16, "out", None, 0, B, n, 1, y32=f["y"], resid=z)
+        return f["y"]
+
+    def backward(self, g_scale_target, B, n):
+        """Gradients of 0.5 * scale * ||y - target||^2 (scale, target) with
+        respect to every parameter, into self.grad (uses the activations of
+        the last forward at this (B, n))."""
+        scale, target = g_scale_target
+        lib = _lib.load()
+        f = self.buffers(B, n)
+        n1, n2, n3 = n // 2, n // 4, n // 8
+        s = _stream()
+        ws = f["wg_ws"]
+        cat0, cat1, cat2 = f["cat0"], f["cat1"], f["cat2"]
+        g_cat0, g_cat1, g_cat2 = f["g_cat0"], f["g_cat1"], f["g_cat2"]
+        _check(lib.adrt_hip_nn_loss_grad(_p(f["y"]), _p(target), _p(f["g_out"]), B, n, n,
+                                         float(scale), s), "loss_grad")
+        # output layer and level-0 decoder
+        self._wgrad(f["d0"], 16, f["g_out"], 16, "out", B, n, ws)
+        self._conv(f["g_out"], 16, "out", f["g16a"], 16, B, n, 2, mask=f["d0"], ms=16,
+                   transposed=True)
+        self._wgrad(f["t0d"], 16, f["g16a"], 16, "d0b", B, n, ws)
+        self._conv(f["g16a"], 16, "d0b", f["g16b"], 16, B, n, 2, mask=f["t0d"], ms=16,
+                   transposed=True)
+        self._wgrad(cat0, 48, f["g16b"], 16, "d0a", B, n, ws)
+        self._conv(f["g16b"], 16, "d0a", g_cat0, 48, B, n, 2, mask=cat0, ms=48, transposed=True)
+        _check(lib.adrt_hip_nn_upsample_backward(_p(g_cat0[..., 16:]), 48, _p(f["g32a"]), 32, B,
+                                                 n1, n1, 32, s), "upsample_backward")
+        # level-1 decoder
+        self._wgrad(f["t1d"], 32, f["g32a"], 32, "d1b", B, n1, ws)
+        self._conv(f["g32a"], 32, "d1b", f["g32b"], 32, B, n1, 2, mask=f["t1d"], ms=32,
+                   transposed=True)
+        self._wgrad(cat1, 96, f["g32b"], 32, "d1a", B, n1, ws)
+        self._conv(f["g32b"], 32, "d1a", g_cat1, 96, B, n1, 2, mask=cat1, ms=96, transposed=True)
+        _check(lib.adrt_hip_nn_upsample_backward(_p(g_cat1[..., 32:]), 96, _p(f["g64a"]), 64, B,
+                                                 n2, n2, 64, s), "upsample_backward")
+        # level-2 decoder
+        self._wgrad(f["t2d"], 64, f["g64a"], 64, "d2b", B, n2, ws)
+        self._conv(f["g64a"], 64, "d2b", f["g64b"], 64, B, n2, 2, mask=f["t2d"], ms=64,
+                   transposed=True)
+        self._wgrad(cat2, 192, f["g64b"], 64, "d2a", B, n2, ws)
+        self._conv(f["g64b"], 64, "d2a", g_cat2, 192, B, n2, 2, mask=cat2, ms=192,
+                   transposed=True)
+        _check(lib.adrt_hip_nn_upsample_backward(_p(g_cat2[..., 64:]), 192, _p(f["g128a"]), 128,
+                                                 B, n3, n3, 128, s), "upsample_backward")
+        # bottleneck
+        self._wgrad(f["t3"], 128, f["g128a"], 128, "b3b", B, n3, ws)
+        self._conv(f["g128a"], 128, "b3b", f["g128b"], 128, B, n3, 2, mask=f["t3"], ms=128,
+                   transposed=True)
+        self._wgrad(f["p2"], 64, f["g128b"], 128, "b3a", B, n3, ws)
+        self._conv(f["g128b"], 128, "b3a", f["g_p2"], 64, B, n3, 3, transposed=True)
+        # level-2 encoder
+        _check(lib.adrt_hip_nn_pool_backward(_p(f["g_p2"]), 64, _p(g_cat2), 192, _p(cat2), 192,
+                                             _p(f["g64a"]), 64, B, n2, n2, 64, s), "pool_backward")
+        self._wgrad(f["t2"], 64, f["g64a"], 64, "e2b", B, n2, ws)
+        self._conv(f["g64a"], 64, "e2b", f["g64b"], 64, B, n2, 2, mask=f["t2"], ms=64,
+                   transposed=True)
+        self._wgrad(f["p1"], 32, f["g64b"], 64, "e2a", B, n2, ws)
+        self._conv(f["g64b"], 64, "e2a", f["g_p1"], 32, B, n2, 3, transposed=True)
+        # level-1 encoder
+        _check(lib.adrt_hip_nn_pool_backward(_p(f["g_p1"]), 32, _p(g_cat1), 96, _p(cat1), 96,
+                                             _p(f["g32a"]), 32, B, n1, n1, 32, s), "pool_backward")
+        self._wgrad(f["t1"], 32, f["g32a"], 32, "e1b", B, n1, ws)
+        self._conv(f["g32a"], 32, "e1b", f["g32b"], 32, B, n1, 2, mask=f["t1"], ms=32,
+                   transposed=True)
+        self._wgrad(f["p0"], 16, f["g32b"], 32, "e1a", B, n1, ws)
+        self._conv(f["g32b"], 32, "e1a", f["g_p0"], 16, B, n1, 3, transposed=True)
+        # level-0 encoder
+        _check(lib.adrt_hip_nn_pool_backward(_p(f["g_p0"]), 16, _p(g_cat0), 48, _p(cat0), 48,
+                                             _p(f["g16a"]), 16, B, n, n, 16, s), "pool_backward")
+        self._wgrad(f["t0"], 16, f["g16a"], 16, "e0b", B, n, ws)
+        self._conv(f["g16a"], 16, "e0b", f["g16b"], 16, B, n, 2, mask=f["t0"], ms=16,
+                   transposed=True)
+        self._wgrad(f["x8"], 8, f["g16b"], 16, "e0a", B, n, ws)
+        return self.grad
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step_count += 1
+        t = self.step_count
+        _check(_lib.load().adrt_hip_nn_adam(_p(self.w), _p(self.grad), _p(self.m), _p(self.v),
+                                            _p(self.w16), self.count, lr, beta1, beta2, eps,
+                                            1.0 - beta1 ** t, 1.0 - beta2 ** t, _stream()),
+               "adam")
+        self.refresh_transposed()
+
+    def loss(self, target):
+        y = self._buffers[(target.shape[0], target.shape[-1])]["y"]
+        return float(((y.double() - target.double()) ** 2).mean())
+
+
+# ---------------------------------------------------------------------------
+# the learned solver
+# ---------------------------------------------------------------------------
+class LearnedSolver:
+    """Pyramid preconditioner + U-Net initial guess for images of side n."""
+
+    def __init__(self, n=256, sigma_frac=0.005, device=None, seed=0):
+        torch = _torch()
+        self.n = n
+        self.sigma_frac = sigma_frac
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.pyramid = Pyramid(levels=int(math.log2(n)), device=self.device)
+        self.net = UNet(seed=seed, device=self.device)
+
+    def data(self, seed, start, count, noise_seed):
+        torch = _torch()
+        x = phantom_batch(self.n, seed, start, count, self.device)
+        g = torch.Generator(device=self.device)
+        g.manual_seed(int(noise_seed))
+        d = noisy_data(x, self.sigma_frac, g)
+        return x, normal_rhs(d)
+
+    def fit_preconditioner(self, seed=1, count=16):
+        """Least-squares fit of the pyramid on phantoms and white noise."""
+        torch = _torch()
+        x = phantom_batch(self.n, seed, 0, count, self.device)
+        g = torch.Generator(device=self.device)
+        g.manual_seed(seed)
+        noise = torch.randn(x.shape, generator=g, device=self.device)
+        return self.pyramid.fit(torch.cat([x, noise]))
+
+    def initial_guess(self, b):
+        z = self.pyramid(b)
+        return self.net.forward(z)
+
+    def train(self, steps, batch=64, seed=1000, pairs=50000, lr=1e-3, log_every=0, log=None):
+        """Adam on 0.5 * mean ||x0 - x||^2 over seeded phantom pairs (pair i =
+        phantom i of ``seed`` with noise seed (seed, i)); batches cycle through
+        ``pairs`` pairs.  Returns the loss history (one value per logged step)."""
+        hist = []
+        n = self.n
+        nb = max(pairs // batch, 1)
+        for step in range(steps):
+            k = step % nb
+            x, b = self.data(seed, k * batch, batch, noise_seed=seed * 100003 + k)
+            self.initial_guess(b)
+            self.net.backward((1.0 / (batch * n * n), x), batch, n)
+            self.net.adam_step(lr=lr)
+            if log_every and (step % log_every == 0 or step == steps - 1):
+                hist.append((step, self.net.loss(x)))
+                if log:
+                    log(step, hist[-1][1])
+        return hist
+
+
+def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500):
+    """Batched preconditioned CG (plain CG when prec is None) for (B, n, n)
+    systems with float64 scalars per image.  Stops each image when
+    ||b - M x|| <= rtol * ||b||.  An image whose preconditioned step fails
+    (r.z <= 0 or p.Mp <= 0) restarts as plain CG from its iterate.  Returns
+    (x, iterations per image, converged mask, operator applications)."""
+    torch = _torch()
+    B = b.shape[0]
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    r = b - op(x) if x0 is not None else b.clone()
+    bn = bdot(b, b).sqrt()
+    tol = rtol * bn
+    use_p = torch.full((B,), prec is not None, dtype=torch.bool, device=b.device)
+
+    def precond(rr):
+        if prec is None:
+            return rr
+        z = prec(rr)
+        return torch.where(use_p[:, None, None], z, rr)
+
+    z = precond(r)
+    p = z.clone()
+    rz = bdot(r, z)
+    bad = rz <= 0
+    if bool(bad.any()):
+        use_p &= ~bad
+        z = precond(r)
+        p = z.clone()
+        rz = bdot(r, z)
+    iters = torch.zeros(B, dtype=torch.int64, device=b.device)
+    active = bdot(r, r).sqrt() > tol
+    apps = 0
+    for _ in range(maxiter):
+        if not bool(active.any()):
+            break
+        q = op(p)
+        apps += 1
+        pq = bdot(p, q)
+        ok = active & (pq > 0)
+        alpha = torch.where(ok, rz / torch.where(pq > 0, pq, torch.ones_like(pq)),
+                            torch.zeros_like(pq))
+        a = alpha.to(b.dtype)[:, None, None]
+        x = x + a * p
+        r = r - a * q
+        iters += active.long()
+        active = active & (bdot(r, r).sqrt() > tol)
+        z = precond(r)
+        rz_new = bdot(r, z)
+        bad = active & ((rz_new <= 0) | (pq <= 0)) & use_p
+        if bool(bad.any()):
+            use_p &= ~bad
+            z = precond(r)
+            rz_new = bdot(r, z)
+            beta = torch.where(bad, torch.zeros_like(rz), rz_new / rz)
+        else:
+            beta = rz_new / torch.where(rz != 0, rz, torch.ones_like(rz))
+        p = z + beta.to(b.dtype)[:, None, None] * p
+        rz = rz_new
+    converged = ~active
+    return x, iters, converged, apps

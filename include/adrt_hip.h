@@ -189,6 +189,87 @@ int adrt_hip_set_config(int op, size_t n, int variant, size_t group_bytes);
 /* Reads the configuration of (op, n) (group_bytes: the budget in effect). */
 int adrt_hip_get_config(int op, size_t n, int *variant, size_t *group_bytes);
 
+/* ---- learned CG initial guess / preconditioner (no reference counterpart) --
+ *
+ * Kernels of the 4-level U-Net that supplies CG's initial guess, its
+ * training, the multiscale linear preconditioner and the on-device phantom
+ * generator.  Activations are NHWC bf16 (2-byte elements) with a pixel
+ * stride in elements (a multiple of 8, >= the channel count); weights are
+ * bf16 [cout][9][cin] (cin a multiple of 8); biases and gradients fp32.
+ * Pointers to bf16 data must be 16-byte aligned. */
+
+/* 3x3 convolution with zero padding, on the matrix cores (bf16 in, fp32
+ * accumulation).  mode 0: y = relu(conv + bias); 1: y32[p] = resid[p] +
+ * conv[p][0] + bias[0] (fp32 output, channel 0); 2: y = conv * (mask > 0);
+ * 3: y = conv.  Modes 2 and 3 with the weights of
+ * adrt_hip_nn_transpose_weights give the input gradient.  cout % 16 == 0,
+ * batch*h*w_ % 64 == 0. */
+int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float *bias, void *y,
+                        int y_stride, float *y32, const float *resid, const void *mask,
+                        int mask_stride, int batch, int h, int w_, int cin, int cout, int mode,
+                        void *stream);
+
+/* Workspace bytes of adrt_hip_nn_wgrad. */
+size_t adrt_hip_nn_wgrad_workspace(int batch, int h, int w, int cin, int cout);
+
+/* Weight and bias gradients of a 3x3 convolution: dw[co][t][ci] =
+ * sum_p dz[p][co] * x[p + tap t][ci], dbias[co] = sum_p dz[p][co]
+ * (overwritten, fp32). */
+int adrt_hip_nn_wgrad(const void *x, int x_stride, const void *dz, int dz_stride, float *dw,
+                      float *dbias, void *workspace, size_t workspace_bytes, int batch, int h,
+                      int w, int cin, int cout, void *stream);
+
+/* wt[ci][8 - t][co] = w[co][t][ci]: weights of the input-gradient convolution. */
+int adrt_hip_nn_transpose_weights(const void *w, void *wt, int cin, int cout, void *stream);
+
+/* 2x2 average pooling (h, w: input size, even). */
+int adrt_hip_nn_pool(const void *x, int x_stride, void *y, int y_stride, int batch, int h, int w,
+                     int c, void *stream);
+
+/* ds = (dp[parent] / 4 + dskip) * (s > 0), at the fine size h x w. */
+int adrt_hip_nn_pool_backward(const void *dp, int dp_stride, const void *dskip, int dskip_stride,
+                              const void *s, int s_stride, void *ds, int ds_stride, int batch,
+                              int h, int w, int c, void *stream);
+
+/* Nearest 2x upsampling of an h x w input. */
+int adrt_hip_nn_upsample(const void *x, int x_stride, void *y, int y_stride, int batch, int h,
+                         int w, int c, void *stream);
+
+/* dx = sum over the 2x2 children of dy (h x w: the coarse size). */
+int adrt_hip_nn_upsample_backward(const void *dy, int dy_stride, void *dx, int dx_stride,
+                                  int batch, int h, int w, int c, void *stream);
+
+/* fp32 images -> 8-channel bf16 NHWC (channel 0 = scale * x, others 0). */
+int adrt_hip_nn_pack_input(const float *x, void *y, int batch, int h, int w, float scale,
+                           void *stream);
+
+/* 16-channel bf16 gradient: channel 0 = scale * (pred - target), others 0. */
+int adrt_hip_nn_loss_grad(const float *pred, const float *target, void *dz, int batch, int h,
+                          int w, float scale, void *stream);
+
+/* One Adam step on count fp32 parameters (c1, c2: the bias corrections
+ * 1 - beta^t); w16, if not NULL, receives the bf16 copy. */
+int adrt_hip_nn_adam(float *w, const float *g, float *m, float *v, void *w16, size_t count,
+                     float lr, float beta1, float beta2, float eps, float c1, float c2,
+                     void *stream);
+
+/* Multiscale preconditioner pieces (fp32 images, batch x h x w):
+ * 2x2 averaging; y = coef25 (5x5, zero padding) applied to x, plus
+ * coarse[parent] / 4 when coarse is not NULL; and partial[block][o] =
+ * sum_p e[p] * x[p + o] over o in the 5x5 window (parts blocks). */
+int adrt_hip_pyr_pool(const float *x, float *y, int batch, int h, int w, void *stream);
+int adrt_hip_pyr_stencil_up(const float *x, const float *coarse, float *y, const float *coef25,
+                            int batch, int h, int w, void *stream);
+int adrt_hip_pyr_correlate(const float *e, const float *x, float *partial, int parts, int batch,
+                           int h, int w, void *stream);
+
+/* Ellipse phantoms: out[b][i][j] = sum of val over the ellipses of image b
+ * (params[b][k] = cx, cy, cos, sin, ax, ay, val as float64; counts[b] <=
+ * max_ellipses) whose closed interior holds the pixel centre
+ * (-1 + (2j+1)/n, 1 - (2i+1)/n). */
+int adrt_hip_phantom_raster(const double *params, const int *counts, int max_ellipses, float *out,
+                            int batch, int n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

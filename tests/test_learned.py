@@ -1,0 +1,488 @@
+"""Learned CG initial guess and preconditioner (adrt_amd.learned, csrc/nn.hip).
+
+The network kernels are checked against a float32 CPU reference of the same
+operation (torch on the CPU, fed the same bf16-rounded values); the whole
+U-Net's forward and parameter gradients against a CPU autograd model with
+the same weights; the multiscale preconditioner against a NumPy float64
+restatement; the solver against plain CG; and training against the method
+without the network (the network must cut the preconditioned CG iterations
+on held-out data)."""
+
+import numpy as np
+import pytest
+
+from adrt_amd import learned as LN
+
+gpu = pytest.mark.gpu
+
+
+# ---------------------------------------------------------------------------
+# host-side bookkeeping (no GPU)
+# ---------------------------------------------------------------------------
+def test_symmetry_classes_cover_stencil():
+    E = LN._expansion()
+    assert E.shape == (25, 6) and np.all(E.sum(axis=1) == 1)
+    # class sizes: centre 1, axis-1 4, diag-1 4, axis-2 4, knight 8, diag-2 4
+    assert list(E.sum(axis=0)) == [1, 4, 4, 4, 8, 4]
+    k = (E @ np.arange(6)).reshape(5, 5)
+    assert np.array_equal(k, k.T) and np.array_equal(k, k[::-1]) and np.array_equal(k, k[:, ::-1])
+
+
+def test_layer_channels_chain():
+    ch = {name: (cin, cout) for name, cin, cout in LN.LAYERS}
+    # decoder inputs are skip + upsampled channels
+    assert ch["d2a"][0] == ch["e2b"][1] + ch["b3b"][1]
+    assert ch["d1a"][0] == ch["e1b"][1] + ch["d2b"][1]
+    assert ch["d0a"][0] == ch["e0b"][1] + ch["d1b"][1]
+    for a, b in [("e0a", "e0b"), ("e1a", "e1b"), ("e2a", "e2b"), ("b3a", "b3b"),
+                 ("d2a", "d2b"), ("d1a", "d1b"), ("d0a", "d0b"), ("d0b", "out")]:
+        assert ch[a][1] == ch[b][0]
+    assert [ch[n][1] for n in ("e0b", "e1b", "e2b", "b3b")] == [16, 32, 64, 128]
+    assert all(cin % 8 == 0 and cout % 16 == 0 for cin, cout in ch.values())
+
+
+def np_pyramid(r, coef):
+    """P r = sum_l D_l^T (h_l * D_l r) in float64 (D: 2x2 mean, D^T: nearest / 4)."""
+    levels = coef.shape[0]
+    pyr = [np.asarray(r, np.float64)]
+    for _ in range(1, levels):
+        x = pyr[-1]
+        pyr.append(0.25 * ((x[:, ::2, ::2] + x[:, ::2, 1::2]) + (x[:, 1::2, ::2] + x[:, 1::2, 1::2])))
+    acc = None
+    for lvl in range(levels - 1, -1, -1):
+        x = pyr[lvl]
+        b, h, w = x.shape
+        xp = np.pad(x, ((0, 0), (2, 2), (2, 2)))
+        y = np.zeros_like(x)
+        k = coef[lvl].reshape(5, 5)
+        for dy in range(5):
+            for dx in range(5):
+                y += k[dy, dx] * xp[:, dy:dy + h, dx:dx + w]
+        if acc is not None:
+            y += 0.25 * np.repeat(np.repeat(acc, 2, 1), 2, 2)
+        acc = y
+    return acc
+
+
+def test_np_pyramid_is_symmetric():
+    """The restatement itself: <P x, y> = <x, P y> for symmetric stencils."""
+    rng = np.random.default_rng(0)
+    coef = rng.standard_normal((5, 6)) @ LN._expansion().T
+    x, y = rng.standard_normal((2, 1, 32, 32))
+    assert abs(np.vdot(np_pyramid(x, coef), y) - np.vdot(x, np_pyramid(y, coef))) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# kernels vs CPU references
+# ---------------------------------------------------------------------------
+def _bf(t):
+    import torch
+
+    return t.to(torch.bfloat16).float()
+
+
+def _nchw_weight(w, cin, cout):
+    """[cout][9][cin] -> (cout, cin, 3, 3) (tap = ky * 3 + kx)."""
+    return w.reshape(cout, 3, 3, cin).permute(0, 3, 1, 2).contiguous()
+
+
+def _conv_call(x, xs, w, bias, y, ys, B, h, wd, cin, cout, mode, mask=None, ms=0, y32=None,
+               resid=None):
+    from adrt_amd import _lib
+
+    LN._check(_lib.load().adrt_hip_nn_conv3x3(LN._p(x), xs, LN._p(w), LN._p(bias), LN._p(y), ys,
+                                              LN._p(y32), LN._p(resid), LN._p(mask), ms, B, h, wd,
+                                              cin, cout, mode, LN._stream()), "conv")
+
+
+def _bf16_close(got, want):
+    got, want = got.float(), want.float()
+    tol = 2.0 ** -7 * want.abs() + 1e-5
+    return bool(((got - want).abs() <= tol).all())
+
+
+SHAPES = [(8, 16), (16, 16), (16, 32), (48, 16), (32, 64), (64, 128), (96, 32), (192, 64),
+          (128, 128)]
+
+
+@gpu
+class TestConvKernels:
+    @pytest.mark.parametrize("cin,cout", SHAPES)
+    @pytest.mark.parametrize("hw", [(16, 16), (8, 8), (4, 32)])
+    def test_forward_relu_bias(self, cuda, cin, cout, hw):
+        import torch
+        import torch.nn.functional as F
+
+        h, wd = hw
+        B = 2
+        g = torch.Generator().manual_seed(cin * 1000 + cout + h)
+        x = _bf(torch.randn(B, h, wd, cin, generator=g))
+        w = _bf(torch.randn(cout, 9, cin, generator=g) / np.sqrt(9 * cin))
+        bias = torch.randn(cout, generator=g) * 0.1
+        want = _bf(F.relu(F.conv2d(x.permute(0, 3, 1, 2), _nchw_weight(w, cin, cout), bias,
+                                   padding=1)).permute(0, 2, 3, 1))
+        xd = x.to(cuda, torch.bfloat16).contiguous()
+        wdv = w.to(cuda, torch.bfloat16).contiguous()
+        bd = bias.to(cuda)
+        y = torch.full((B, h, wd, cout), float("nan"), device=cuda, dtype=torch.bfloat16)
+        _conv_call(xd, cin, wdv, bd, y, cout, B, h, wd, cin, cout, 0)
+        assert _bf16_close(y.cpu(), want)
+
+    def test_strided_input_and_output(self, cuda):
+        """Channels at an offset inside a wider pixel stride (the skip
+        concatenation buffers), untouched channels preserved."""
+        import torch
+        import torch.nn.functional as F
+
+        B, h, cin, cout = 2, 16, 16, 32
+        g = torch.Generator().manual_seed(5)
+        buf = _bf(torch.randn(B, h, h, 48, generator=g))
+        w = _bf(torch.randn(cout, 9, cin, generator=g) / 12)
+        bias = torch.zeros(cout)
+        out = torch.full((B, h, h, 96), 7.0, dtype=torch.bfloat16, device=cuda)
+        bd = buf.to(cuda, torch.bfloat16)
+        _conv_call(bd[..., 16:], 48, w.to(cuda, torch.bfloat16), bias.to(cuda), out[..., 32:], 96,
+                   B, h, h, cin, cout, 0)
+        want = _bf(F.relu(F.conv2d(buf[..., 16:32].permute(0, 3, 1, 2), _nchw_weight(w, cin, cout),
+                                   padding=1)).permute(0, 2, 3, 1))
+        o = out.cpu()
+        assert _bf16_close(o[..., 32:64], want)
+        assert torch.all(o[..., :32] == 7.0) and torch.all(o[..., 64:] == 7.0)
+
+    def test_residual_output(self, cuda):
+        import torch
+        import torch.nn.functional as F
+
+        B, h, cin = 2, 16, 16
+        g = torch.Generator().manual_seed(6)
+        x = _bf(torch.randn(B, h, h, cin, generator=g))
+        w = torch.zeros(16, 9, cin)
+        w[0] = _bf(torch.randn(9, cin, generator=g) / 12)
+        bias = torch.zeros(16)
+        bias[0] = 0.25
+        resid = torch.randn(B, h, h, generator=g)
+        y32 = torch.empty(B, h, h, device=cuda)
+        _conv_call(x.to(cuda, torch.bfloat16), cin, w.to(cuda, torch.bfloat16), bias.to(cuda), None,
+                   0, B, h, h, cin, 16, 1, y32=y32, resid=resid.to(cuda))
+        want = resid + F.conv2d(x.permute(0, 3, 1, 2), _nchw_weight(w, cin, 16), bias,
+                                padding=1)[:, 0]
+        assert torch.allclose(y32.cpu(), want, rtol=1e-5, atol=1e-5)
+
+    @pytest.mark.parametrize("cin,cout", [(16, 16), (48, 16), (96, 32), (192, 64), (64, 128)])
+    def test_input_gradient(self, cuda, cin, cout):
+        """Transposed weights + mode 2 (mask by the forward input) and mode 3
+        give conv2d's input gradient."""
+        import torch
+        from adrt_amd import _lib
+
+        B, h = 2, 16
+        g = torch.Generator().manual_seed(cin + 7 * cout)
+        x = _bf(torch.randn(B, h, h, cin, generator=g))
+        w = _bf(torch.randn(cout, 9, cin, generator=g) / np.sqrt(9 * cin))
+        dz = _bf(torch.randn(B, h, h, cout, generator=g))
+        xt = x.permute(0, 3, 1, 2).clone().requires_grad_(True)
+        y = torch.nn.functional.conv2d(xt, _nchw_weight(w, cin, cout), padding=1)
+        (gx,) = torch.autograd.grad(y, xt, dz.permute(0, 3, 1, 2))
+        gx = gx.permute(0, 2, 3, 1)
+        wd_ = w.to(cuda, torch.bfloat16).contiguous()
+        wt = torch.empty_like(wd_)
+        LN._check(_lib.load().adrt_hip_nn_transpose_weights(LN._p(wd_), LN._p(wt), cin, cout,
+                                                            LN._stream()), "transpose")
+        dzd = dz.to(cuda, torch.bfloat16)
+        out = torch.empty(B, h, h, cin, device=cuda, dtype=torch.bfloat16)
+        _conv_call(dzd, cout, wt, None, out, cin, B, h, h, cout, cin, 3)
+        assert _bf16_close(out.cpu(), _bf(gx))
+        xd = x.to(cuda, torch.bfloat16)
+        _conv_call(dzd, cout, wt, None, out, cin, B, h, h, cout, cin, 2, mask=xd, ms=cin)
+        assert _bf16_close(out.cpu(), _bf(gx * (x > 0)))
+
+    @pytest.mark.parametrize("cin,cout", [(8, 16), (16, 16), (48, 16), (32, 64), (192, 64),
+                                          (128, 128)])
+    @pytest.mark.parametrize("hw", [16, 8])
+    def test_weight_gradient(self, cuda, cin, cout, hw):
+        import torch
+        from adrt_amd import _lib
+
+        B = 3
+        g = torch.Generator().manual_seed(cin * 31 + cout + hw)
+        x = _bf(torch.randn(B, hw, hw, cin, generator=g))
+        dz = _bf(torch.randn(B, hw, hw, cout, generator=g))
+        wt = torch.zeros(cout, cin, 3, 3, requires_grad=True)
+        bt = torch.zeros(cout, requires_grad=True)
+        y = torch.nn.functional.conv2d(x.permute(0, 3, 1, 2), wt, bt, padding=1)
+        gw, gb = torch.autograd.grad(y, (wt, bt), dz.permute(0, 3, 1, 2))
+        want_w = gw.permute(0, 2, 3, 1).reshape(cout, 9, cin)  # [co][ky*3+kx][ci]
+        lib = _lib.load()
+        nbytes = lib.adrt_hip_nn_wgrad_workspace(B, hw, hw, cin, cout)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=cuda)
+        dw = torch.full((cout, 9, cin), float("nan"), device=cuda)
+        db = torch.full((cout,), float("nan"), device=cuda)
+        # device tensors stay bound to names until the kernel has run: a
+        # temporary freed after its pointer is taken can be reused by the
+        # next allocation before the launch
+        xd = x.to(cuda, torch.bfloat16)
+        dzd = dz.to(cuda, torch.bfloat16)
+        LN._check(lib.adrt_hip_nn_wgrad(LN._p(xd), cin, LN._p(dzd), cout, LN._p(dw), LN._p(db),
+                                        LN._p(ws), nbytes, B, hw, hw, cin, cout, LN._stream()),
+                  "wgrad")
+        torch.cuda.synchronize()
+        scale = float(want_w.abs().max())
+        assert torch.allclose(dw.cpu(), want_w, rtol=1e-4, atol=1e-5 * scale)
+        assert torch.allclose(db.cpu(), gb, rtol=1e-4, atol=1e-4)
+
+    def test_pool_upsample_and_gradients(self, cuda):
+        import torch
+        import torch.nn.functional as F
+        from adrt_amd import _lib
+
+        lib = _lib.load()
+        B, h, c = 2, 16, 32
+        g = torch.Generator().manual_seed(11)
+        x = _bf(torch.randn(B, h, h, c, generator=g))
+        xd = x.to(cuda, torch.bfloat16)
+        y = torch.empty(B, h // 2, h // 2, c, device=cuda, dtype=torch.bfloat16)
+        LN._check(lib.adrt_hip_nn_pool(LN._p(xd), c, LN._p(y), c, B, h, h, c, LN._stream()), "p")
+        want = _bf(F.avg_pool2d(x.permute(0, 3, 1, 2), 2).permute(0, 2, 3, 1))
+        assert _bf16_close(y.cpu(), want)
+        up = torch.empty(B, h, h, c, device=cuda, dtype=torch.bfloat16)
+        LN._check(lib.adrt_hip_nn_upsample(LN._p(y), c, LN._p(up), c, B, h // 2, h // 2, c,
+                                           LN._stream()), "u")
+        assert torch.equal(up.cpu().float(),
+                           y.cpu().float().repeat_interleave(2, 1).repeat_interleave(2, 2))
+        # upsample backward: 2x2 sums
+        dy = _bf(torch.randn(B, h, h, c, generator=g))
+        dyd = dy.to(cuda, torch.bfloat16)
+        dx = torch.empty(B, h // 2, h // 2, c, device=cuda, dtype=torch.bfloat16)
+        LN._check(lib.adrt_hip_nn_upsample_backward(LN._p(dyd), c, LN._p(dx), c, B, h // 2, h // 2,
+                                                    c, LN._stream()), "ub")
+        assert _bf16_close(dx.cpu(), _bf(4 * F.avg_pool2d(dy.permute(0, 3, 1, 2), 2)
+                                         .permute(0, 2, 3, 1)))
+        # pool backward with skip add and mask
+        dp = _bf(torch.randn(B, h // 2, h // 2, c, generator=g))
+        dk = _bf(torch.randn(B, h, h, c, generator=g))
+        ds = torch.empty(B, h, h, c, device=cuda, dtype=torch.bfloat16)
+        dpd, dkd = dp.to(cuda, torch.bfloat16), dk.to(cuda, torch.bfloat16)
+        LN._check(lib.adrt_hip_nn_pool_backward(LN._p(dpd), c, LN._p(dkd), c, LN._p(xd), c,
+                                                LN._p(ds), c, B, h, h, c, LN._stream()), "pb")
+        want = (0.25 * dp.repeat_interleave(2, 1).repeat_interleave(2, 2) + dk) * (x > 0)
+        assert _bf16_close(ds.cpu(), _bf(want))
+
+    def test_adam(self, cuda):
+        import torch
+        from adrt_amd import _lib
+
+        rng = np.random.default_rng(3)
+        w, gr, m, v = (rng.standard_normal(1000).astype(np.float32) for _ in range(4))
+        v = np.abs(v)
+        t = 5
+        lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+        tw, tg, tm, tv = (torch.from_numpy(a.copy()).to(cuda) for a in (w, gr, m, v))
+        w16 = torch.empty(1000, dtype=torch.bfloat16, device=cuda)
+        LN._check(_lib.load().adrt_hip_nn_adam(LN._p(tw), LN._p(tg), LN._p(tm), LN._p(tv),
+                                               LN._p(w16), 1000, lr, b1, b2, eps, 1 - b1 ** t,
+                                               1 - b2 ** t, LN._stream()), "adam")
+        m2 = b1 * m + (1 - b1) * gr
+        v2 = b2 * v + (1 - b2) * gr * gr
+        w2 = w - lr * (m2 / (1 - b1 ** t)) / (np.sqrt(v2 / (1 - b2 ** t)) + eps)
+        assert np.allclose(tw.cpu().numpy(), w2, rtol=1e-5, atol=1e-7)
+        # (1 - beta) is formed in float32 on the device, in float64 here
+        assert np.allclose(tm.cpu().numpy(), m2, rtol=1e-5, atol=1e-7)
+        assert np.allclose(tv.cpu().numpy(), v2, rtol=1e-5, atol=1e-7)
+        assert torch.equal(w16.cpu(), torch.from_numpy(w2).to(torch.bfloat16)) or \
+            _bf16_close(w16.cpu(), torch.from_numpy(w2))
+
+
+# ---------------------------------------------------------------------------
+# the whole U-Net against a CPU autograd model
+# ---------------------------------------------------------------------------
+def cpu_unet(flat, z):
+    """Float32 CPU model of the network with the same parameters and the same
+    bf16 rounding points as the device activations."""
+    import torch
+    import torch.nn.functional as F
+
+    params = {}
+    off = 0
+    for name, cin, cout in LN.LAYERS:
+        nw = cout * 9 * cin
+        w = _bf(flat[off:off + nw]).reshape(cout, 9, cin)
+        params[name] = (_nchw_weight(w, cin, cout), flat[off + nw:off + nw + cout])
+        off += nw + cout
+
+    def conv(x, name, relu=True):
+        w, b = params[name]
+        y = F.conv2d(x, w, b, padding=1)
+        return _bf(F.relu(y)) if relu else y
+
+    B, n, _ = z.shape
+    x8 = torch.zeros(B, 8, n, n)
+    x8[:, 0] = _bf(z)
+    up = lambda t: t.repeat_interleave(2, 2).repeat_interleave(2, 3)  # noqa: E731
+    s0 = conv(conv(x8, "e0a"), "e0b")
+    s1 = conv(conv(_bf(F.avg_pool2d(s0, 2)), "e1a"), "e1b")
+    s2 = conv(conv(_bf(F.avg_pool2d(s1, 2)), "e2a"), "e2b")
+    b3 = conv(conv(_bf(F.avg_pool2d(s2, 2)), "b3a"), "b3b")
+    d2 = conv(conv(torch.cat([s2, up(b3)], 1), "d2a"), "d2b")
+    d1 = conv(conv(torch.cat([s1, up(d2)], 1), "d1a"), "d1b")
+    d0 = conv(conv(torch.cat([s0, up(d1)], 1), "d0a"), "d0b")
+    return z + conv(d0, "out", relu=False)[:, 0]
+
+
+@gpu
+class TestUNet:
+    def test_forward_and_gradients_match_cpu(self, cuda):
+        import torch
+
+        net = LN.UNet(seed=1, device=cuda)
+        # a non-trivial output layer so every gradient path carries signal
+        rng = np.random.default_rng(2)
+        w = net.w.cpu().numpy()
+        o0, b0, cin, cout = net.offsets["out"]
+        wo = np.zeros((cout, 9, cin), np.float32)
+        wo[0] = rng.standard_normal((9, cin)) * 0.2
+        w[o0:b0] = wo.reshape(-1)
+        for name, _, c in LN.LAYERS:
+            _, bb, _, cc = net.offsets[name]
+            w[bb:bb + cc] = rng.standard_normal(cc) * 0.05
+        w[b0 + 1:b0 + cout] = 0
+        net.load_state(w)
+        B, n = 2, 64
+        z = torch.from_numpy(rng.standard_normal((B, n, n)).astype(np.float32))
+        target = torch.from_numpy(rng.standard_normal((B, n, n)).astype(np.float32))
+        y = net.forward(z.to(cuda)).cpu()
+        flat = torch.from_numpy(w).requires_grad_(True)
+        y_ref = cpu_unet(flat, z)
+        rel = float((y - y_ref.detach()).norm() / (y_ref.detach() - z).norm())
+        assert rel < 3e-2, rel
+        scale = 1.0 / (B * n * n)
+        net.backward((scale, target.to(cuda)), B, n)
+        loss = 0.5 * scale * ((y_ref - target) ** 2).sum()
+        (g_ref,) = torch.autograd.grad(loss, flat)
+        g = net.grad.cpu()
+        for name, cin, cout in LN.LAYERS:
+            w0, b0, _, _ = net.offsets[name]
+            for a, b, what in ((w0, b0, "w"), (b0, b0 + cout, "b")):
+                gr, gg = g_ref[a:b], g[a:b]
+                if float(gr.norm()) == 0:
+                    assert float(gg.norm()) == 0
+                    continue
+                err = float((gg - gr).norm() / gr.norm())
+                cos = float((gg * gr).sum() / (gg.norm() * gr.norm()))
+                assert err < 6e-2 and cos > 0.998, (name, what, err, cos)
+
+    def test_training_reduces_loss(self, cuda):
+        solver = LN.LearnedSolver(n=64, device=cuda)
+        solver.fit_preconditioner(count=8)
+        hist = solver.train(60, batch=8, pairs=480, lr=2e-3, log_every=1)
+        first = np.mean([h for _, h in hist[:5]])
+        last = np.mean([h for _, h in hist[-5:]])
+        assert last < 0.8 * first, (first, last)
+
+
+# ---------------------------------------------------------------------------
+# preconditioner, phantoms, solver
+# ---------------------------------------------------------------------------
+@gpu
+class TestPreconditioner:
+    def test_apply_matches_numpy(self, cuda):
+        import torch
+
+        rng = np.random.default_rng(4)
+        pyr = LN.Pyramid(levels=6, theta=rng.standard_normal((6, 6)), device=cuda)
+        r = rng.standard_normal((3, 64, 64)).astype(np.float32)
+        got = pyr(torch.from_numpy(r).to(cuda)).cpu().numpy()
+        want = np_pyramid(r, pyr.coef.cpu().numpy().astype(np.float64))
+        assert np.allclose(got, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
+
+    def test_symmetric(self, cuda):
+        import torch
+
+        rng = np.random.default_rng(5)
+        pyr = LN.Pyramid(levels=7, theta=rng.standard_normal((7, 6)), device=cuda)
+        x, y = (torch.from_numpy(rng.standard_normal((1, 128, 128)).astype(np.float32)).to(cuda)
+                for _ in range(2))
+        a, b = float(LN.bdot(pyr(x), y)), float(LN.bdot(x, pyr(y)))
+        assert abs(a - b) <= 1e-4 * max(abs(a), 1.0)
+
+    def test_fit_is_least_squares_optimum(self, cuda):
+        """The fitted coefficients minimise sum ||P(M v) - v||^2: no random
+        perturbation (or the unpreconditioned scaling) does better."""
+        import torch
+
+        n = 32
+        solver = LN.LearnedSolver(n=n, device=cuda)
+        x = LN.phantom_batch(n, 3, 0, 4, cuda)
+        g = torch.Generator(device=cuda)
+        g.manual_seed(3)
+        vecs = torch.cat([x, torch.randn(x.shape, generator=g, device=cuda)])
+        theta = solver.pyramid.fit(vecs)
+        w = LN.normal_op(vecs)
+
+        def obj(th):
+            solver.pyramid.set_theta(th)
+            return float(((solver.pyramid(w) - vecs).double() ** 2).sum())
+
+        best = obj(theta)
+        rng = np.random.default_rng(0)
+        for _ in range(4):
+            pert = theta + 1e-1 * np.abs(theta).max() * rng.standard_normal(theta.shape)
+            assert obj(pert) >= best * (1 - 1e-5)
+        ident = np.zeros_like(theta)
+        ident[0, 0] = float((vecs * w).sum() / (w * w).sum())
+        assert obj(ident) > best
+
+    def test_phantom_raster_matches_phantoms(self, cuda):
+        from adrt_amd.phantom import phantoms
+
+        got = LN.phantom_batch(128, 7, 3, 4, cuda).cpu().numpy()
+        want = phantoms(4, 128, seed=7, start=3, device=cuda).cpu().numpy()
+        # cos/sin come from NumPy here and from the device in phantoms():
+        # an ulp can move a pixel centre across an ellipse boundary
+        assert np.mean(got != want) < 1e-3
+
+
+@gpu
+class TestSolver:
+    def _problem(self, cuda, n=64, count=3):
+        solver = LN.LearnedSolver(n=n, device=cuda)
+        x, b = solver.data(seed=77, start=0, count=count, noise_seed=5)
+        return solver, x, b
+
+    def test_plain_pcg_matches_cg_solve(self, cuda):
+        from adrt_amd import cg
+
+        _, _, b = self._problem(cuda)
+        x1, it1, conv1, _ = LN.pcg(LN.normal_op, b, rtol=1e-5)
+        x2, info, _ = cg.cg_solve(cg.ADRTNormalOperator(64), b, rtol=1e-5)
+        assert bool(conv1.all()) and int(info.max()) == 0
+        assert float((x1 - x2).norm() / x2.norm()) < 1e-4
+
+    def test_preconditioned_converges_faster_to_same_solution(self, cuda):
+        solver, _, b = self._problem(cuda)
+        solver.fit_preconditioner(count=8)
+        xp, itp, cp, _ = LN.pcg(LN.normal_op, b, rtol=1e-6)
+        xq, itq, cq, _ = LN.pcg(LN.normal_op, b, prec=solver.pyramid, rtol=1e-6)
+        assert bool(cp.all()) and bool(cq.all())
+        assert int(itq.max()) < int(itp.min())
+        assert float((xq - xp).norm() / xp.norm()) < 1e-3
+
+    def test_indefinite_preconditioner_falls_back(self, cuda):
+        _, _, b = self._problem(cuda)
+        xp, itp, cp, _ = LN.pcg(LN.normal_op, b, rtol=1e-5)
+        xf, itf, cf, _ = LN.pcg(LN.normal_op, b, prec=lambda r: -r, rtol=1e-5)
+        assert bool(cf.all())
+        assert float((xf - xp).norm() / xp.norm()) < 1e-3
+
+    def test_network_cuts_iterations(self, cuda):
+        """Trained briefly at 64 x 64, the U-Net's initial guess needs fewer
+        preconditioned CG iterations on held-out images than the same
+        preconditioned CG started from zero (the method without it)."""
+        solver = LN.LearnedSolver(n=64, device=cuda)
+        solver.fit_preconditioner(count=16)
+        solver.train(300, batch=16, pairs=4800, lr=2e-3)
+        x, b = solver.data(seed=999, start=0, count=16, noise_seed=999)
+        x0 = solver.initial_guess(b).clone()
+        _, it_net, c1, _ = LN.pcg(LN.normal_op, b, x0, prec=solver.pyramid, rtol=1e-4)
+        _, it_zero, c2, _ = LN.pcg(LN.normal_op, b, prec=solver.pyramid, rtol=1e-4)
+        assert bool(c1.all()) and bool(c2.all())
+        assert float(it_net.float().mean()) < float(it_zero.float().mean())
