@@ -311,9 +311,9 @@ class UNet:
         self.refresh_transposed()
 
     # -- buffers -------------------------------------------------------------
-    def buffers(self, B, n):
+    def buffers(self, B, n, train=True):
         key = (B, n)
-        if key in self._buffers:
+        if key in self._buffers and (not train or "g_out" in self._buffers[key]):
             return self._buffers[key]
         torch = _torch()
         bf = torch.bfloat16
@@ -331,13 +331,18 @@ class UNet:
             "t2d": t(n2, 64), "d2": t(n2, 64), "t1d": t(n1, 32), "d1": t(n1, 32),
             "t0d": t(n, 16), "d0": t(n, 16),
             "y": torch.empty((B, n, n), dtype=torch.float32, device=d),
+        }
+        if not train:
+            self._buffers[key] = buf
+            return buf
+        buf.update({
             # gradients
             "g_out": t(n, 16), "g16a": t(n, 16), "g16b": t(n, 16), "g_cat0": t(n, 48),
             "g32a": t(n1, 32), "g32b": t(n1, 32), "g_cat1": t(n1, 96),
             "g64a": t(n2, 64), "g64b": t(n2, 64), "g_cat2": t(n2, 192),
             "g128a": t(n3, 128), "g128b": t(n3, 128), "g_p2": t(n3, 64),
             "g_p1": t(n2, 32), "g_p0": t(n1, 16),
-        }
+        })
         lib = _lib.load()
         ws = 0
         for name, cin, cout in LAYERS:
@@ -367,13 +372,14 @@ class UNet:
                                              ws.numel(), B, h, h, cin, cout, _stream()),
                f"wgrad {name}")
 
-    def forward(self, z):
+    def forward(self, z, train=True):
         """x0 = z + net(z) for (B, n, n) float32 device images z; keeps the
-        activations for backward()."""
+        activations for backward() (``train=False`` allocates no gradient
+        buffers)."""
         lib = _lib.load()
         B, n, _ = z.shape
         z = z.contiguous()
-        f = self.buffers(B, n)
+        f = self.buffers(B, n, train)
         n1, n2, n3 = n // 2, n // 4, n // 8
         s = _stream()
         cat0, cat1, cat2 = f["cat0"], f["cat1"], f["cat2"]
@@ -522,11 +528,24 @@ class LearnedSolver:
         noise = torch.randn(x.shape, generator=g, device=self.device)
         return self.pyramid.fit(torch.cat([x, noise]))
 
-    def initial_guess(self, b):
+    def initial_guess(self, b, train=True):
         z = self.pyramid(b)
-        return self.net.forward(z)
+        return self.net.forward(z, train)
 
-    def train(self, steps, batch=64, seed=1000, pairs=50000, lr=1e-3, log_every=0, log=None):
+    def save(self, path, **extra):
+        np.savez(path, unet=self.net.w.cpu().numpy(), theta=self.pyramid.theta, n=self.n,
+                 sigma_frac=self.sigma_frac, **extra)
+
+    @classmethod
+    def load(cls, path, device=None):
+        with np.load(path, allow_pickle=False) as f:
+            solver = cls(n=int(f["n"]), sigma_frac=float(f["sigma_frac"]), device=device)
+            solver.pyramid.set_theta(f["theta"])
+            solver.net.load_state(f["unet"])
+        return solver
+
+    def train(self, steps, batch=64, seed=1000, pairs=50000, lr=1e-3, log_every=0, log=None,
+              lr_fn=None, start=0):
         """Adam on 0.5 * mean ||x0 - x||^2 over seeded phantom pairs (pair i =
         phantom i of ``seed`` with noise seed (seed, i)); batches cycle through
         ``pairs`` pairs.  Returns the loss history (one value per logged step)."""
@@ -534,11 +553,11 @@ class LearnedSolver:
         n = self.n
         nb = max(pairs // batch, 1)
         for step in range(steps):
-            k = step % nb
+            k = (start + step) % nb
             x, b = self.data(seed, k * batch, batch, noise_seed=seed * 100003 + k)
             self.initial_guess(b)
             self.net.backward((1.0 / (batch * n * n), x), batch, n)
-            self.net.adam_step(lr=lr)
+            self.net.adam_step(lr=lr if lr_fn is None else lr_fn(step))
             if log_every and (step % log_every == 0 or step == steps - 1):
                 hist.append((step, self.net.loss(x)))
                 if log:
@@ -606,3 +625,54 @@ def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500):
         rz = rz_new
     converged = ~active
     return x, iters, converged, apps
+
+
+def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
+    """Wall time to relative residual ``rtol`` on ``count`` held-out phantoms
+    (phantom seed ``seed``, noise seed ``seed``) for plain CG, CG with the
+    multiscale preconditioner from zero, and the learned method (U-Net initial
+    guess + preconditioned CG).  Each solve is timed from the right-hand side
+    to the converged batch, device synchronised on both sides."""
+    import time
+
+    torch = _torch()
+    x, b = solver.data(seed, 0, count, noise_seed=seed)
+    bn = bdot(b, b).sqrt()
+
+    def run(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    def stats(name, res, secs):
+        xs, it, conv, apps = res
+        resid = (bdot(b - normal_op(xs), b - normal_op(xs)).sqrt() / bn)
+        return {"seconds": round(secs, 4), "iterations_mean": round(float(it.float().mean()), 2),
+                "iterations_max": int(it.max()), "operator_applications": apps,
+                "converged": int(conv.sum()), "max_rel_residual": float(resid.max()),
+                "rel_error_vs_phantom": float((bdot(xs - x, xs - x) / bdot(x, x)).sqrt().mean())}
+
+    plain, t_plain = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter))
+    # warm the network buffers outside the timed region (allocation only)
+    solver.net.buffers(count, solver.n, train=False)
+    learned, t_learned = run(lambda: pcg(normal_op, b, solver.initial_guess(b, train=False),
+                                         prec=solver.pyramid, rtol=rtol, maxiter=maxiter))
+    prec_only, t_prec = run(lambda: pcg(normal_op, b, prec=solver.pyramid, rtol=rtol,
+                                        maxiter=maxiter))
+    x0, t_net = run(lambda: solver.initial_guess(b, train=False))
+    out = {
+        "count": count, "n": solver.n, "rtol": rtol,
+        "plain_cg": stats("plain", plain, t_plain),
+        "preconditioner_only": stats("prec", prec_only, t_prec),
+        "learned": stats("learned", learned, t_learned),
+        "network_seconds": round(t_net, 4),
+        "initial_guess_rel_error": float((bdot(x0 - x, x0 - x) / bdot(x, x)).sqrt().mean()),
+        "speedup_learned_vs_plain": round(t_plain / t_learned, 3),
+        "speedup_preconditioner_only_vs_plain": round(t_plain / t_prec, 3),
+        "learned_vs_plain_solution_rel_diff": float(
+            (bdot(learned[0] - plain[0], learned[0] - plain[0]) / bdot(plain[0], plain[0]))
+            .sqrt().max()),
+    }
+    return out

@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--extras", action="store_true", help="also time bdrt / iadrt / fp64")
+    p.add_argument("--no-learned", action="store_true",
+                   help="skip the learned-vs-plain CG time-to-1e-4 leg")
+    p.add_argument("--cg-count", type=int, default=1000)
     p.add_argument("--profile", action="store_true",
                    help="short run for rocprofv3 (1 warm-up + 3 steps, no CPU leg), followed "
                         "by a 256 MiB device copy that calibrates the memory counters")
@@ -93,6 +96,31 @@ def time_calls(fn, steps, warmup, stream):
 
 
 UNEVEN_SIDES = (32, 64, 128, 256, 512, 1024)
+LEARNED_WEIGHTS = os.path.join(ROOT, "adrt_amd", "weights", "learned_256.npz")
+
+
+def learned_cg_leg(count):
+    """Second half of the metric: wall time to relative residual 1e-4 of CG
+    on the normal equations for ``count`` held-out 256x256 float32 phantoms
+    (noise 0.5% of max|adrt(x)|), learned (U-Net initial guess + learned
+    multiscale preconditioner) vs plain CG, on the same GPU in the same run.
+    Uses the committed weights trained on the GPU by
+    ``python -m adrt_amd.train_learned``; trains for 60 s if they are absent."""
+    from adrt_amd import learned
+
+    if os.path.exists(LEARNED_WEIGHTS):
+        solver = learned.LearnedSolver.load(LEARNED_WEIGHTS)
+        source = "adrt_amd/weights/learned_256.npz"
+    else:
+        solver = learned.LearnedSolver(n=256)
+        solver.fit_preconditioner()
+        solver.train(2000, batch=64)
+        source = "trained in this run"
+    out = learned.evaluate(solver, count=count)
+    out["weights"] = source
+    out["workload"] = (f"{count} held-out 256x256 float32 ellipse phantoms (seed 2000), "
+                       "d = adrt(x) + 0.5% noise, CG on mean_q truncate(bdrt(adrt(.)))")
+    return out
 
 
 def uneven_extra(steps, warmup):
@@ -255,6 +283,7 @@ def main():
         return
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.extras = 3, 1, True, False
+        args.no_learned = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: "
               "this benchmark runs on a single MI355X (a batch of images would shard "
@@ -310,6 +339,7 @@ def main():
         return
 
     wall, dev_ms = time_calls(fwd, args.steps, args.warmup, stream)
+    first_out = out[:1].cpu().numpy()
     value = batch * args.steps / wall
     algo = batch * bytes_fwd(n, s)
     achieved = algo / (dev_ms * 1e-3) / 1e9
@@ -370,6 +400,10 @@ def main():
                             "GBps_algorithmic": round(gbs, 1)}
         extras["uneven"] = uneven_extra(max(3, args.steps // 4), 2)
         result["extras"] = extras
+    if not args.no_learned:
+        del out, ws
+        torch.cuda.empty_cache()
+        result["cg_time_to_1e-4"] = learned_cg_leg(args.cg_count)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(host, args.cpu_seconds)
         # bit-exact spot check of the first image against the CPU leg's library
@@ -377,7 +411,7 @@ def main():
         from oracle import oracle as O
 
         ref_fn = R.adrt if R.available() else O.adrt
-        got = out[:1].cpu().numpy()
+        got = first_out
         want = ref_fn(np.ascontiguousarray(host[:1]))
         got = np.where(got == 0, 0, got)
         want = np.where(want == 0, 0, want)
