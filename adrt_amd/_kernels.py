@@ -141,3 +141,52 @@ def truncate_combine(t, divisor=1.0, mode=MEAN):
                                              float(divisor), int(mode), dev.stream_ptr(t)),
                "truncate_combine")
     return out
+
+
+# ---------------------------------------------------------------------------
+# batched CG vector operations: (B, ...) tensors, one system per leading index
+# ---------------------------------------------------------------------------
+def _bm(t):
+    b = int(t.shape[0])
+    return b, int(t.numel() // max(b, 1))
+
+
+def cg_workspace(t):
+    b, m = _bm(t)
+    return dev.workspace(t, _lib.load().adrt_hip_cg_workspace(b, m))
+
+
+def batched_dot(u, v, ws=None, out=None):
+    """Per-system float64 inner products of two (B, ...) tensors."""
+    lib = _lib.load()
+    b, m = _bm(u)
+    ws = cg_workspace(u) if ws is None else ws
+    if out is None:
+        out = dev.torch().empty(b, dtype=dev.torch().float64, device=u.device)
+    _lib.check(lib.adrt_hip_batched_dot(dev.ptr(u), dev.ptr(v), dev.ptr(out), dev.ptr(ws),
+                                        ws.numel(), _code(u), b, m, dev.stream_ptr(u)),
+               "batched_dot")
+    return out
+
+
+def cg_update(x, r, p, q, alpha, ws=None, rr=None):
+    """In place: x += a p, r -= a q (a = alpha[b] in the element type);
+    returns the float64 ||r||^2 per system."""
+    lib = _lib.load()
+    b, m = _bm(x)
+    ws = cg_workspace(x) if ws is None else ws
+    if rr is None:
+        rr = dev.torch().empty(b, dtype=dev.torch().float64, device=x.device)
+    _lib.check(lib.adrt_hip_cg_update(dev.ptr(x), dev.ptr(r), dev.ptr(p), dev.ptr(q),
+                                      dev.ptr(alpha), dev.ptr(rr), dev.ptr(ws), ws.numel(),
+                                      _code(x), b, m, dev.stream_ptr(x)), "cg_update")
+    return rr
+
+
+def cg_direction(p, z, beta):
+    """In place: p = z + c p (c = beta[b] in the element type)."""
+    lib = _lib.load()
+    b, m = _bm(p)
+    _lib.check(lib.adrt_hip_cg_direction(dev.ptr(p), dev.ptr(z), dev.ptr(beta), _code(p), b, m,
+                                         dev.stream_ptr(p)), "cg_direction")
+    return p

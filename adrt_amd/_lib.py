@@ -63,6 +63,10 @@ SIGNATURES = {
     "adrt_hip_adrt_init": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_truncate": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_truncate_combine": (_INT, [_VP, _VP, _INT, _SZ, _SZ, ctypes.c_double, _INT, _VP]),
+    "adrt_hip_cg_workspace": (_SZ, [_SZ, _SZ]),
+    "adrt_hip_batched_dot": (_INT, [_VP, _VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_cg_update": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_cg_direction": (_INT, [_VP, _VP, _VP, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_set_config": (_INT, [_INT, _SZ, _INT, _SZ]),
     "adrt_hip_get_config": (_INT, [_INT, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_SZ)]),
     "adrt_hip_nn_conv3x3": (_INT, [_VP, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _INT, _INT,

@@ -21,10 +21,12 @@ Two learned parts, both applied on the GPU by this package's own kernels
   hand-written kernels; the gradients are checked against a float32 CPU
   autograd model in the tests.
 
-``solve`` runs preconditioned CG from x0 with the float64 scalars of
-``adrt_amd.cg``; an image whose preconditioned step is not a descent step
-(P not positive definite there) or that has not converged by ``maxiter``
-continues with plain CG from its current iterate (the fallback).
+``pcg`` runs preconditioned CG from x0 with float64 scalars per image (fused
+device kernels for the dot products and updates, one iteration captured as a
+hipGraph); an image whose preconditioned step is not a descent step (P not
+positive definite there) continues as plain CG from its current iterate (the
+fallback), and ``evaluate`` reports every image's convergence and its error
+against plain CG's solution and the phantom.
 """
 
 import ctypes
@@ -68,8 +70,8 @@ def normal_rhs(d):
 
 
 def bdot(u, v):
-    """Per-image float64 inner products of (B, n, n) tensors."""
-    return (u.double() * v.double()).sum(dim=(-2, -1))
+    """Per-image float64 inner products of (B, n, n) tensors (device kernel)."""
+    return _kernels.batched_dot(u.contiguous(), v.contiguous())
 
 
 # ---------------------------------------------------------------------------
@@ -565,64 +567,78 @@ class LearnedSolver:
         return hist
 
 
-def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500):
+def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=True):
     """Batched preconditioned CG (plain CG when prec is None) for (B, n, n)
     systems with float64 scalars per image.  Stops each image when
-    ||b - M x|| <= rtol * ||b||.  An image whose preconditioned step fails
-    (r.z <= 0 or p.Mp <= 0) restarts as plain CG from its iterate.  Returns
-    (x, iterations per image, converged mask, operator applications)."""
+    ||b - M x|| <= rtol * ||b|| (the recursively updated residual); stopped
+    images are frozen while the others continue.  An image whose
+    preconditioned step fails (r.z <= 0 or p.Mp <= 0) continues as plain CG
+    from its iterate (steepest-descent restart).  With ``graph`` one
+    iteration is captured once as a hipGraph and replayed; the host only
+    checks for convergence between replays.  Returns (x, iterations per
+    image, converged mask, operator applications)."""
     torch = _torch()
     B = b.shape[0]
-    x = torch.zeros_like(b) if x0 is None else x0.clone()
-    r = b - op(x) if x0 is not None else b.clone()
-    bn = bdot(b, b).sqrt()
-    tol = rtol * bn
+    b = b.contiguous()
+    ws = _kernels.cg_workspace(b)
+    x = torch.zeros_like(b) if x0 is None else x0.contiguous().clone()
+    r = (b - op(x)).contiguous() if x0 is not None else b.clone()
+    tol = rtol * bdot(b, b).sqrt()
     use_p = torch.full((B,), prec is not None, dtype=torch.bool, device=b.device)
-
-    def precond(rr):
-        if prec is None:
-            return rr
-        z = prec(rr)
-        return torch.where(use_p[:, None, None], z, rr)
-
-    z = precond(r)
-    p = z.clone()
-    rz = bdot(r, z)
-    bad = rz <= 0
-    if bool(bad.any()):
-        use_p &= ~bad
-        z = precond(r)
-        p = z.clone()
+    zero = torch.zeros(B, dtype=torch.float64, device=b.device)
+    one = torch.ones(B, dtype=torch.float64, device=b.device)
+    rr = bdot(r, r)
+    if prec is None:
+        z, rz = r, rr.clone()
+    else:
+        z = prec(r)
         rz = bdot(r, z)
+        bad = rz <= 0
+        use_p &= ~bad
+        z = torch.where(bad[:, None, None], r, z)
+        rz = torch.where(bad, rr, rz)
+    p = z.clone()
+    active = rr.sqrt() > tol
     iters = torch.zeros(B, dtype=torch.int64, device=b.device)
-    active = bdot(r, r).sqrt() > tol
+
+    def step():
+        q = op(p)
+        pq = _kernels.batched_dot(p, q, ws)
+        ok = active & (pq > 0)
+        alpha = torch.where(ok, rz / torch.where(pq > 0, pq, one), zero)
+        rr_new = _kernels.cg_update(x, r, p, q, alpha, ws)
+        iters.add_(active.long())
+        new_active = active & (rr_new.sqrt() > tol)
+        if prec is None:
+            zz, rz_new = r, rr_new
+            beta = rz_new / torch.where(rz != 0, rz, one)
+        else:
+            zz = torch.where(use_p[:, None, None], prec(r), r)
+            rz_new = _kernels.batched_dot(r, zz, ws)
+            bad = new_active & use_p & ((rz_new <= 0) | (pq <= 0))
+            use_p.logical_and_(~bad)
+            zz = torch.where(bad[:, None, None], r, zz)
+            rz_new = torch.where(bad, rr_new, rz_new)
+            beta = torch.where(bad, zero, rz_new / torch.where(rz != 0, rz, one))
+        _kernels.cg_direction(p, zz, beta)
+        rz.copy_(rz_new)
+        active.copy_(new_active)
+
+    g = None
+    if graph:
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            step()
     apps = 0
     for _ in range(maxiter):
         if not bool(active.any()):
             break
-        q = op(p)
-        apps += 1
-        pq = bdot(p, q)
-        ok = active & (pq > 0)
-        alpha = torch.where(ok, rz / torch.where(pq > 0, pq, torch.ones_like(pq)),
-                            torch.zeros_like(pq))
-        a = alpha.to(b.dtype)[:, None, None]
-        x = x + a * p
-        r = r - a * q
-        iters += active.long()
-        active = active & (bdot(r, r).sqrt() > tol)
-        z = precond(r)
-        rz_new = bdot(r, z)
-        bad = active & ((rz_new <= 0) | (pq <= 0)) & use_p
-        if bool(bad.any()):
-            use_p &= ~bad
-            z = precond(r)
-            rz_new = bdot(r, z)
-            beta = torch.where(bad, torch.zeros_like(rz), rz_new / rz)
+        if g is not None:
+            g.replay()
         else:
-            beta = rz_new / torch.where(rz != 0, rz, torch.ones_like(rz))
-        p = z + beta.to(b.dtype)[:, None, None] * p
-        rz = rz_new
+            step()
+        apps += 1
     converged = ~active
     return x, iters, converged, apps
 
@@ -635,9 +651,14 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
     to the converged batch, device synchronised on both sides."""
     import time
 
+    from . import batched
+
     torch = _torch()
     x, b = solver.data(seed, 0, count, noise_seed=seed)
     bn = bdot(b, b).sqrt()
+    # kernel configuration of the operator's transforms for this batch shape
+    tuning = [batched.tune(op, solver.n, count)["choice"]
+              for op in (_lib.OP_ADRT, _lib.OP_BDRT)]
 
     def run(fn):
         torch.cuda.synchronize()
@@ -663,7 +684,7 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
                                         maxiter=maxiter))
     x0, t_net = run(lambda: solver.initial_guess(b, train=False))
     out = {
-        "count": count, "n": solver.n, "rtol": rtol,
+        "count": count, "n": solver.n, "rtol": rtol, "tuned_adrt_bdrt": tuning,
         "plain_cg": stats("plain", plain, t_plain),
         "preconditioner_only": stats("prec", prec_only, t_prec),
         "learned": stats("learned", learned, t_learned),

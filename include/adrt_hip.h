@@ -162,6 +162,25 @@ int adrt_hip_truncate(const void *in, void *out, size_t elem_bytes, size_t outer
 int adrt_hip_truncate_combine(const void *in, void *out, int dtype, size_t batch, size_t n,
                               double divisor, int mode, void *stream);
 
+/* ---- batched CG vector operations (no reference counterpart) ------------- *
+ * For B independent systems of m unknowns (C-contiguous, B <= 65535), with
+ * float64 scalars per system and deterministic float64 sums:
+ *   adrt_hip_batched_dot:  out[b] = sum_i u[b][i] * v[b][i]
+ *   adrt_hip_cg_update:    x += a_b p, r -= a_b q (a_b = alpha[b] rounded to
+ *                          the element type), rr[b] = sum_i r[b][i]^2 after
+ *   adrt_hip_cg_direction: p = z + c_b p (c_b = beta[b] rounded)
+ * alpha, beta, out and rr are device arrays of B doubles; the workspace is
+ * adrt_hip_cg_workspace(B, m) bytes.  Used by the solvers; capturable in a
+ * graph (no allocation, no memset). */
+size_t adrt_hip_cg_workspace(size_t batch, size_t m);
+int adrt_hip_batched_dot(const void *u, const void *v, double *out, void *workspace,
+                         size_t workspace_bytes, int dtype, size_t batch, size_t m, void *stream);
+int adrt_hip_cg_update(void *x, void *r, const void *p, const void *q, const double *alpha,
+                       double *rr, void *workspace, size_t workspace_bytes, int dtype,
+                       size_t batch, size_t m, void *stream);
+int adrt_hip_cg_direction(void *p, const void *z, const double *beta, int dtype, size_t batch,
+                          size_t m, void *stream);
+
 /* ---- kernel configuration (no reference counterpart) --------------------- */
 
 /* Variants of the forward and back-projection paths for one size:
