@@ -17,7 +17,8 @@ Two learned parts, both applied on the GPU by this package's own kernels
   per level, ReLU, 2x2 average pooling, nearest upsampling, skip
   concatenation), BF16 weights and activations with FP32 accumulation on the
   matrix cores.  Input z = P b; output x0 = z + net(z), trained with Adam on
-  ||x0 - x||^2 against the clean phantom x.  Forward and backward are
+  ||x0 - x*||^2 against the solution x* of the training pair's normal
+  equations (or against the clean phantom).  Forward and backward are
   hand-written kernels; the gradients are checked against a float32 CPU
   autograd model in the tests.
 
@@ -546,47 +547,76 @@ class LearnedSolver:
             solver.net.load_state(f["unet"])
         return solver
 
+    def solution_target(self, b, rtol=1e-6, maxiter=200):
+        """The normal equations' solution for right-hand sides b (CG with the
+        preconditioner), the training target of ``target='solution'``."""
+        return pcg(normal_op, b, prec=self.pyramid, rtol=rtol, maxiter=maxiter, graph=False)[0]
+
     def train(self, steps, batch=64, seed=1000, pairs=50000, lr=1e-3, log_every=0, log=None,
-              lr_fn=None, start=0):
-        """Adam on 0.5 * mean ||x0 - x||^2 over seeded phantom pairs (pair i =
-        phantom i of ``seed`` with noise seed (seed, i)); batches cycle through
-        ``pairs`` pairs.  Returns the loss history (one value per logged step)."""
+              lr_fn=None, start=0, target="solution"):
+        """Adam on 0.5 * mean ||x0 - t||^2 over seeded phantom pairs (pair i =
+        phantom i of ``seed`` with its batch's noise seed); batches cycle
+        through ``pairs`` pairs.  The target t is the normal equations'
+        solution for the pair's noisy data (``target='solution'``: the point
+        CG converges to, so the initial guess is judged by what is left for
+        CG to do), the clean phantom (``target='phantom'``), or none: with
+        ``target='residual'`` the loss is sum_i ||M x0_i - b_i||^2 / ||b_i||^2,
+        the quantity CG's stopping test measures (its gradient is
+        2 M (M x0 - b) / ||b||^2, M being symmetric); the logged value is then
+        the mean relative residual.  Returns the loss history (one value per
+        logged step)."""
         hist = []
         n = self.n
         nb = max(pairs // batch, 1)
         for step in range(steps):
             k = (start + step) % nb
             x, b = self.data(seed, k * batch, batch, noise_seed=seed * 100003 + k)
-            self.initial_guess(b)
-            self.net.backward((1.0 / (batch * n * n), x), batch, n)
+            if target == "residual":
+                y = self.initial_guess(b)
+                # d/dx0 of sum_i ||M x0_i - b_i||^2 / (batch ||b_i||^2) = 2 M (M x0 - b) / ...
+                w = 2.0 / (batch * bdot(b, b))
+                g = normal_op((normal_op(y) - b).contiguous()) * w.to(y.dtype)[:, None, None]
+                self.net.backward((1.0, (y - g).contiguous()), batch, n)
+            else:
+                if target == "solution":
+                    x = self.solution_target(b)
+                self.initial_guess(b)
+                self.net.backward((1.0 / (batch * n * n), x), batch, n)
             self.net.adam_step(lr=lr if lr_fn is None else lr_fn(step))
             if log_every and (step % log_every == 0 or step == steps - 1):
-                hist.append((step, self.net.loss(x)))
+                if target == "residual":
+                    y = self.net._buffers[(batch, n)]["y"]
+                    e = normal_op(y) - b
+                    hist.append((step, float((bdot(e, e) / bdot(b, b)).sqrt().mean())))
+                else:
+                    hist.append((step, self.net.loss(x)))
                 if log:
                     log(step, hist[-1][1])
         return hist
 
 
-def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=True):
+def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=True, compact=0.75):
     """Batched preconditioned CG (plain CG when prec is None) for (B, n, n)
     systems with float64 scalars per image.  Stops each image when
-    ||b - M x|| <= rtol * ||b|| (the recursively updated residual); stopped
-    images are frozen while the others continue.  An image whose
-    preconditioned step fails (r.z <= 0 or p.Mp <= 0) continues as plain CG
-    from its iterate (steepest-descent restart).  With ``graph`` one
-    iteration is captured once as a hipGraph and replayed; the host only
-    checks for convergence between replays.  Returns (x, iterations per
-    image, converged mask, operator applications)."""
+    ||b - M x|| <= rtol * ||b|| (the recursively updated residual).  An image
+    whose preconditioned step fails (r.z <= 0 or p.Mp <= 0) continues as plain
+    CG from its iterate (steepest-descent restart).
+
+    With ``graph`` one iteration is captured as a hipGraph and replayed; the
+    host only checks for convergence between replays.  When the images still
+    running fall to ``compact`` times the current batch (and ``compact`` > 0),
+    they are gathered into a smaller dense batch and the iteration is
+    re-captured, so converged images cost nothing.  Returns (x, iterations
+    per image, converged mask, image-operator applications = sum over
+    iterations of the batch size operated on)."""
     torch = _torch()
     B = b.shape[0]
     b = b.contiguous()
-    ws = _kernels.cg_workspace(b)
+    dev_ = b.device
     x = torch.zeros_like(b) if x0 is None else x0.contiguous().clone()
     r = (b - op(x)).contiguous() if x0 is not None else b.clone()
     tol = rtol * bdot(b, b).sqrt()
-    use_p = torch.full((B,), prec is not None, dtype=torch.bool, device=b.device)
-    zero = torch.zeros(B, dtype=torch.float64, device=b.device)
-    one = torch.ones(B, dtype=torch.float64, device=b.device)
+    use_p = torch.full((B,), prec is not None, dtype=torch.bool, device=dev_)
     rr = bdot(r, r)
     if prec is None:
         z, rz = r, rr.clone()
@@ -599,48 +629,78 @@ def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=True):
         rz = torch.where(bad, rr, rz)
     p = z.clone()
     active = rr.sqrt() > tol
-    iters = torch.zeros(B, dtype=torch.int64, device=b.device)
+    iters = torch.zeros(B, dtype=torch.int64, device=dev_)
+    # results by original index; the working set is a dense sub-batch
+    x_out = x
+    iters_out = iters
+    conv_out = ~active
+    idx = torch.arange(B, device=dev_)
+    st = {"x": x, "r": r, "p": p, "rz": rz, "tol": tol, "use_p": use_p, "active": active,
+          "iters": iters}
 
-    def step():
-        q = op(p)
-        pq = _kernels.batched_dot(p, q, ws)
-        ok = active & (pq > 0)
-        alpha = torch.where(ok, rz / torch.where(pq > 0, pq, one), zero)
-        rr_new = _kernels.cg_update(x, r, p, q, alpha, ws)
-        iters.add_(active.long())
-        new_active = active & (rr_new.sqrt() > tol)
-        if prec is None:
-            zz, rz_new = r, rr_new
-            beta = rz_new / torch.where(rz != 0, rz, one)
-        else:
-            zz = torch.where(use_p[:, None, None], prec(r), r)
-            rz_new = _kernels.batched_dot(r, zz, ws)
-            bad = new_active & use_p & ((rz_new <= 0) | (pq <= 0))
-            use_p.logical_and_(~bad)
-            zz = torch.where(bad[:, None, None], r, zz)
-            rz_new = torch.where(bad, rr_new, rz_new)
-            beta = torch.where(bad, zero, rz_new / torch.where(rz != 0, rz, one))
-        _kernels.cg_direction(p, zz, beta)
-        rz.copy_(rz_new)
-        active.copy_(new_active)
+    def make_step(S):
+        nb = S["x"].shape[0]
+        ws = _kernels.cg_workspace(S["x"])
+        zero = torch.zeros(nb, dtype=torch.float64, device=dev_)
+        one = torch.ones(nb, dtype=torch.float64, device=dev_)
+        xs, rs, ps, rzs, tols = S["x"], S["r"], S["p"], S["rz"], S["tol"]
+        ups, acts, its = S["use_p"], S["active"], S["iters"]
 
-    g = None
-    if graph:
+        def step():
+            q = op(ps)
+            pq = _kernels.batched_dot(ps, q, ws)
+            ok = acts & (pq > 0)
+            alpha = torch.where(ok, rzs / torch.where(pq > 0, pq, one), zero)
+            rr_new = _kernels.cg_update(xs, rs, ps, q, alpha, ws)
+            its.add_(acts.long())
+            new_active = acts & (rr_new.sqrt() > tols)
+            if prec is None:
+                zz, rz_new = rs, rr_new
+                beta = rz_new / torch.where(rzs != 0, rzs, one)
+            else:
+                zz = torch.where(ups[:, None, None], prec(rs), rs)
+                rz_new = _kernels.batched_dot(rs, zz, ws)
+                bad = new_active & ups & ((rz_new <= 0) | (pq <= 0))
+                ups.logical_and_(~bad)
+                zz = torch.where(bad[:, None, None], rs, zz)
+                rz_new = torch.where(bad, rr_new, rz_new)
+                beta = torch.where(bad, zero, rz_new / torch.where(rzs != 0, rzs, one))
+            _kernels.cg_direction(ps, zz, beta)
+            rzs.copy_(rz_new)
+            acts.copy_(new_active)
+
+        if not graph:
+            return step
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             step()
+        return g.replay
+
+    def write_back(S, rows):
+        if S["x"] is not x_out:  # the full batch is written in place
+            x_out[rows] = S["x"]
+            iters_out[rows] = S["iters"]
+        conv_out[rows] = ~S["active"]
+
+    run = make_step(st)
     apps = 0
     for _ in range(maxiter):
-        if not bool(active.any()):
+        n_active = int(st["active"].sum())
+        if n_active == 0:
             break
-        if g is not None:
-            g.replay()
-        else:
-            step()
-        apps += 1
-    converged = ~active
-    return x, iters, converged, apps
+        nb = st["x"].shape[0]
+        if compact and nb > 8 and n_active <= compact * nb:
+            write_back(st, idx)
+            keep = st["active"].nonzero().squeeze(1)
+            idx = idx[keep]
+            st = {k: v[keep].contiguous() for k, v in st.items()}
+            run = make_step(st)
+            nb = n_active
+        run()
+        apps += nb
+    write_back(st, idx)
+    return x_out, iters_out, conv_out, apps
 
 
 def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
@@ -671,18 +731,23 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
         xs, it, conv, apps = res
         resid = (bdot(b - normal_op(xs), b - normal_op(xs)).sqrt() / bn)
         return {"seconds": round(secs, 4), "iterations_mean": round(float(it.float().mean()), 2),
-                "iterations_max": int(it.max()), "operator_applications": apps,
+                "iterations_max": int(it.max()), "image_operator_applications": apps,
                 "converged": int(conv.sum()), "max_rel_residual": float(resid.max()),
                 "rel_error_vs_phantom": float((bdot(xs - x, xs - x) / bdot(x, x)).sqrt().mean())}
 
-    plain, t_plain = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter))
-    # warm the network buffers outside the timed region (allocation only)
+    # one-time costs (kernel loading, first graph captures, the network's
+    # buffers) are paid before any timed solve
     solver.net.buffers(count, solver.n, train=False)
+    solver.initial_guess(b, train=False)
+    pcg(normal_op, b, rtol=rtol, maxiter=3)
+    pcg(normal_op, b, prec=solver.pyramid, rtol=rtol, maxiter=3)
+    plain, t_plain = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter))
     learned, t_learned = run(lambda: pcg(normal_op, b, solver.initial_guess(b, train=False),
                                          prec=solver.pyramid, rtol=rtol, maxiter=maxiter))
     prec_only, t_prec = run(lambda: pcg(normal_op, b, prec=solver.pyramid, rtol=rtol,
                                         maxiter=maxiter))
     x0, t_net = run(lambda: solver.initial_guess(b, train=False))
+    xs = plain[0]
     out = {
         "count": count, "n": solver.n, "rtol": rtol, "tuned_adrt_bdrt": tuning,
         "plain_cg": stats("plain", plain, t_plain),
@@ -690,6 +755,8 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
         "learned": stats("learned", learned, t_learned),
         "network_seconds": round(t_net, 4),
         "initial_guess_rel_error": float((bdot(x0 - x, x0 - x) / bdot(x, x)).sqrt().mean()),
+        "initial_guess_rel_error_vs_solution": float(
+            (bdot(x0 - xs, x0 - xs) / bdot(xs, xs)).sqrt().mean()),
         "speedup_learned_vs_plain": round(t_plain / t_learned, 3),
         "speedup_preconditioner_only_vs_plain": round(t_plain / t_prec, 3),
         "learned_vs_plain_solution_rel_diff": float(

@@ -5,7 +5,10 @@
 Fits the multiscale preconditioner (least squares), then trains the U-Net
 with Adam (cosine learning-rate decay) on seeded phantom pairs generated on
 the device: phantom i of seed ``--seed``, noisy ADRT data at ``--sigma`` of
-max|adrt(x)|, input = preconditioned normal-equation right-hand side.
+max|adrt(x)|, input = preconditioned normal-equation right-hand side,
+target = the clean phantom (default), the normal equations' solution
+(``--target solution``, computed by preconditioned CG per batch) or no target
+(``--target residual``: the relative residual of the guess itself).
 Progress lines go to stdout; the weights, preconditioner coefficients and
 loss history go to ``--out`` (NumPy .npz, no pickles).
 """
@@ -29,6 +32,7 @@ def main(argv=None):
     p.add_argument("--time-limit", type=float, default=0.0,
                    help="stop after this many seconds of training (0: no limit)")
     p.add_argument("--log-every", type=int, default=100)
+    p.add_argument("--target", default="phantom", choices=["phantom", "solution", "residual"])
     p.add_argument("--out", required=True)
     args = p.parse_args(argv)
 
@@ -62,14 +66,14 @@ def main(argv=None):
         offset = steps_done
         solver.train(k, batch=args.batch, seed=args.seed, pairs=args.pairs, log_every=k,
                      log=lambda s, l: log(offset + s, l),
-                     lr_fn=lambda s: lr_fn(offset + s), start=offset)
+                     lr_fn=lambda s: lr_fn(offset + s), start=offset, target=args.target)
         steps_done += k
         if args.time_limit and time.perf_counter() - t1 > args.time_limit:
             break
     torch.cuda.synchronize()
     secs = time.perf_counter() - t1
     solver.save(args.out, losses=losses, steps=steps_done, train_seconds=secs,
-                batch=args.batch, pairs=args.pairs)
+                batch=args.batch, pairs=args.pairs, target=args.target)
     print(json.dumps({"event": "done", "steps": steps_done, "seconds": round(secs, 2),
                       "ms_per_step": round(1e3 * secs / max(steps_done, 1), 2),
                       "out": args.out}), flush=True)

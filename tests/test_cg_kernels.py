@@ -89,3 +89,21 @@ class TestGraphPCG:
             xg, ig, cg_, ag = LN.pcg(LN.normal_op, b, prec=prec, rtol=1e-5, graph=True)
             assert ae == ag and torch.equal(ie, ig) and torch.equal(ce, cg_)
             assert torch.equal(xe, xg)
+
+    def test_compaction_matches_full_batch(self, cuda):
+        """Gathering the still-running images into smaller batches gives the
+        full-batch solve's iterates (up to the float64 sum order of the
+        per-image dot products, which depends on the batch shape)."""
+        import torch
+        from adrt_amd import learned as LN
+
+        solver = LN.LearnedSolver(n=64, device=cuda)
+        solver.fit_preconditioner(count=4)
+        x, b = solver.data(seed=8, start=0, count=24, noise_seed=9)
+        for prec in (None, solver.pyramid):
+            xf, itf, cf, af = LN.pcg(LN.normal_op, b, prec=prec, rtol=1e-5, compact=0)
+            xc, itc, cc, ac = LN.pcg(LN.normal_op, b, prec=prec, rtol=1e-5, compact=0.9)
+            assert bool(cf.all()) and bool(cc.all())
+            assert int((itf - itc).abs().max()) <= 1
+            assert float((xf - xc).norm() / xf.norm()) < 1e-5
+            assert ac <= af
