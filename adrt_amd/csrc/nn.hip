@@ -161,6 +161,115 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_kernel(ConvArgs a) {
     }
 }
 
+// LDS-tiled form for W a multiple of 16 (>= 16), H a multiple of 4 and
+// Cin >= 16: a work-group computes 4 output rows x TW = min(W, 64) columns x
+// 16*NT output channels (one row per wave).  Input channels go through LDS
+// 32 at a time: the (4+2) x (TW+2) pixel tile with its zero-padded halo is
+// loaded once per chunk and read by all 9 taps from LDS.  Inside a chunk
+// k = tap * ccv + c runs through the chunk's taps and channels as in the
+// kernel above.  Pixel rows in LDS are padded to 40 channels (80 bytes), so
+// 16 lanes reading 16 consecutive pixels hit distinct banks.
+constexpr int kTH = 4, kCC = 32, kCCP = kCC + 8, kTWmax = 64;
+
+template <int NT, int MODE>
+__global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int TW) {
+    __shared__ __align__(16) bf16 tile[kTH + 2][kTWmax + 2][kCCP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int co0 = int(blockIdx.y) * 16 * NT;
+    const int tiles_x = a.W / TW, tiles_y = a.H / kTH;
+    int tt = int(blockIdx.x);
+    const int tx = tt % tiles_x;
+    tt /= tiles_x;
+    const int ty = tt % tiles_y;
+    const long long b = tt / tiles_y;
+    const int x0 = tx * TW, y0 = ty * kTH;
+    const int MT = TW / 16;
+    const long long img = b * a.H * a.W;  // first pixel of the image
+
+    f32x4 acc[kMT][NT];
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int K = 9 * a.Cin;
+    for (int c0 = 0; c0 < a.Cin; c0 += kCC) {
+        const int ccv = a.Cin - c0 < kCC ? a.Cin - c0 : kCC;
+        const int nvec = ccv / 8;
+        const int npix = (kTH + 2) * (TW + 2);
+        __syncthreads();  // the previous chunk's readers are done
+        for (int i = threadIdx.x; i < npix * nvec; i += kWgThreads) {
+            const int v = i % nvec, pix = i / nvec;
+            const int py = pix / (TW + 2), px = pix - py * (TW + 2);
+            const int gy = y0 + py - 1, gx = x0 + px - 1;
+            const bool ok = unsigned(gy) < unsigned(a.H) && unsigned(gx) < unsigned(a.W);
+            const long long q = ok ? img + (long long)gy * a.W + gx : img;
+            const bf16x8 val = keep_or_zero(load8(a.x + q * a.x_stride + c0 + 8 * v), ok);
+            *reinterpret_cast<bf16x8 *>(&tile[py][px][8 * v]) = val;
+        }
+        __syncthreads();
+        const int Kc = 9 * ccv;
+        int k = 8 * g;
+        int tap = k / ccv, ci = k - tap * ccv;
+        for (int k0 = 0; k0 < Kc; k0 += 32) {
+            const bool kok = k < Kc;
+            const int tc = kok ? tap : 4;
+            const int dy = tc / 3, dx = tc - (tc / 3) * 3;
+            const int cc = kok ? ci : 0;
+            bf16x8 A[kMT], Bf[NT];
+#pragma unroll
+            for (int mt = 0; mt < kMT; ++mt)
+                if (mt < MT)
+                    A[mt] = keep_or_zero(
+                        *reinterpret_cast<const bf16x8 *>(&tile[wave + dy][16 * mt + r + dx][cc]),
+                        kok);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                Bf[nt] = keep_or_zero(
+                    load8(a.w + (size_t)(co0 + 16 * nt + r) * K + tc * a.Cin + c0 + cc), kok);
+#pragma unroll
+            for (int mt = 0; mt < kMT; ++mt)
+                if (mt < MT)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(A[mt], Bf[nt], acc[mt][nt]);
+            k += 32;
+            ci += 32;
+            while (ci >= ccv) {
+                ci -= ccv;
+                ++tap;
+            }
+        }
+    }
+
+    const long long prow = img + (long long)(y0 + wave) * a.W + x0;
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt) {
+        if (mt >= MT) break;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = co0 + 16 * nt + r;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long p = prow + 16 * mt + 4 * g + j;
+                float v = acc[mt][nt][j];
+                if (MODE == kReluBias) {
+                    v += a.bias[co];
+                    v = v > 0.f ? v : 0.f;
+                    a.y[p * a.y_stride + co] = bf16(v);
+                } else if (MODE == kResidual) {
+                    if (co == 0) a.y32[p] = a.resid[p] + (v + a.bias[0]);
+                } else if (MODE == kMaskInput) {
+                    const float m = float(a.mask[p * a.mask_stride + co]);
+                    a.y[p * a.y_stride + co] = bf16(m > 0.f ? v : 0.f);
+                } else {
+                    a.y[p * a.y_stride + co] = bf16(v);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Weight gradient: partial[part][co][t][ci] = sum over the part's pixels of
 // dZ[p][co] * X[p + tap t][ci]; partial_bias[part][co] = sum dZ[p][co].
@@ -303,17 +412,18 @@ __device__ __forceinline__ bf16x8 to_bf16(const Vec8 &f) {
 }
 __device__ __forceinline__ void store8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
 
+// Grids: blockIdx.y = image, x covers one image's (pixel, 8-channel group)
+// items in 32-bit arithmetic.
+
 // y (B, H/2, W/2) = average of 2x2 blocks of x (B, H, W); C channels.
 __global__ void __launch_bounds__(kWgThreads)
-    pool_kernel(const bf16 *x, int xs, bf16 *y, int ys, long long Pc, int H, int W, int C) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c8 = C / 8;
-    if (i >= Pc * c8) return;
-    const int c = int(i % c8) * 8;
-    const long long q = i / c8;
-    const int Wc = W / 2, Hc = H / 2;
-    const int xq = int(q % Wc), yq = int((q / Wc) % Hc);
-    const long long b = q / ((long long)Wc * Hc);
+    pool_kernel(const bf16 *x, int xs, bf16 *y, int ys, int H, int W, int C) {
+    const int c8 = C / 8, Wc = W / 2, Hc = H / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Hc * Wc * c8) return;
+    const int c = (i % c8) * 8, q = i / c8;
+    const int xq = q % Wc, yq = q / Wc;
+    const long long b = blockIdx.y;
     const long long p00 = (b * H + 2 * yq) * W + 2 * xq;
     const Vec8 a0 = to_f32(load8(x + p00 * xs + c)), a1 = to_f32(load8(x + (p00 + 1) * xs + c));
     const Vec8 a2 = to_f32(load8(x + (p00 + W) * xs + c)),
@@ -321,20 +431,20 @@ __global__ void __launch_bounds__(kWgThreads)
     Vec8 o;
 #pragma unroll
     for (int j = 0; j < 8; ++j) o.v[j] = 0.25f * ((a0.v[j] + a1.v[j]) + (a2.v[j] + a3.v[j]));
-    store8(y + q * ys + c, to_bf16(o));
+    store8(y + ((b * Hc + yq) * Wc + xq) * ys + c, to_bf16(o));
 }
 
 // ds (B, H, W) = (dp[parent] / 4 + dskip) * (s > 0)   (H, W: the fine size)
 __global__ void __launch_bounds__(kWgThreads)
     pool_backward_kernel(const bf16 *dp, int dps, const bf16 *dskip, int dks, const bf16 *s,
-                         int ss, bf16 *ds, int dss, long long P, int H, int W, int C) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                         int ss, bf16 *ds, int dss, int H, int W, int C) {
     const int c8 = C / 8;
-    if (i >= P * c8) return;
-    const int c = int(i % c8) * 8;
-    const long long p = i / c8;
-    const int xx = int(p % W), yy = int((p / W) % H);
-    const long long b = p / ((long long)W * H);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W * c8) return;
+    const int c = (i % c8) * 8, pi = i / c8;
+    const int xx = pi % W, yy = pi / W;
+    const long long b = blockIdx.y;
+    const long long p = b * H * W + pi;
     const long long q = (b * (H / 2) + yy / 2) * (W / 2) + xx / 2;
     const Vec8 g = to_f32(load8(dp + q * dps + c));
     const Vec8 k = to_f32(load8(dskip + p * dks + c));
@@ -345,32 +455,33 @@ __global__ void __launch_bounds__(kWgThreads)
     store8(ds + p * dss + c, to_bf16(o));
 }
 
-// y (B, 2H, 2W) = x[parent]   (H, W: the coarse size)
+// y (B, 2H, 2W) = x[parent]   (H, W: the coarse size); a thread writes the
+// 2x2 children of one coarse pixel.
 __global__ void __launch_bounds__(kWgThreads)
-    upsample_kernel(const bf16 *x, int xs, bf16 *y, int ys, long long Pf, int H, int W, int C) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    upsample_kernel(const bf16 *x, int xs, bf16 *y, int ys, int H, int W, int C) {
     const int c8 = C / 8;
-    if (i >= Pf * c8) return;
-    const int c = int(i % c8) * 8;
-    const long long p = i / c8;
-    const int W2 = 2 * W, H2 = 2 * H;
-    const int xx = int(p % W2), yy = int((p / W2) % H2);
-    const long long b = p / ((long long)W2 * H2);
-    const long long q = (b * H + yy / 2) * W + xx / 2;
-    store8(y + p * ys + c, load8(x + q * xs + c));
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W * c8) return;
+    const int c = (i % c8) * 8, qi = i / c8;
+    const int xq = qi % W, yq = qi / W;
+    const long long b = blockIdx.y;
+    const bf16x8 v = load8(x + (b * H * W + qi) * xs + c);
+    const long long p00 = (b * 2 * H + 2 * yq) * (2 * W) + 2 * xq;
+    store8(y + p00 * ys + c, v);
+    store8(y + (p00 + 1) * ys + c, v);
+    store8(y + (p00 + 2 * W) * ys + c, v);
+    store8(y + (p00 + 2 * W + 1) * ys + c, v);
 }
 
 // dx (B, H, W) = sum of the 2x2 children of dy (B, 2H, 2W)   (H, W coarse)
 __global__ void __launch_bounds__(kWgThreads)
-    upsample_backward_kernel(const bf16 *dy, int dys, bf16 *dx, int dxs, long long Pc, int H,
-                             int W, int C) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    upsample_backward_kernel(const bf16 *dy, int dys, bf16 *dx, int dxs, int H, int W, int C) {
     const int c8 = C / 8;
-    if (i >= Pc * c8) return;
-    const int c = int(i % c8) * 8;
-    const long long q = i / c8;
-    const int xq = int(q % W), yq = int((q / W) % H);
-    const long long b = q / ((long long)W * H);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * W * c8) return;
+    const int c = (i % c8) * 8, qi = i / c8;
+    const int xq = qi % W, yq = qi / W;
+    const long long b = blockIdx.y;
     const long long p00 = (b * 2 * H + 2 * yq) * (2 * W) + 2 * xq;
     const Vec8 a0 = to_f32(load8(dy + p00 * dys + c)), a1 = to_f32(load8(dy + (p00 + 1) * dys + c));
     const Vec8 a2 = to_f32(load8(dy + (p00 + 2 * W) * dys + c)),
@@ -378,7 +489,7 @@ __global__ void __launch_bounds__(kWgThreads)
     Vec8 o;
 #pragma unroll
     for (int j = 0; j < 8; ++j) o.v[j] = (a0.v[j] + a1.v[j]) + (a2.v[j] + a3.v[j]);
-    store8(dx + q * dxs + c, to_bf16(o));
+    store8(dx + (b * H * W + qi) * dxs + c, to_bf16(o));
 }
 
 // y (P, 8) bf16: channel 0 = scale * x[p], channels 1..7 = 0.
@@ -535,7 +646,24 @@ inline unsigned blocks_for(long long work, int block = kWgThreads) {
 }
 
 template <int NT>
+static void launch_conv_lds(const ConvArgs &a, int mode, hipStream_t s) {
+    const int TW = a.W < kTWmax ? a.W : kTWmax;
+    const long long tiles = (a.P / a.W / kTH) * (a.W / TW);
+    const dim3 grid(unsigned(tiles), unsigned(a.Cout / (16 * NT)));
+    switch (mode) {
+        case kReluBias: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kReluBias>), grid, dim3(kWgThreads), 0, s, a, TW); break;
+        case kResidual: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kResidual>), grid, dim3(kWgThreads), 0, s, a, TW); break;
+        case kMaskInput: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kMaskInput>), grid, dim3(kWgThreads), 0, s, a, TW); break;
+        default: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kLinear>), grid, dim3(kWgThreads), 0, s, a, TW); break;
+    }
+}
+
+template <int NT>
 static void launch_conv_nt(const ConvArgs &a, int mode, hipStream_t s) {
+    if (a.W % 16 == 0 && a.H % kTH == 0 && a.Cin >= 16 && (a.W <= kTWmax || a.W % kTWmax == 0)) {
+        launch_conv_lds<NT>(a, mode, s);
+        return;
+    }
     const long long px_blocks = a.P / 64;
     const unsigned grid = unsigned((px_blocks + 3) / 4 * (a.Cout / (16 * NT)));
     switch (mode) {
@@ -677,10 +805,11 @@ int adrt_hip_nn_pool(const void *x, int x_stride, void *y, int y_stride, int bat
                      int c, void *stream) {
     if (batch < 1 || !nhwc_ok(h, w, c, x_stride) || y_stride < c || y_stride % 8) return ADRT_HIP_ERR_ARG;
     if (!x || !y) return ADRT_HIP_ERR_NULL;
-    const long long Pc = (long long)batch * (h / 2) * (w / 2);
-    hipLaunchKernelGGL(pool_kernel, dim3(blocks_for(Pc * (c / 8))), dim3(kWgThreads), 0,
+    const long long items = (long long)(h / 2) * (w / 2) * (c / 8);
+    if (items >= (1ll << 31) || batch > 65535) return ADRT_HIP_ERR_TOO_BIG;
+    hipLaunchKernelGGL(pool_kernel, dim3(blocks_for(items), unsigned(batch)), dim3(kWgThreads), 0,
                        static_cast<hipStream_t>(stream), static_cast<const bf16 *>(x), x_stride,
-                       static_cast<bf16 *>(y), y_stride, Pc, h, w, c);
+                       static_cast<bf16 *>(y), y_stride, h, w, c);
     return launch_status();
 }
 
@@ -691,11 +820,13 @@ int adrt_hip_nn_pool_backward(const void *dp, int dp_stride, const void *dskip, 
         ds_stride < c || dp_stride % 8 || dskip_stride % 8 || ds_stride % 8)
         return ADRT_HIP_ERR_ARG;
     if (!dp || !dskip || !s || !ds) return ADRT_HIP_ERR_NULL;
-    const long long P = (long long)batch * h * w;
-    hipLaunchKernelGGL(pool_backward_kernel, dim3(blocks_for(P * (c / 8))), dim3(kWgThreads), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(dp), dp_stride,
-                       static_cast<const bf16 *>(dskip), dskip_stride, static_cast<const bf16 *>(s),
-                       s_stride, static_cast<bf16 *>(ds), ds_stride, P, h, w, c);
+    const long long items = (long long)h * w * (c / 8);
+    if (items >= (1ll << 31) || batch > 65535) return ADRT_HIP_ERR_TOO_BIG;
+    hipLaunchKernelGGL(pool_backward_kernel, dim3(blocks_for(items), unsigned(batch)),
+                       dim3(kWgThreads), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const bf16 *>(dp), dp_stride, static_cast<const bf16 *>(dskip),
+                       dskip_stride, static_cast<const bf16 *>(s), s_stride,
+                       static_cast<bf16 *>(ds), ds_stride, h, w, c);
     return launch_status();
 }
 
@@ -705,10 +836,11 @@ int adrt_hip_nn_upsample(const void *x, int x_stride, void *y, int y_stride, int
         y_stride < c || y_stride % 8)
         return ADRT_HIP_ERR_ARG;
     if (!x || !y) return ADRT_HIP_ERR_NULL;
-    const long long Pf = (long long)batch * (2 * h) * (2 * w);
-    hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for(Pf * (c / 8))), dim3(kWgThreads), 0,
-                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(x), x_stride,
-                       static_cast<bf16 *>(y), y_stride, Pf, h, w, c);
+    const long long items = (long long)h * w * (c / 8);
+    if (items >= (1ll << 31) || batch > 65535) return ADRT_HIP_ERR_TOO_BIG;
+    hipLaunchKernelGGL(upsample_kernel, dim3(blocks_for(items), unsigned(batch)), dim3(kWgThreads),
+                       0, static_cast<hipStream_t>(stream), static_cast<const bf16 *>(x), x_stride,
+                       static_cast<bf16 *>(y), y_stride, h, w, c);
     return launch_status();
 }
 
@@ -718,10 +850,12 @@ int adrt_hip_nn_upsample_backward(const void *dy, int dy_stride, void *dx, int d
         dx_stride < c || dx_stride % 8)
         return ADRT_HIP_ERR_ARG;
     if (!dy || !dx) return ADRT_HIP_ERR_NULL;
-    const long long Pc = (long long)batch * h * w;
-    hipLaunchKernelGGL(upsample_backward_kernel, dim3(blocks_for(Pc * (c / 8))), dim3(kWgThreads),
-                       0, static_cast<hipStream_t>(stream), static_cast<const bf16 *>(dy),
-                       dy_stride, static_cast<bf16 *>(dx), dx_stride, Pc, h, w, c);
+    const long long items = (long long)h * w * (c / 8);
+    if (items >= (1ll << 31) || batch > 65535) return ADRT_HIP_ERR_TOO_BIG;
+    hipLaunchKernelGGL(upsample_backward_kernel, dim3(blocks_for(items), unsigned(batch)),
+                       dim3(kWgThreads), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const bf16 *>(dy), dy_stride, static_cast<bf16 *>(dx), dx_stride,
+                       h, w, c);
     return launch_status();
 }
 
