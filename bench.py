@@ -63,7 +63,7 @@ def parse():
                    help="calls in the profiled run (--summarize)")
     p.add_argument("--match", default="adrt_hip",
                    help="count only kernels whose name contains this (--summarize)")
-    p.add_argument("--op", default="adrt", choices=["adrt", "bdrt", "iadrt"],
+    p.add_argument("--op", default="adrt", choices=["adrt", "bdrt", "iadrt", "unet"],
                    help="operation a --profile run calls (bdrt/iadrt on the forward output)")
     return p.parse_args()
 
@@ -302,6 +302,19 @@ def main():
     lib = _lib.load()
     torch.cuda.init()
     device = torch.device("cuda", 0)
+    if args.profile and args.op == "unet":
+        # two U-Net forwards (the learned initial guess) on 1000 random
+        # 256x256 images with the committed weights
+        from adrt_amd import learned
+
+        solver = learned.LearnedSolver.load(LEARNED_WEIGHTS)
+        z = torch.randn(1000, 256, 256, device=device, generator=torch.Generator(
+            device=device).manual_seed(0))
+        for _ in range(2):
+            solver.net.forward(z, train=False)
+        torch.cuda.synchronize()
+        print(json.dumps({"op": "unet", "images": 1000, "n": 256}), flush=True)
+        return
     n, batch = args.n, args.batch
     s = 4 if args.dtype == "float32" else 8
     imgs = phantoms(batch, n, seed=0, dtype=args.dtype, device=device)
