@@ -388,6 +388,7 @@ __host__ __device__ __forceinline__ int x_pitch(int n, int k1) {
 // ---------------------------------------------------------------------------
 template <typename T, int KB>
 struct Pass1 {
+    static constexpr bool kStaged = false;
     static constexpr int KA = 3;
     static constexpr int k1 = KA + KB;
     static constexpr int RA = 1 << KA, RB = 1 << KB;
@@ -602,6 +603,7 @@ struct Pass1 {
 // ---------------------------------------------------------------------------
 template <typename T, int KD>
 struct Pass2 {
+    static constexpr bool kStaged = false;
     static constexpr int KC = 3;
     static constexpr int k2 = KC + KD;
     static constexpr int RC = 1 << KC, RD = 1 << KD;
@@ -798,6 +800,17 @@ struct BdrtA {
     static constexpr int tasks0 = R1;                // level-1 strips g
     static constexpr int tasks1 = K2 == 0 ? 0 : 8;  // level-2 strips s1
     static constexpr size_t lds_elems = K2 == 0 ? 0 : size_t(R1) * 8 * kRow;
+    // With LDS (K2 > 0) the item's input block - 256 rows x 2^kA angles,
+    // 2^kA * sizeof(T) contiguous bytes per row - is first staged through
+    // LDS by the whole work-group with row-wise loads, transposed to
+    // [angle][row] so that each strip lane reads its 4 rows of an angle as
+    // one 16-byte LDS read.  (Read directly, a strip's loads touch one cache
+    // line per lane.)  The staging area is reused by the level-1 rows after a
+    // barrier.
+    static constexpr bool kStaged = K2 > 0;
+    static constexpr int kSA = 8 * R1;                 // angles of an item
+    static constexpr int kSP = kLanes * kW + 4;        // staged row pitch
+    static_assert(!kStaged || size_t(kSA) * kSP <= lds_elems, "staging fits the LDS");
 
 #ifndef ADRT_HOST_EMULATION
     __device__ __forceinline__ static void strip0(T (&v)[R0][kW]) { strip_bdrt<3, T>(v); }
@@ -857,6 +870,42 @@ struct BdrtA {
             const T *src = it.y + unsigned(rc * it.n + a0);
             load_vec4(src, v[0][i], v[1][i], v[2][i], v[3][i]);
             load_vec4(src + 4, v[4][i], v[5][i], v[6][i], v[7][i]);
+        }
+#pragma unroll
+        for (int i = 0; i < kW; ++i) {
+            const int r = it.c0 + kW * lane + i;
+            if (r >= it.C) {
+                const T pad = r == it.C ? neg_zero<T>() : T(0);
+#pragma unroll
+                for (int j = 0; j < R0; ++j) v[j][i] = pad;
+            }
+        }
+    }
+
+    // stg[a][rr] = y[min(c0 + rr, C - 1)][p * 2^kA + a], rr < 256, a < 2^kA.
+    __host__ __device__ __forceinline__ static void stage(const Item &it, T *lds, int tid) {
+        constexpr int vecs = kSA / 4;  // 4-element vectors per input row
+        for (int f = tid; f < kLanes * kW * vecs; f += kThreads) {
+            const int rr = f / vecs, vq = f - rr * vecs;
+            const int r = it.c0 + rr;
+            const int rc = r < it.C ? r : it.C - 1;
+            T a, b, c, d;
+            load_vec4(it.y + unsigned(rc * it.n + (it.p << kA) + 4 * vq), a, b, c, d);
+            T *col = lds + (4 * vq) * kSP + rr;
+            col[0] = a;
+            col[kSP] = b;
+            col[2 * kSP] = c;
+            col[3 * kSP] = d;
+        }
+    }
+
+    // load0 from the staged block (same values, then the same padding).
+    __host__ __device__ __forceinline__ static void load0s(const Item &it, int g, int lane,
+                                                           const T *lds, T (&v)[R0][kW]) {
+#pragma unroll
+        for (int j = 0; j < R0; ++j) {
+            const T *src = lds + (R0 * g + j) * kSP + kW * lane;
+            load_vec4(src, v[j][0], v[j][1], v[j][2], v[j][3]);
         }
 #pragma unroll
         for (int i = 0; i < kW; ++i) {
@@ -930,6 +979,7 @@ struct BdrtA {
 
 template <typename T, int K4>
 struct BdrtB {
+    static constexpr bool kStaged = false;
     static constexpr int kB = 3 + K4;
     static constexpr int R0 = 8, R1 = 1 << K4;
     static constexpr int A2 = 1 << kB;  // output columns per tile row
@@ -1077,7 +1127,19 @@ __device__ __forceinline__ void run_item(int items, T *lds, A... args) {
     const typename P::Item it = P::item(args..., sc.idx);
     const int wave = wave_uniform(threadIdx.x / kLanes);
     const int lane = threadIdx.x % kLanes;
-    if (wave < P::tasks0) {
+    if constexpr (P::kStaged) {
+        // cooperative staged load; the staging area is the level-1 area, so
+        // every strip reads its block before any strip stores
+        P::stage(it, lds, threadIdx.x);
+        __syncthreads();
+        T v[P::R0][kW];
+        if (wave < P::tasks0) P::load0s(it, wave, lane, lds, v);
+        __syncthreads();
+        if (wave < P::tasks0) {
+            P::strip0(v);
+            P::store0(it, wave, lane, v, lds);
+        }
+    } else if (wave < P::tasks0) {
         T v[P::R0][kW];
         P::load0(it, wave, lane, v);
         P::strip0(v);

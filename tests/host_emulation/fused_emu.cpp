@@ -40,10 +40,27 @@ static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args.
             const auto it = P::item(args..., idx);
             for (auto &x : lds) x = T(NAN);
             T *l = P::lds_elems > 0 ? lds.data() : nullptr;
-            for (int task = 0; task < P::tasks0; ++task) {
-                for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
-                P::strip0_host(v0);
-                for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
+            if constexpr (P::kStaged) {
+                // staged load by the whole work-group, then every strip reads
+                // its block before any strip stores (they share the LDS area)
+                for (int tid = 0; tid < kThreads; ++tid) P::stage(it, l, tid);
+                std::vector<T> keep0(size_t(P::tasks0) * buf0.size());
+                for (int task = 0; task < P::tasks0; ++task) {
+                    for (int lane = 0; lane < kLanes; ++lane) P::load0s(it, task, lane, l, v0[lane]);
+                    std::memcpy(keep0.data() + size_t(task) * buf0.size(), buf0.data(), buf0.size() * sizeof(T));
+                }
+                for (auto &x : lds) x = T(NAN);
+                for (int task = 0; task < P::tasks0; ++task) {
+                    std::memcpy(buf0.data(), keep0.data() + size_t(task) * buf0.size(), buf0.size() * sizeof(T));
+                    P::strip0_host(v0);
+                    for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
+                }
+            } else {
+                for (int task = 0; task < P::tasks0; ++task) {
+                    for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
+                    P::strip0_host(v0);
+                    for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
+                }
             }
             // Phase 1: every strip loads and computes before any stores (the
             // stores may overwrite the LDS rows the loads read).
