@@ -58,6 +58,21 @@ def _step(fn_name, t, step):
     return out
 
 
+def stage_chain(t, first=0, last=None):
+    """adrt_step(first), ..., adrt_step(last - 1) in one call (ADRT layout)."""
+    lib = _lib.load()
+    b, n = _batch_n(t, 3)
+    L = int(lib.adrt_hip_num_iters(n))
+    last = L if last is None else last
+    out = dev.empty_like_dtype(t, t.shape)
+    code = _code(t)
+    nbytes = lib.adrt_hip_stage_chain_workspace(code, b, n) if last - first > 1 else 0
+    ws = dev.workspace(t, nbytes)
+    _lib.check(lib.adrt_hip_stage_chain(dev.ptr(t), dev.ptr(out), dev.ptr(ws), nbytes, code, b, n,
+                                        int(first), int(last), dev.stream_ptr(t)), "stage_chain")
+    return out
+
+
 def adrt_step(t, step):
     return _step("adrt_hip_adrt_step", t, step)
 

@@ -31,11 +31,12 @@ def _adjoint(g):
 
 
 def _stage_chain(h):
-    """adrt_step(0) then ... adrt_step(L-1) on an ADRT-layout buffer."""
-    out = h.contiguous()
-    for k in range(num_iters(int(h.shape[-1]))):
-        out = _kernels.adrt_step(out, k)
-    return out
+    """adrt_step(0) then ... adrt_step(L-1) on an ADRT-layout buffer (one
+    library call)."""
+    h = h.contiguous()
+    if num_iters(int(h.shape[-1])) == 0:
+        return h.clone()
+    return _kernels.stage_chain(h)
 
 
 def _function_classes():

@@ -243,6 +243,34 @@ int adrt_hip_truncate_combine(const void *in, void *out, int dtype, size_t batch
                                    static_cast<hipStream_t>(stream));
 }
 
+size_t adrt_hip_stage_chain_workspace(int dtype, size_t batch, size_t n) {
+    if (!dtype_ok(dtype) || !valid_output(batch, n)) return 0;
+    return batch * 4 * (2 * n - 1) * n * (dtype == ADRT_HIP_F64 ? 8 : 4);
+}
+
+int adrt_hip_stage_chain(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                         int dtype, size_t batch, size_t n, int first, int last, void *stream) {
+    const size_t shape[4] = {batch, 4, 2 * n - 1, n};
+    if (n == 0 || !adrt_hip_valid_output_shape(shape)) return ADRT_HIP_ERR_SHAPE;
+    const int L = num_iters(n);
+    if (first < 0 || last > L || first >= last) return ADRT_HIP_ERR_STEP;
+    if (!dtype_ok(dtype)) return ADRT_HIP_ERR_DTYPE;
+    if (!in || !out) return ADRT_HIP_ERR_NULL;
+    const int steps = last - first;
+    const size_t need = steps > 1 ? adrt_hip_stage_chain_workspace(dtype, batch, n) : 0;
+    if (workspace_bytes < need || (need && !workspace)) return ADRT_HIP_ERR_WORKSPACE;
+    // ping-pong so that the last step writes `out`
+    void *bufs[2] = {steps % 2 == 1 ? out : workspace, steps % 2 == 1 ? workspace : out};
+    const void *src = in;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int k = 0; k < steps; ++k) {
+        const int st = launch_adrt_step(src, bufs[k & 1], dtype, batch, n, first + k, s);
+        if (st != ADRT_HIP_OK) return st;
+        src = bufs[k & 1];
+    }
+    return ADRT_HIP_OK;
+}
+
 int adrt_hip_set_config(int op, size_t n, int variant, size_t group_bytes) {
     return set_fused_config(op, n, variant, group_bytes);
 }

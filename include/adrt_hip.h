@@ -142,6 +142,18 @@ int adrt_hip_fmg_prolongation(const void *in, void *out, int dtype, size_t batch
 int adrt_hip_fmg_highpass(const void *in, void *out, int dtype, size_t batch, size_t h,
                           size_t w, void *stream);
 
+/* Forward stages first, first+1, ..., last-1 applied in turn to an
+ * ADRT-layout array (batch, 4, 2n-1, n): the same operations as that many
+ * adrt_hip_adrt_step calls (adrt_cdefs_adrt.hpp:214-258), without the host
+ * round trips.  0 <= first < last <= num_iters(n).  The whole chain
+ * (first = 0, last = num_iters(n)) is the transpose of adrt_hip_bdrt on that
+ * layout, since bdrt_step(k) is the transpose of adrt_step(L-1-k)
+ * (tests/test_bdrt_step.py:76-92): the back-projection's adjoint.  Needs
+ * adrt_hip_stage_chain_workspace() bytes when last - first > 1. */
+size_t adrt_hip_stage_chain_workspace(int dtype, size_t batch, size_t n);
+int adrt_hip_stage_chain(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                         int dtype, size_t batch, size_t n, int first, int last, void *stream);
+
 /* ---- layout helpers (any element size: 1, 2, 4 or 8 bytes) --------------- */
 
 /* adrt.core.adrt_init (core.py:123-176): (batch, n, n) -> (batch, 4, 2n-1, n),

@@ -136,3 +136,56 @@ class TestStepCompute:
     def test_iter_refuses_int32(self, cuda):
         with pytest.raises(TypeError, match="int32"):
             list(adrt_amd.core.adrt_iter(np.ones((16, 16), dtype=np.int32)))
+
+
+@pytest.mark.gpu
+class TestStageChain:
+    """adrt_hip_stage_chain: forward stages [first, last) in one call, equal
+    to that many adrt_step calls (and to the oracle's steps)."""
+
+    @pytest.mark.parametrize("n", [1, 2, 8, 64, 256])
+    @pytest.mark.parametrize("dt", ["float32", "float64"])
+    def test_full_chain_matches_steps(self, cuda, n, dt):
+        import torch
+        from adrt_amd import _kernels
+        from oracle import oracle as O
+
+        a = np.random.default_rng(n).standard_normal((2, 4, 2 * n - 1, n)).astype(dt)
+        L = O.num_iters(n)
+        want = a
+        for k in range(L):
+            want = O.adrt_step(want, k)
+        t = torch.from_numpy(a).to(cuda)
+        if L == 0:
+            return
+        got = _kernels.stage_chain(t)
+        assert got.cpu().numpy().tobytes() == np.ascontiguousarray(want).tobytes()
+
+    def test_partial_ranges(self, cuda):
+        import torch
+        from adrt_amd import _kernels
+
+        a = torch.from_numpy(np.random.default_rng(1).standard_normal((3, 4, 127, 64))).to(cuda)
+        for first, last in [(0, 1), (2, 3), (1, 5), (3, 6)]:
+            want = a
+            for k in range(first, last):
+                want = _kernels.adrt_step(want, k)
+            assert torch.equal(_kernels.stage_chain(a, first, last), want)
+
+    def test_errors(self, cuda):
+        import torch
+        from adrt_amd import _lib
+
+        lib = _lib.load()
+        a = torch.zeros((1, 4, 15, 8), device=cuda)
+        out = torch.empty_like(a)
+        ws = torch.empty(lib.adrt_hip_stage_chain_workspace(_lib.F32, 1, 8), dtype=torch.uint8,
+                         device=cuda)
+        call = lambda f, l, w=ws: lib.adrt_hip_stage_chain(  # noqa: E731
+            a.data_ptr(), out.data_ptr(), w.data_ptr(), w.numel(), _lib.F32, 1, 8, f, l,
+            torch.cuda.current_stream().cuda_stream)
+        assert call(0, 4) == _lib.ERR_STEP and call(2, 2) == _lib.ERR_STEP
+        assert call(-1, 2) == _lib.ERR_STEP
+        small = torch.empty(16, dtype=torch.uint8, device=cuda)
+        assert call(0, 3, small) == _lib.ERR_WORKSPACE
+        assert call(1, 2, small) == _lib.OK  # one step needs no workspace
