@@ -86,8 +86,8 @@ def bdrt(y):
 
 
 def _bdot(u, v):
-    """Per-image float64 inner product of two (B, ...) tensors."""
-    return (u.double() * v.double()).reshape(u.shape[0], -1).sum(dim=1)
+    """Per-image float64 inner product of two (B, ...) tensors (device kernel)."""
+    return _kernels.batched_dot(u.contiguous(), v.contiguous())
 
 
 def _apply_adjoint(x, r):

@@ -99,6 +99,37 @@ UNEVEN_SIDES = (32, 64, 128, 256, 512, 1024)
 LEARNED_WEIGHTS = os.path.join(ROOT, "adrt_amd", "weights", "learned_256.npz")
 
 
+def lsq_leg(iters=200):
+    """C4: least squares min 0.5 ||adrt(x) - b||^2 through autograd for 32
+    float32 images of 512x512 from x = 0, b = adrt(phantom) + N(0, (0.01
+    max|b|)^2), ``iters`` CGLS steps whose A^T is the autograd backward
+    (the fused back-projection).  Each step applies A once and A^T once."""
+    import torch
+
+    from adrt_amd import autograd, learned
+
+    x_true = learned.phantom_batch(512, 4, 0, 32)
+    a = learned._kernels.adrt(x_true)
+    g = torch.Generator(device=a.device)
+    g.manual_seed(4)
+    b = a + 0.01 * a.abs().amax() * torch.randn(a.shape, generator=g, device=a.device)
+    autograd.lsq_cg(b, iters=2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, hist = autograd.lsq_cg(b, iters=iters)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    per_iter = bytes_fwd(512, 4) * 32 * 2  # A and A^T, compulsory bytes each
+    return {
+        "workload": f"CGLS through autograd, 32 x 512x512 float32, {iters} steps, x0 = 0",
+        "seconds": round(secs, 4),
+        "ms_per_step": round(secs / iters * 1e3, 3),
+        "loss_first": float(hist[0].mean()), "loss_last": float(hist[-1].mean()),
+        "rel_error_vs_phantom": float(((x - x_true).norm() / x_true.norm())),
+        "GBps_algorithmic": round(per_iter / (secs / iters) / 1e9, 1),
+    }
+
+
 def learned_cg_leg(count):
     """Second half of the metric: wall time to relative residual 1e-4 of CG
     on the normal equations for ``count`` held-out 256x256 float32 phantoms
@@ -412,6 +443,7 @@ def main():
             extras[name] = {"ms_per_call": round(ms, 3), "imgs_per_s": round(batch / ms * 1e3, 2),
                             "GBps_algorithmic": round(gbs, 1)}
         extras["uneven"] = uneven_extra(max(3, args.steps // 4), 2)
+        extras["lsq_autograd"] = lsq_leg()
         result["extras"] = extras
     if not args.no_learned:
         del out, ws
