@@ -213,3 +213,17 @@ def press_fmg_highpass(a, /):
         raise ValueError("array is too small to high-pass filter")
     _dtype_code(arr)
     return _compute(arr, _kernels.fmg_highpass)
+
+
+def interp_to_cart(a, /):
+    """adrt_cdefs_py.cpp:621-682 -> adrt_hip_interp_to_cart: the shape rule of
+    bdrt plus n > 1 (adrt_cdefs_common.cpp:276-279), then the float index
+    limit (4n exact in float), then the dtype."""
+    arr, shape = _output_shape_op(a)
+    lib = _lib.load()
+    if not lib.adrt_hip_valid_output_shape(_lib.shape4(shape)) or shape[3] <= 1:
+        raise ValueError("array must have a valid ADRT output shape")
+    if shape[3] > (1 << 24) // 4:
+        raise ValueError("array is too big for interpolation index calculations")
+    _dtype_code(arr)
+    return _compute(arr, _kernels.interp_to_cart)

@@ -205,3 +205,35 @@ def cg_direction(p, z, beta):
     _lib.check(lib.adrt_hip_cg_direction(dev.ptr(p), dev.ptr(z), dev.ptr(beta), _code(p), b, m,
                                          dev.stream_ptr(p)), "cg_direction")
     return p
+
+
+_interp_tables = {}
+
+
+def interp_table(code, n, device):
+    """The interpolation table for (dtype, n) on ``device`` (computed on the
+    host by the library, uploaded once, cached)."""
+    key = (code, n, str(device))
+    tab = _interp_tables.get(key)
+    if tab is None:
+        import numpy as np
+
+        lib = _lib.load()
+        nbytes = int(lib.adrt_hip_interp_table_bytes(code, n))
+        host = np.zeros(nbytes, dtype=np.uint8)
+        _lib.check(lib.adrt_hip_interp_table(code, n, host.ctypes.data, nbytes), "interp_table")
+        tab = dev.torch().from_numpy(host).to(device)
+        _interp_tables[key] = tab
+    return tab
+
+
+def interp_to_cart(t):
+    """(..., 4, 2n-1, n) -> (..., n, 4n), nearest-neighbour interpolation."""
+    lib = _lib.load()
+    b, n = _batch_n(t, 3)
+    code = _code(t)
+    tab = interp_table(code, n, t.device)
+    out = dev.empty_like_dtype(t, (*t.shape[:-3], n, 4 * n))
+    _lib.check(lib.adrt_hip_interp_to_cart(dev.ptr(t), dev.ptr(out), dev.ptr(tab), tab.numel(),
+                                           code, b, n, dev.stream_ptr(t)), "interp_to_cart")
+    return out

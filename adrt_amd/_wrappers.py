@@ -55,6 +55,16 @@ def adrt(a, /):
     return _adrt_cdefs.adrt(_normalize_array(a))
 
 
+@_set_module("adrt_amd.utils")
+def interp_to_cart(a, /):
+    """Nearest-neighbour interpolation of an ADRT output onto a regular
+    Cartesian (theta, t) grid (drop-in for ``adrt.utils.interp_to_cart``,
+    _wrappers.py:328-360): ``(..., 4, 2N-1, N)`` float32/float64 with at
+    most one batch dimension -> ``(..., N, 4N)``, bit-identical to the
+    reference."""
+    return _adrt_cdefs.interp_to_cart(_normalize_array(a))
+
+
 @_set_module("adrt_amd.core")
 def adrt_step(a, /, step):
     """One forward ADRT step (drop-in for ``adrt.core.adrt_step``)."""

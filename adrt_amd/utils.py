@@ -1,16 +1,19 @@
-"""Utilities (counterpart of src/adrt/utils.py, hot-path part).
+"""Utilities (counterpart of src/adrt/utils.py).
 
 ``truncate`` (utils.py:191-242) runs as a gfx950 copy kernel for every
 element size of 1, 2, 4 or 8 bytes, on NumPy arrays (host round trip) and
 torch tensors, with any number of leading dimensions and any base size N.
+``interp_to_cart`` (_wrappers.py:328-360) is the nearest-neighbour
+resampling kernel.
 """
 
 import numpy as np
 
 from . import _device as dev
 from . import _kernels
+from ._wrappers import interp_to_cart
 
-__all__ = ["truncate"]
+__all__ = ["truncate", "interp_to_cart"]
 
 
 def _check_copyable(a):

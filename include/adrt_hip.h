@@ -154,6 +154,21 @@ size_t adrt_hip_stage_chain_workspace(int dtype, size_t batch, size_t n);
 int adrt_hip_stage_chain(const void *in, void *out, void *workspace, size_t workspace_bytes,
                          int dtype, size_t batch, size_t n, int first, int last, void *stream);
 
+/* adrt.utils.interp_to_cart: nearest-neighbour interpolation of an ADRT
+ * output (batch, 4, 2n-1, n) onto the regular grid (batch, n, 4n).
+ * Replaces adrt._adrt_cdefs.interp_to_cart (adrt_cdefs_py.cpp:621-682 ->
+ * interp_adrtcart<T, float>, adrt_cdefs_interp_adrtcart.hpp:60-114).  The
+ * angle- and offset-dependent quantities are computed on the host by
+ * adrt_hip_interp_table into a caller buffer of adrt_hip_interp_table_bytes
+ * (a plain CPU function, same float math as the reference); the caller
+ * copies that table to the device once per (dtype, n) and passes it to
+ * adrt_hip_interp_to_cart, which launches the per-element kernel.  n must
+ * be a power of two > 1 and 4n exact in float (else ADRT_HIP_ERR_TOO_BIG). */
+size_t adrt_hip_interp_table_bytes(int dtype, size_t n);
+int adrt_hip_interp_table(int dtype, size_t n, void *host_buf, size_t bytes);
+int adrt_hip_interp_to_cart(const void *in, void *out, const void *table, size_t table_bytes,
+                            int dtype, size_t batch, size_t n, void *stream);
+
 /* ---- layout helpers (any element size: 1, 2, 4 or 8 bytes) --------------- */
 
 /* adrt.core.adrt_init (core.py:123-176): (batch, n, n) -> (batch, 4, 2n-1, n),

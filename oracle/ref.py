@@ -31,7 +31,7 @@ def _load():
         sz = ctypes.c_size_t
         vp = ctypes.c_void_p
         for suf in ("f32", "f64"):
-            for name in ("adrt", "bdrt", "iadrt", "fmg_restriction"):
+            for name in ("adrt", "bdrt", "iadrt", "fmg_restriction", "interp"):
                 fn = getattr(lib, f"ref_{name}_{suf}")
                 fn.argtypes = [vp, sz, sz, vp]
                 fn.restype = ctypes.c_int
@@ -103,3 +103,10 @@ def fmg_highpass(a):
 
 def openmp_enabled():
     return bool(_load().ref_openmp_enabled())
+
+
+def interp_to_cart(a):
+    """The reference's interp_adrtcart (float index math): (B?, 4, 2n-1, n)
+    -> (B?, n, 4n)."""
+    n = a.shape[-1]
+    return _run("interp", a, (*a.shape[:-3], n, 4 * n), _batch(a, 3), n)

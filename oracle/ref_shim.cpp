@@ -3,7 +3,8 @@
 // This file holds no algorithm: it includes the reference's headers from the
 // reference checkout (src/adrt/adrt_cdefs_*.hpp, compiled where they lie by
 // oracle/Makefile) and exposes adrt_basic / adrt_step / bdrt_basic /
-// bdrt_step / iadrt_basic / fmg_* through plain C entry points so the oracle
+// bdrt_step / iadrt_basic / fmg_* / interp_adrtcart (float index math, as
+// the reference's binding instantiates it) through plain C entry points so the oracle
 // can be checked against the reference itself and bench.py can time the
 // reference's OpenMP CPU path.  The built library lands in oracle/_ref/.
 #include <array>
@@ -15,6 +16,7 @@
 #include "adrt_cdefs_bdrt.hpp"
 #include "adrt_cdefs_iadrt.hpp"
 #include "adrt_cdefs_fmg.hpp"
+#include "adrt_cdefs_interp_adrtcart.hpp"
 
 namespace {
 
@@ -115,6 +117,17 @@ int ref_fmg_highpass_f32(const float *in, std::size_t B, std::size_t H, std::siz
 int ref_fmg_highpass_f64(const double *in, std::size_t B, std::size_t H, std::size_t W, double *out) {
     const std::array<std::size_t, 3> shape{B, H, W};
     adrt::fmg_highpass<double>(in, shape, out);
+    return 0;
+}
+
+int ref_interp_f32(const float *in, std::size_t B, std::size_t N, float *out) {
+    const std::array<std::size_t, 4> shape{B, 4, 2 * N - 1, N};
+    adrt::interp_adrtcart<float, float>(in, shape, out);
+    return 0;
+}
+int ref_interp_f64(const double *in, std::size_t B, std::size_t N, double *out) {
+    const std::array<std::size_t, 4> shape{B, 4, 2 * N - 1, N};
+    adrt::interp_adrtcart<double, float>(in, shape, out);
     return 0;
 }
 

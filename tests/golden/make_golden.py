@@ -7,7 +7,8 @@ reference sources (``make -C oracle``):
 
 Outputs (committed):
   golden_small.npz   seeded inputs and the reference's outputs for n <= 16
-                     (adrt, adrt_step, bdrt, bdrt_step, iadrt, fmg_*), f32/f64
+                     (adrt, adrt_step, bdrt, bdrt_step, iadrt, fmg_*,
+                     interp_to_cart), f32/f64
   golden_hashes.json SHA-256 of the reference's outputs on larger seeded
                      inputs (values canonicalised: -0.0 -> +0.0)
 The reference itself never enters the repository; only these vectors do.
@@ -56,6 +57,8 @@ def small_vectors():
             out[f"{key}_hp_in"] = hp
             out[f"{key}_highpass"] = ref.fmg_highpass(hp)
             out[f"{key}_prolongation"] = ref.fmg_prolongation(hp)
+            if n >= 2:
+                out[f"{key}_interp"] = ref.interp_to_cart(dat)
     return out
 
 
@@ -71,6 +74,8 @@ def hash_vectors():
     res["iadrt_f32_of_adrt_8x128_seed0"] = canonical_sha256(ref.iadrt(y))
     big = np.random.default_rng(1).random((1, 1024, 1024), dtype=np.float32)
     res["adrt_f32_1x1024_seed1"] = canonical_sha256(ref.adrt(big))
+    res["interp_f32_of_adrt_8x128_seed0"] = canonical_sha256(ref.interp_to_cart(y))
+    res["interp_f64_of_adrt_2x256_seed0"] = canonical_sha256(ref.interp_to_cart(ref.adrt(x64)))
     return res
 
 
