@@ -63,6 +63,8 @@ SIGNATURES = {
     "adrt_hip_adrt_init": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_truncate": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_truncate_combine": (_INT, [_VP, _VP, _INT, _SZ, _SZ, ctypes.c_double, _INT, _VP]),
+    "adrt_hip_stitch": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _INT, _VP]),
+    "adrt_hip_unstitch": (_INT, [_VP, _VP, _SZ, _SZ, _SZ, _INT, _VP]),
     "adrt_hip_interp_table_bytes": (_SZ, [_INT, _SZ]),
     "adrt_hip_interp_table": (_INT, [_INT, _SZ, _VP, _SZ]),
     "adrt_hip_interp_to_cart": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),

@@ -271,6 +271,34 @@ int adrt_hip_stage_chain(const void *in, void *out, void *workspace, size_t work
     return ADRT_HIP_OK;
 }
 
+static int stitch_args(size_t elem, size_t outer, size_t n, int flag, const void *in,
+                       const void *out) {
+    if (outer == 0 || n == 0 || (flag != 0 && flag != 1) || (flag == 1 && n < 2))
+        return ADRT_HIP_ERR_SHAPE;
+    if (elem != 1 && elem != 2 && elem != 4 && elem != 8) return ADRT_HIP_ERR_DTYPE;
+    size_t total;
+    const size_t dims[4] = {outer, 3 * n, 4, n};
+    if (n > (SIZE_MAX >> 4) || !product_ok(dims, 4, &total)) return ADRT_HIP_ERR_TOO_BIG;
+    if (!in || !out) return ADRT_HIP_ERR_NULL;
+    return ADRT_HIP_OK;
+}
+
+int adrt_hip_stitch(const void *in, void *out, size_t elem_bytes, size_t outer, size_t n,
+                    int remove_repeated, void *stream) {
+    const int st = stitch_args(elem_bytes, outer, n, remove_repeated, in, out);
+    if (st != ADRT_HIP_OK) return st;
+    return launch_stitch(in, out, elem_bytes, outer, n, remove_repeated ? n - 1 : n, false,
+                         static_cast<hipStream_t>(stream));
+}
+
+int adrt_hip_unstitch(const void *in, void *out, size_t elem_bytes, size_t outer, size_t n,
+                      int removed_repeated, void *stream) {
+    const int st = stitch_args(elem_bytes, outer, n, removed_repeated, in, out);
+    if (st != ADRT_HIP_OK) return st;
+    return launch_stitch(in, out, elem_bytes, outer, n, removed_repeated ? n - 1 : n, true,
+                         static_cast<hipStream_t>(stream));
+}
+
 int adrt_hip_set_config(int op, size_t n, int variant, size_t group_bytes) {
     return set_fused_config(op, n, variant, group_bytes);
 }

@@ -102,6 +102,8 @@ int launch_fmg_highpass(const void *in, void *out, int dtype, size_t B, size_t H
 int launch_adrt_init(const void *in, void *out, size_t elem, size_t B, size_t N, hipStream_t s);
 int launch_truncate(const void *in, void *out, size_t elem, size_t outer, size_t N,
                     hipStream_t s);
+int launch_stitch(const void *in, void *out, size_t elem, size_t outer, size_t n, size_t vc,
+                  bool inverse, hipStream_t s);
 int launch_truncate_combine(const void *in, void *out, int dtype, size_t B, size_t N,
                             double divisor, int mode, hipStream_t s);
 }  // namespace adrt_hip

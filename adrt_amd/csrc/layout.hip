@@ -129,6 +129,69 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// stitch_adrt (utils.py:65-130): out (outer, 3n-2, 4 * vc), vc = n or n - 1.
+// Quadrant i occupies column block i; quadrants 0, 1 fill rows [0, 2n-1),
+// quadrants 2, 3 rows [n-1, 3n-2); odd quadrants are flipped in both axes;
+// everything else is zero.
+template <typename E>
+__global__ void __launch_bounds__(kBlock)
+    stitch_kernel(const E *__restrict__ in, E *__restrict__ out, size_t total, size_t n,
+                  size_t vc) {
+    const size_t R = 2 * n - 1, rows = 3 * n - 2, cols = 4 * vc;
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += size_t(gridDim.x) * blockDim.x) {
+        const size_t col = idx % cols;
+        const size_t r = (idx / cols) % rows;
+        const size_t o = idx / (cols * rows);
+        const size_t i = col / vc, c = col % vc;
+        const size_t lo = i < 2 ? 0 : n - 1;
+        E v = E(0);
+        if (r >= lo && r < lo + R) {
+            size_t rr = r - lo, cc = c;
+            if (i % 2) {
+                rr = R - 1 - rr;
+                cc = n - 1 - cc;
+            }
+            v = in[((o * 4 + i) * R + rr) * n + cc];
+        }
+        out[idx] = v;
+    }
+}
+
+// unstitch_adrt (utils.py:133-188): the inverse, out (outer, 4, 2n-1, n)
+// from (outer, 3n-2, 4 * vc).  With vc = n - 1 the last column of quadrant
+// q (before its flip) is column 0 of quadrant q+1 (mod 4) on the same
+// stitched row, row-reversed for q = 3.
+template <typename E>
+__global__ void __launch_bounds__(kBlock)
+    unstitch_kernel(const E *__restrict__ in, E *__restrict__ out, size_t total, size_t n,
+                    size_t vc) {
+    const size_t R = 2 * n - 1, rows = 3 * n - 2, cols = 4 * vc;
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+         idx += size_t(gridDim.x) * blockDim.x) {
+        const size_t c = idx % n;
+        const size_t r = (idx / n) % R;
+        const size_t q = (idx / (n * R)) % 4;
+        const size_t o = idx / (n * R * 4);
+        size_t rr = r, cc = c;
+        if (q % 2) {
+            rr = R - 1 - rr;
+            cc = n - 1 - cc;
+        }
+        const size_t srow = rr + (q < 2 ? 0 : n - 1);
+        const E *img = in + o * rows * cols;
+        E v;
+        if (cc < vc) {
+            v = img[srow * cols + q * vc + cc];
+        } else if (q == 3) {
+            v = img[(rows - 1 - srow) * cols];
+        } else {
+            v = img[srow * cols + (q + 1) * vc];
+        }
+        out[idx] = v;
+    }
+}
+
 template <typename E>
 static void init_typed(const void *in, void *out, size_t B, size_t n, hipStream_t s) {
     const size_t total = B * 4 * (2 * n - 1) * n;
@@ -190,6 +253,31 @@ int launch_truncate_combine(const void *in, void *out, int dtype, size_t B, size
         hipLaunchKernelGGL(truncate_combine_kernel<double>, dim3(g), dim3(kBlock), 0, s,
                            static_cast<const double *>(in), static_cast<double *>(out), total, n,
                            divisor, divide, mode);
+    return launch_status();
+}
+
+template <typename E>
+static void stitch_typed(const void *in, void *out, size_t outer, size_t n, size_t vc,
+                         bool inverse, hipStream_t s) {
+    if (!inverse) {
+        const size_t total = outer * (3 * n - 2) * 4 * vc;
+        hipLaunchKernelGGL(stitch_kernel<E>, dim3(grid_for(total)), dim3(kBlock), 0, s,
+                           static_cast<const E *>(in), static_cast<E *>(out), total, n, vc);
+    } else {
+        const size_t total = outer * 4 * (2 * n - 1) * n;
+        hipLaunchKernelGGL(unstitch_kernel<E>, dim3(grid_for(total)), dim3(kBlock), 0, s,
+                           static_cast<const E *>(in), static_cast<E *>(out), total, n, vc);
+    }
+}
+
+int launch_stitch(const void *in, void *out, size_t elem, size_t outer, size_t n, size_t vc,
+                  bool inverse, hipStream_t s) {
+    switch (elem) {
+        case 1: stitch_typed<uint8_t>(in, out, outer, n, vc, inverse, s); break;
+        case 2: stitch_typed<uint16_t>(in, out, outer, n, vc, inverse, s); break;
+        case 4: stitch_typed<uint32_t>(in, out, outer, n, vc, inverse, s); break;
+        default: stitch_typed<uint64_t>(in, out, outer, n, vc, inverse, s); break;
+    }
     return launch_status();
 }
 

@@ -180,6 +180,20 @@ int adrt_hip_adrt_init(const void *in, void *out, size_t elem_bytes, size_t batc
 int adrt_hip_truncate(const void *in, void *out, size_t elem_bytes, size_t outer, size_t n,
                       void *stream);
 
+/* adrt.utils.stitch_adrt (utils.py:65-130): (outer, 4, 2n-1, n) ->
+ * (outer, 3n-2, 4 * vc), vc = n (remove_repeated 0) or n - 1 (1, n >= 2);
+ * any n >= 1.  Quadrants side by side, odd ones flipped in both axes, zero
+ * fill elsewhere. */
+int adrt_hip_stitch(const void *in, void *out, size_t elem_bytes, size_t outer, size_t n,
+                    int remove_repeated, void *stream);
+
+/* adrt.utils.unstitch_adrt (utils.py:133-188): the inverse,
+ * (outer, 3n-2, 4 * vc) -> (outer, 4, 2n-1, n); with removed_repeated 1
+ * (vc = n - 1) the dropped columns are restored from the neighbouring
+ * quadrant as the reference does. */
+int adrt_hip_unstitch(const void *in, void *out, size_t elem_bytes, size_t outer, size_t n,
+                      int removed_repeated, void *stream);
+
 /* ---- fused helpers for the solvers (no reference counterpart) ------------ */
 
 /* out[b] = combine_q(truncate(in)[b, q] / divisor), in (batch, 4, 2n-1, n),

@@ -100,3 +100,34 @@ class TestInterpToCart:
         out = adrt_amd.utils.interp_to_cart(np.ones((4, 2 * n - 1, n), dtype=np.float64))
         nz = out[out != 0]
         assert nz.size > 0 and np.all(nz >= 1.0) and np.all(nz <= np.sqrt(2) + 1e-15)
+
+
+def _cell_centres(a, b, n):
+    grid, step = np.linspace(a, b, num=n, endpoint=False, retstep=True)
+    return grid + 0.5 * step
+
+
+def model_interp(adrt_out):
+    """interp_to_cart through the host coordinate map (the reference's test
+    model): cell-centred theta and t grids, nearest ADRT entry, scale factor,
+    zero where the height falls outside the output."""
+    n = adrt_out.shape[-1]
+    res = np.zeros((*adrt_out.shape[:-3], n, 4 * n))
+    ang, off = np.meshgrid(_cell_centres(-0.5 * np.pi, 0.5 * np.pi, 4 * n),
+                           _cell_centres(np.sqrt(2) / 2, -np.sqrt(2) / 2, n))
+    q, h, s, f = adrt_amd.utils.coord_cart_to_adrt(ang, off, n)
+    ok = (h > -1) & (h < 2 * n - 1)
+    res[..., ok] = f[ok] * adrt_out[..., q[ok], h[ok], s[ok]]
+    return res
+
+
+@gpu
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_matches_coordinate_model(cuda, dtype):
+    a = np.arange(48, dtype=dtype).reshape((3, 4, 4))
+    out = adrt_amd.adrt(a)
+    native = adrt_amd.utils.interp_to_cart(out)
+    assert native.dtype == a.dtype and native.shape == (3, 4, 16)
+    assert np.allclose(model_interp(out), native)
+    for i in range(3):
+        assert np.array_equal(adrt_amd.utils.interp_to_cart(out[i]), native[i])
