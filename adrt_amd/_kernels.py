@@ -207,6 +207,54 @@ def cg_direction(p, z, beta):
     return p
 
 
+class PcgState:
+    """The per-image device state of a batched CG solve (the layout of
+    adrt_hip_pcg_state_bytes): float64 fields rz, pq, rr, tol, alpha, beta,
+    int32 fields active, next, use_p, iters and the int32 active ``count``,
+    each a view into one byte buffer."""
+
+    F64 = ("rz", "pq", "rr", "tol", "alpha", "beta")
+    I32 = ("active", "next", "use_p", "iters")
+
+    def __init__(self, batch, device):
+        torch = dev.torch()
+        self.batch = batch
+        nbytes = int(_lib.load().adrt_hip_pcg_state_bytes(batch))
+        self.buf = torch.zeros(nbytes, dtype=torch.uint8, device=device)
+        for i, k in enumerate(self.F64):
+            setattr(self, k, self.buf[8 * batch * i:8 * batch * (i + 1)].view(torch.float64))
+        base = 8 * batch * len(self.F64)
+        for j, k in enumerate(self.I32):
+            setattr(self, k, self.buf[base + 4 * batch * j:base + 4 * batch * (j + 1)]
+                    .view(torch.int32))
+        off = base + 4 * batch * len(self.I32)
+        self.count = self.buf[off:off + 4].view(torch.int32)
+
+    def select(self, keep):
+        """A new state holding the images ``keep`` (a device index tensor)."""
+        out = PcgState(int(keep.numel()), self.buf.device)
+        for k in self.F64 + self.I32:
+            getattr(out, k).copy_(getattr(self, k)[keep])
+        return out
+
+
+def pcg_update(x, r, p, q, state, ws):
+    """One CG step's update on the device (adrt_hip_pcg_update)."""
+    b, m = _bm(x)
+    _lib.check(_lib.load().adrt_hip_pcg_update(
+        dev.ptr(x), dev.ptr(r), dev.ptr(p), dev.ptr(q), dev.ptr(state.buf), dev.ptr(ws),
+        ws.numel(), _code(x), b, m, dev.stream_ptr(x)), "pcg_update")
+
+
+def pcg_direction(p, r, z, state, ws):
+    """The next search direction on the device (adrt_hip_pcg_direction);
+    z None for plain CG."""
+    b, m = _bm(p)
+    _lib.check(_lib.load().adrt_hip_pcg_direction(
+        dev.ptr(p), dev.ptr(r), None if z is None else dev.ptr(z), dev.ptr(state.buf),
+        dev.ptr(ws), ws.numel(), _code(p), b, m, dev.stream_ptr(p)), "pcg_direction")
+
+
 _interp_tables = {}
 
 

@@ -74,6 +74,9 @@ SIGNATURES = {
     "adrt_hip_batched_dot": (_INT, [_VP, _VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_cg_update": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_cg_direction": (_INT, [_VP, _VP, _VP, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_pcg_state_bytes": (_SZ, [_SZ]),
+    "adrt_hip_pcg_update": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_pcg_direction": (_INT, [_VP, _VP, _VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_set_config": (_INT, [_INT, _SZ, _INT, _SZ]),
     "adrt_hip_get_config": (_INT, [_INT, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_SZ)]),
     "adrt_hip_nn_conv3x3": (_INT, [_VP, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _INT, _INT,

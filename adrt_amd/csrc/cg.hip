@@ -10,6 +10,19 @@
 // update is the same two roundings as the unfused expression x + a * p.
 // Sums are deterministic: each block reduces a fixed chunk in a fixed order
 // and a second pass adds the chunks of an image in order.
+//
+// The pcg_* entry points run a whole (preconditioned) CG iteration's scalar
+// logic on the device, over a per-image state buffer, so that one iteration
+// is the operator, the preconditioner and four small launch pairs with no
+// host round trip:
+//   pcg_update:    pq = p.q; alpha = rz / pq (0 for inactive images or
+//                  pq <= 0); x += alpha p; r -= alpha q; rr = r.r;
+//                  iters += active; next = active && sqrt(rr) > tol
+//   pcg_direction: rz' = r.z (preconditioned images) or rr; an image whose
+//                  preconditioned step fails (rz' <= 0 or pq <= 0) drops the
+//                  preconditioner and restarts from p = r; otherwise
+//                  beta = rz' / rz, p = z + beta p; active = next;
+//                  count = number of active images
 #include "common.hpp"
 
 namespace adrt_hip {
@@ -114,6 +127,108 @@ inline Split split(size_t B, size_t m) {
     return s;
 }
 
+// Per-image CG state, pcg_state_bytes(B) bytes:
+//   double rz[B], pq[B], rr[B], tol[B], alpha[B], beta[B]
+//   int32  active[B], next[B], use_p[B], iters[B], count
+struct PcgState {
+    double *rz, *pq, *rr, *tol, *alpha, *beta;
+    int *active, *next, *use_p, *iters, *count;
+};
+
+inline size_t pcg_bytes(size_t B) { return 64 * B + 8; }
+
+inline PcgState pcg_state(void *base, size_t B) {
+    double *d = static_cast<double *>(base);
+    int *i = reinterpret_cast<int *>(d + 6 * B);
+    PcgState s;
+    s.rz = d;
+    s.pq = d + B;
+    s.rr = d + 2 * B;
+    s.tol = d + 3 * B;
+    s.alpha = d + 4 * B;
+    s.beta = d + 5 * B;
+    s.active = i;
+    s.next = i + B;
+    s.use_p = i + 2 * B;
+    s.iters = i + 3 * B;
+    s.count = i + 4 * B;
+    return s;
+}
+
+__device__ __forceinline__ double row_sum(const double *partial, size_t b, int chunks) {
+    double v = 0.0;
+    for (int c = 0; c < chunks; ++c) v += partial[b * chunks + c];
+    return v;
+}
+
+// pq and the step length of each image
+__global__ void __launch_bounds__(kThreads)
+    pq_rows_kernel(const double *__restrict__ partial, PcgState s, size_t B, int chunks) {
+    const size_t b = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= B) return;
+    const double pq = row_sum(partial, b, chunks);
+    s.pq[b] = pq;
+    s.alpha[b] = (s.active[b] && pq > 0.0) ? s.rz[b] / pq : 0.0;
+}
+
+// ||r||^2, the iteration count and the convergence test; clears the count
+// that pcg_direction accumulates
+__global__ void __launch_bounds__(kThreads)
+    rr_rows_kernel(const double *__restrict__ partial, PcgState s, size_t B, int chunks) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = 0;
+    const size_t b = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= B) return;
+    const double rr = row_sum(partial, b, chunks);
+    s.rr[b] = rr;
+    const int act = s.active[b];
+    s.iters[b] += act;
+    s.next[b] = (act && sqrt(rr) > s.tol[b]) ? 1 : 0;
+}
+
+// beta, the preconditioner's breakdown test and the new active set
+// (partial == nullptr: no preconditioner, rz' = rr)
+__global__ void __launch_bounds__(kThreads)
+    rz_rows_kernel(const double *__restrict__ partial, PcgState s, size_t B, int chunks) {
+    const size_t b = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= B) return;
+    const double rr = s.rr[b], rz = s.rz[b];
+    const double denom = rz != 0.0 ? rz : 1.0;
+    double rzn = rr, beta = rr / denom;
+    if (partial != nullptr && s.use_p[b]) {
+        const double d = row_sum(partial, b, chunks);
+        if (s.next[b] && (d <= 0.0 || s.pq[b] <= 0.0)) {
+            s.use_p[b] = 0;  // restart as plain CG from p = r
+            rzn = rr;
+            beta = 0.0;
+        } else {
+            rzn = d;
+            beta = d / denom;
+        }
+    }
+    s.rz[b] = rzn;
+    s.beta[b] = beta;
+    const int nxt = s.next[b];
+    s.active[b] = nxt;
+    if (nxt) atomicAdd(s.count, 1);
+}
+
+// p = (use_p ? z : r) + beta p
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    pcg_dir_kernel(T *__restrict__ p, const T *__restrict__ r, const T *__restrict__ z,
+                   const PcgState s, size_t m, size_t chunk) {
+    const size_t b = blockIdx.y;
+    const size_t lo = size_t(blockIdx.x) * chunk;
+    const size_t hi = lo + chunk < m ? lo + chunk : m;
+    const T c = T(s.beta[b]);
+    T *pb = p + b * m;
+    const T *zb = (z != nullptr && s.use_p[b]) ? z + b * m : r + b * m;
+    for (size_t i = lo + threadIdx.x; i < hi; i += kThreads) {
+        const T cp = c * pb[i];
+        pb[i] = zb[i] + cp;
+    }
+}
+
 }  // namespace cgk
 
 using namespace cgk;
@@ -192,6 +307,83 @@ int adrt_hip_cg_direction(void *p, const void *z, const double *beta, int dtype,
         hipLaunchKernelGGL(direction_kernel<double>, grid, dim3(kThreads), 0, s,
                            static_cast<double *>(p), static_cast<const double *>(z), beta, m,
                            sp.chunk);
+    return launch_status();
+}
+
+size_t adrt_hip_pcg_state_bytes(size_t batch) { return batch >= 1 ? pcg_bytes(batch) : 0; }
+
+int adrt_hip_pcg_update(void *x, void *r, const void *p, const void *q, void *state,
+                        void *workspace, size_t workspace_bytes, int dtype, size_t batch,
+                        size_t m, void *stream) {
+    if (!batch_ok(batch, m)) return ADRT_HIP_ERR_SHAPE;
+    if (dtype != ADRT_HIP_F32 && dtype != ADRT_HIP_F64) return ADRT_HIP_ERR_DTYPE;
+    if (!x || !r || !p || !q || !state || !workspace) return ADRT_HIP_ERR_NULL;
+    if (workspace_bytes < adrt_hip_cg_workspace(batch, m)) return ADRT_HIP_ERR_WORKSPACE;
+    const Split sp = split(batch, m);
+    const PcgState st = pcg_state(state, batch);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *part = static_cast<double *>(workspace);
+    const dim3 grid(unsigned(sp.chunks), unsigned(batch));
+    const dim3 rows(unsigned((batch + kThreads - 1) / kThreads));
+    if (dtype == ADRT_HIP_F32) {
+        hipLaunchKernelGGL(dot_kernel<float>, grid, dim3(kThreads), 0, s,
+                           static_cast<const float *>(p), static_cast<const float *>(q), part, m,
+                           sp.chunk, sp.chunks);
+        hipLaunchKernelGGL(pq_rows_kernel, rows, dim3(kThreads), 0, s, (const double *)part, st,
+                           batch, sp.chunks);
+        hipLaunchKernelGGL(update_kernel<float>, grid, dim3(kThreads), 0, s,
+                           static_cast<float *>(x), static_cast<float *>(r),
+                           static_cast<const float *>(p), static_cast<const float *>(q),
+                           (const double *)st.alpha, part, m, sp.chunk, sp.chunks);
+    } else {
+        hipLaunchKernelGGL(dot_kernel<double>, grid, dim3(kThreads), 0, s,
+                           static_cast<const double *>(p), static_cast<const double *>(q), part,
+                           m, sp.chunk, sp.chunks);
+        hipLaunchKernelGGL(pq_rows_kernel, rows, dim3(kThreads), 0, s, (const double *)part, st,
+                           batch, sp.chunks);
+        hipLaunchKernelGGL(update_kernel<double>, grid, dim3(kThreads), 0, s,
+                           static_cast<double *>(x), static_cast<double *>(r),
+                           static_cast<const double *>(p), static_cast<const double *>(q),
+                           (const double *)st.alpha, part, m, sp.chunk, sp.chunks);
+    }
+    hipLaunchKernelGGL(rr_rows_kernel, rows, dim3(kThreads), 0, s, (const double *)part, st,
+                       batch, sp.chunks);
+    return launch_status();
+}
+
+int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, void *workspace,
+                           size_t workspace_bytes, int dtype, size_t batch, size_t m,
+                           void *stream) {
+    if (!batch_ok(batch, m)) return ADRT_HIP_ERR_SHAPE;
+    if (dtype != ADRT_HIP_F32 && dtype != ADRT_HIP_F64) return ADRT_HIP_ERR_DTYPE;
+    if (!p || !r || !state || !workspace) return ADRT_HIP_ERR_NULL;
+    if (workspace_bytes < adrt_hip_cg_workspace(batch, m)) return ADRT_HIP_ERR_WORKSPACE;
+    const Split sp = split(batch, m);
+    const PcgState st = pcg_state(state, batch);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double *part = static_cast<double *>(workspace);
+    const dim3 grid(unsigned(sp.chunks), unsigned(batch));
+    const dim3 rows(unsigned((batch + kThreads - 1) / kThreads));
+    if (z != nullptr) {
+        if (dtype == ADRT_HIP_F32)
+            hipLaunchKernelGGL(dot_kernel<float>, grid, dim3(kThreads), 0, s,
+                               static_cast<const float *>(r), static_cast<const float *>(z), part,
+                               m, sp.chunk, sp.chunks);
+        else
+            hipLaunchKernelGGL(dot_kernel<double>, grid, dim3(kThreads), 0, s,
+                               static_cast<const double *>(r), static_cast<const double *>(z),
+                               part, m, sp.chunk, sp.chunks);
+    }
+    hipLaunchKernelGGL(rz_rows_kernel, rows, dim3(kThreads), 0, s,
+                       z != nullptr ? (const double *)part : nullptr, st, batch, sp.chunks);
+    if (dtype == ADRT_HIP_F32)
+        hipLaunchKernelGGL(pcg_dir_kernel<float>, grid, dim3(kThreads), 0, s,
+                           static_cast<float *>(p), static_cast<const float *>(r),
+                           static_cast<const float *>(z), st, m, sp.chunk);
+    else
+        hipLaunchKernelGGL(pcg_dir_kernel<double>, grid, dim3(kThreads), 0, s,
+                           static_cast<double *>(p), static_cast<const double *>(r),
+                           static_cast<const double *>(z), st, m, sp.chunk);
     return launch_status();
 }
 

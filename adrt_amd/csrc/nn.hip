@@ -542,40 +542,95 @@ __global__ void __launch_bounds__(kWgThreads)
 // D = 2x2 averaging, D^T = nearest upsampling / 4, h_l a 5x5 stencil with
 // zero padding (symmetric coefficients, so P is symmetric).
 // ---------------------------------------------------------------------------
+// One block per (output row of the whole batch, chunk of 256 columns):
+// row R = b * Hc + yq reads input rows 2R and 2R + 1 (H = 2 Hc).
 __global__ void __launch_bounds__(kWgThreads)
-    pyr_pool_kernel(const float *x, float *y, long long Pc, int H, int W) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= Pc) return;
-    const int Wc = W / 2, Hc = H / 2;
-    const int xq = int(q % Wc), yq = int((q / Wc) % Hc);
-    const long long b = q / ((long long)Wc * Hc);
-    const long long p = (b * H + 2 * yq) * W + 2 * xq;
-    y[q] = 0.25f * ((x[p] + x[p + 1]) + (x[p + W] + x[p + W + 1]));
+    pyr_pool_kernel(const float *__restrict__ x, float *__restrict__ y, int W, int chunks) {
+    const unsigned R = blockIdx.x / unsigned(chunks);
+    const int xq = int(blockIdx.x - R * unsigned(chunks)) * kWgThreads + int(threadIdx.x);
+    const int Wc = W / 2;
+    if (xq >= Wc) return;
+    const float *top = x + size_t(2 * R) * W + 2 * xq;
+    const float2 a = *reinterpret_cast<const float2 *>(top);
+    const float2 c = *reinterpret_cast<const float2 *>(top + W);
+    y[size_t(R) * Wc + xq] = 0.25f * ((a.x + a.y) + (c.x + c.y));
 }
 
-// y = h * x (5x5, zero padding) + coarse[parent] / 4 (coarse may be null).
+// y = h * x (5x5, zero padding) + coarse[parent] / 4 (coarse may be null),
+// one block per 64 x 32 tile of one image.  The tile and its 2-pixel halo
+// are staged in LDS (zeros outside the image); each thread computes a 4 x 2
+// block of outputs from a 8 x 6 window read as 128-bit rows.
+constexpr int kPyrTW = 64, kPyrTH = 32;
+constexpr int kPyrLW = kPyrTW + 4, kPyrLH = kPyrTH + 4;  // row pitch 68: 16-byte rows
+
 __global__ void __launch_bounds__(kWgThreads)
-    pyr_stencil_up_kernel(const float *x, const float *coarse, float *y, const float *coef,
-                          long long P, int H, int W) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const int xx = int(p % W), yy = int((p / W) % H);
-    const long long b = p / ((long long)W * H);
-    const float *img = x + b * (long long)H * W;
-    float s = 0.f;
+    pyr_stencil_up_kernel(const float *__restrict__ x, const float *__restrict__ coarse,
+                          float *__restrict__ y, const float *__restrict__ coef, int H, int W,
+                          int tiles_x, int tiles, int vec_store) {
+    __shared__ __align__(16) float t[kPyrLH][kPyrLW];
+    const unsigned b = blockIdx.x / unsigned(tiles);
+    const int tt = int(blockIdx.x - b * unsigned(tiles));
+    const int ty = tt / tiles_x, tx = tt - ty * tiles_x;
+    const int x0 = tx * kPyrTW, y0 = ty * kPyrTH;
+    const float *img = x + size_t(b) * H * W;
+    for (int i = threadIdx.x; i < kPyrLH * kPyrLW; i += kWgThreads) {
+        const int r = i / kPyrLW, c = i - r * kPyrLW;
+        const int gy = y0 - 2 + r, gx = x0 - 2 + c;
+        t[r][c] = (unsigned(gy) < unsigned(H) && unsigned(gx) < unsigned(W))
+                      ? img[size_t(gy) * W + gx]
+                      : 0.f;
+    }
+    float cf[25];
 #pragma unroll
-    for (int dy = -2; dy <= 2; ++dy) {
-        const int y2 = yy + dy;
-        if (unsigned(y2) >= unsigned(H)) continue;
+    for (int k = 0; k < 25; ++k) cf[k] = coef[k];
+    __syncthreads();
+    // thread -> columns 4j .. 4j+3, rows 2i, 2i+1 of the tile
+    const int j = threadIdx.x & 15, i = threadIdx.x >> 4;
+    const int cx = x0 + 4 * j, ry = y0 + 2 * i;
+    if (cx >= W || ry >= H) return;
+    float win[6][8];
 #pragma unroll
-        for (int dx = -2; dx <= 2; ++dx) {
-            const int x2 = xx + dx;
-            if (unsigned(x2) >= unsigned(W)) continue;
-            s += coef[(dy + 2) * 5 + (dx + 2)] * img[(long long)y2 * W + x2];
+    for (int r = 0; r < 6; ++r) {
+        const float4 a = *reinterpret_cast<const float4 *>(&t[2 * i + r][4 * j]);
+        const float4 c = *reinterpret_cast<const float4 *>(&t[2 * i + r][4 * j + 4]);
+        win[r][0] = a.x;
+        win[r][1] = a.y;
+        win[r][2] = a.z;
+        win[r][3] = a.w;
+        win[r][4] = c.x;
+        win[r][5] = c.y;
+        win[r][6] = c.z;
+        win[r][7] = c.w;
+    }
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy) {
+        const int yy = ry + oy;
+        if (yy >= H) break;
+        float o[4];
+#pragma unroll
+        for (int ox = 0; ox < 4; ++ox) {
+            float s = 0.f;
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 5; ++dx) s += cf[dy * 5 + dx] * win[oy + dy][ox + dx];
+            o[ox] = s;
+        }
+        if (coarse) {
+            const float *cr = coarse + (size_t(b) * (H / 2) + yy / 2) * (W / 2) + cx / 2;
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox)
+                if (cx + ox < W) o[ox] += 0.25f * cr[ox / 2];
+        }
+        float *dst = y + (size_t(b) * H + yy) * W + cx;
+        if (vec_store && cx + 4 <= W) {
+            *reinterpret_cast<float4 *>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox)
+                if (cx + ox < W) dst[ox] = o[ox];
         }
     }
-    if (coarse) s += 0.25f * coarse[(b * (H / 2) + yy / 2) * (W / 2) + xx / 2];
-    y[p] = s;
 }
 
 // partial[block][o] = sum over the block's pixels p of e[p] * w[p + o],
@@ -894,9 +949,12 @@ int adrt_hip_nn_adam(float *w, const float *g, float *m, float *v, void *w16, si
 int adrt_hip_pyr_pool(const float *x, float *y, int batch, int h, int w, void *stream) {
     if (batch < 1 || h < 2 || w < 2 || h % 2 || w % 2) return ADRT_HIP_ERR_ARG;
     if (!x || !y) return ADRT_HIP_ERR_NULL;
-    const long long Pc = (long long)batch * (h / 2) * (w / 2);
-    hipLaunchKernelGGL(pyr_pool_kernel, dim3(blocks_for(Pc)), dim3(kWgThreads), 0,
-                       static_cast<hipStream_t>(stream), x, y, Pc, h, w);
+    if (reinterpret_cast<uintptr_t>(x) % 8) return ADRT_HIP_ERR_ARG;  // float2 loads
+    const int chunks = (w / 2 + kWgThreads - 1) / kWgThreads;
+    const long long blocks = (long long)batch * (h / 2) * chunks;
+    if (blocks > 0x7fffffffLL) return ADRT_HIP_ERR_TOO_BIG;
+    hipLaunchKernelGGL(pyr_pool_kernel, dim3(unsigned(blocks), 1, 1), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), x, y, w, chunks);
     return launch_status();
 }
 
@@ -904,9 +962,13 @@ int adrt_hip_pyr_stencil_up(const float *x, const float *coarse, float *y, const
                             int batch, int h, int w, void *stream) {
     if (batch < 1 || h < 1 || w < 1 || (coarse && (h % 2 || w % 2))) return ADRT_HIP_ERR_ARG;
     if (!x || !y || !coef25) return ADRT_HIP_ERR_NULL;
-    const long long P = (long long)batch * h * w;
-    hipLaunchKernelGGL(pyr_stencil_up_kernel, dim3(blocks_for(P)), dim3(kWgThreads), 0,
-                       static_cast<hipStream_t>(stream), x, coarse, y, coef25, P, h, w);
+    const int tiles_x = (w + kPyrTW - 1) / kPyrTW;
+    const int tiles = tiles_x * ((h + kPyrTH - 1) / kPyrTH);
+    const long long blocks = (long long)batch * tiles;
+    if (blocks > 0x7fffffffLL) return ADRT_HIP_ERR_TOO_BIG;
+    hipLaunchKernelGGL(pyr_stencil_up_kernel, dim3(unsigned(blocks), 1, 1), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), x, coarse, y, coef25, h, w, tiles_x,
+                       tiles, int(w % 4 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0));
     return launch_status();
 }
 

@@ -595,79 +595,62 @@ class LearnedSolver:
         return hist
 
 
-def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=True, compact=0.75):
+def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=False, compact=0.75):
     """Batched preconditioned CG (plain CG when prec is None) for (B, n, n)
     systems with float64 scalars per image.  Stops each image when
     ||b - M x|| <= rtol * ||b|| (the recursively updated residual).  An image
     whose preconditioned step fails (r.z <= 0 or p.Mp <= 0) continues as plain
     CG from its iterate (steepest-descent restart).
 
-    With ``graph`` one iteration is captured as a hipGraph and replayed; the
-    host only checks for convergence between replays.  When the images still
-    running fall to ``compact`` times the current batch (and ``compact`` > 0),
-    they are gathered into a smaller dense batch and the iteration is
-    re-captured, so converged images cost nothing.  Returns (x, iterations
-    per image, converged mask, image-operator applications = sum over
-    iterations of the batch size operated on)."""
+    One iteration is the operator, the preconditioner and the device kernels
+    adrt_hip_pcg_update / adrt_hip_pcg_direction, which keep every per-image
+    scalar on the device.  The host never waits for the iteration it has just
+    queued: it reads the number of running images of the previous iteration
+    (copied to pinned memory behind an event) while the GPU works on the
+    current one, so the loop ends one masked (no-op) iteration late at most.
+    With ``graph`` one iteration is captured as a hipGraph and replayed.
+    When the images still running fall to ``compact`` times the current batch
+    (and ``compact`` > 0), they are gathered into a smaller dense batch, so
+    converged images cost nothing.  Returns (x, iterations per image,
+    converged mask, image-operator applications = sum over iterations of the
+    batch size operated on)."""
     torch = _torch()
     B = b.shape[0]
     b = b.contiguous()
     dev_ = b.device
     x = torch.zeros_like(b) if x0 is None else x0.contiguous().clone()
     r = (b - op(x)).contiguous() if x0 is not None else b.clone()
-    tol = rtol * bdot(b, b).sqrt()
-    use_p = torch.full((B,), prec is not None, dtype=torch.bool, device=dev_)
     rr = bdot(r, r)
+    st = _kernels.PcgState(B, dev_)
+    st.tol.copy_(rtol * bdot(b, b).sqrt())
     if prec is None:
-        z, rz = r, rr.clone()
+        z, rz = r, rr
+        st.use_p.zero_()
     else:
         z = prec(r)
         rz = bdot(r, z)
         bad = rz <= 0
-        use_p &= ~bad
+        st.use_p.copy_((~bad).int())
         z = torch.where(bad[:, None, None], r, z)
         rz = torch.where(bad, rr, rz)
+    st.rz.copy_(rz)
     p = z.clone()
-    active = rr.sqrt() > tol
-    iters = torch.zeros(B, dtype=torch.int64, device=dev_)
+    st.active.copy_((rr.sqrt() > st.tol).int())
     # results by original index; the working set is a dense sub-batch
     x_out = x
-    iters_out = iters
-    conv_out = ~active
+    iters_out = torch.zeros(B, dtype=torch.int64, device=dev_)
+    conv_out = torch.zeros(B, dtype=torch.bool, device=dev_)
     idx = torch.arange(B, device=dev_)
-    st = {"x": x, "r": r, "p": p, "rz": rz, "tol": tol, "use_p": use_p, "active": active,
-          "iters": iters}
+    S = {"x": x, "r": r, "p": p, "st": st}
 
     def make_step(S):
-        nb = S["x"].shape[0]
         ws = _kernels.cg_workspace(S["x"])
-        zero = torch.zeros(nb, dtype=torch.float64, device=dev_)
-        one = torch.ones(nb, dtype=torch.float64, device=dev_)
-        xs, rs, ps, rzs, tols = S["x"], S["r"], S["p"], S["rz"], S["tol"]
-        ups, acts, its = S["use_p"], S["active"], S["iters"]
+        xs, rs, ps, ss = S["x"], S["r"], S["p"], S["st"]
 
         def step():
             q = op(ps)
-            pq = _kernels.batched_dot(ps, q, ws)
-            ok = acts & (pq > 0)
-            alpha = torch.where(ok, rzs / torch.where(pq > 0, pq, one), zero)
-            rr_new = _kernels.cg_update(xs, rs, ps, q, alpha, ws)
-            its.add_(acts.long())
-            new_active = acts & (rr_new.sqrt() > tols)
-            if prec is None:
-                zz, rz_new = rs, rr_new
-                beta = rz_new / torch.where(rzs != 0, rzs, one)
-            else:
-                zz = torch.where(ups[:, None, None], prec(rs), rs)
-                rz_new = _kernels.batched_dot(rs, zz, ws)
-                bad = new_active & ups & ((rz_new <= 0) | (pq <= 0))
-                ups.logical_and_(~bad)
-                zz = torch.where(bad[:, None, None], rs, zz)
-                rz_new = torch.where(bad, rr_new, rz_new)
-                beta = torch.where(bad, zero, rz_new / torch.where(rzs != 0, rzs, one))
-            _kernels.cg_direction(ps, zz, beta)
-            rzs.copy_(rz_new)
-            acts.copy_(new_active)
+            _kernels.pcg_update(xs, rs, ps, q, ss, ws)
+            _kernels.pcg_direction(ps, rs, None if prec is None else prec(rs), ss, ws)
 
         if not graph:
             return step
@@ -680,26 +663,44 @@ def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=True, compac
     def write_back(S, rows):
         if S["x"] is not x_out:  # the full batch is written in place
             x_out[rows] = S["x"]
-            iters_out[rows] = S["iters"]
-        conv_out[rows] = ~S["active"]
+        iters_out[rows] = S["st"].iters.long()
+        conv_out[rows] = S["st"].active == 0
 
-    run = make_step(st)
+    counts = torch.zeros(2, dtype=torch.int32, pin_memory=True)
+    events = [torch.cuda.Event(), torch.cuda.Event()]
+    stream = torch.cuda.current_stream(dev_)
+    # running images after the next-to-last queued iteration (exact here)
+    lag = int(st.active.sum())
+    prev = None
+    run = make_step(S)
     apps = 0
-    for _ in range(maxiter):
-        n_active = int(st["active"].sum())
-        if n_active == 0:
-            break
-        nb = st["x"].shape[0]
-        if compact and nb > 8 and n_active <= compact * nb:
-            write_back(st, idx)
-            keep = st["active"].nonzero().squeeze(1)
+    steps = 0
+    while steps < maxiter and lag > 0:
+        nb = S["x"].shape[0]
+        if compact and nb > 8 and lag <= compact * nb:
+            stream.synchronize()
+            act = S["st"].active
+            exact = int(act.sum())
+            if exact == 0:
+                break
+            write_back(S, idx)
+            keep = act.nonzero().squeeze(1)
             idx = idx[keep]
-            st = {k: v[keep].contiguous() for k, v in st.items()}
-            run = make_step(st)
-            nb = n_active
+            S = {"x": S["x"][keep].contiguous(), "r": S["r"][keep].contiguous(),
+                 "p": S["p"][keep].contiguous(), "st": S["st"].select(keep)}
+            run = make_step(S)
+            nb, lag, prev = exact, exact, None
         run()
+        steps += 1
         apps += nb
-    write_back(st, idx)
+        slot = steps & 1
+        counts[slot:slot + 1].copy_(S["st"].count, non_blocking=True)
+        events[slot].record(stream)
+        if prev is not None:
+            events[prev].synchronize()
+            lag = int(counts[prev])
+        prev = slot
+    write_back(S, idx)
     return x_out, iters_out, conv_out, apps
 
 

@@ -222,6 +222,30 @@ int adrt_hip_cg_update(void *x, void *r, const void *p, const void *q, const dou
 int adrt_hip_cg_direction(void *p, const void *z, const double *beta, int dtype, size_t batch,
                           size_t m, void *stream);
 
+/* One (preconditioned) CG iteration's scalar logic on the device, over a
+ * per-image state of adrt_hip_pcg_state_bytes(B) bytes laid out as
+ *   double rz[B], pq[B], rr[B], tol[B], alpha[B], beta[B];
+ *   int32  active[B], next[B], use_p[B], iters[B], count
+ * (the caller fills rz, tol, active, use_p and iters before the first step).
+ *   adrt_hip_pcg_update:    pq = p.q; alpha = active && pq > 0 ? rz / pq : 0;
+ *                           x += alpha p; r -= alpha q; rr = r.r;
+ *                           iters += active; next = active && sqrt(rr) > tol
+ *   adrt_hip_pcg_direction: z == NULL (plain CG) or use_p == 0: rz' = rr,
+ *                           beta = rr / rz; otherwise rz' = r.z and, if
+ *                           next && (rz' <= 0 || pq <= 0), use_p = 0, rz' = rr,
+ *                           beta = 0, else beta = rz' / rz (rz == 0 reads as
+ *                           1); p = (use_p ? z : r) + beta p; rz = rz';
+ *                           active = next; count = number of active images.
+ * An inactive image keeps x and r.  Workspace as adrt_hip_cg_update; no
+ * allocation, no memset, capturable in a graph. */
+size_t adrt_hip_pcg_state_bytes(size_t batch);
+int adrt_hip_pcg_update(void *x, void *r, const void *p, const void *q, void *state,
+                        void *workspace, size_t workspace_bytes, int dtype, size_t batch,
+                        size_t m, void *stream);
+int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, void *workspace,
+                           size_t workspace_bytes, int dtype, size_t batch, size_t m,
+                           void *stream);
+
 /* ---- kernel configuration (no reference counterpart) --------------------- */
 
 /* Variants of the forward and back-projection paths for one size:
@@ -316,7 +340,8 @@ int adrt_hip_nn_adam(float *w, const float *g, float *m, float *v, void *w16, si
 /* Multiscale preconditioner pieces (fp32 images, batch x h x w):
  * 2x2 averaging; y = coef25 (5x5, zero padding) applied to x, plus
  * coarse[parent] / 4 when coarse is not NULL; and partial[block][o] =
- * sum_p e[p] * x[p + o] over o in the 5x5 window (parts blocks). */
+ * sum_p e[p] * x[p + o] over o in the 5x5 window (parts blocks).  The
+ * pooling input must be 8-byte aligned. */
 int adrt_hip_pyr_pool(const float *x, float *y, int batch, int h, int w, void *stream);
 int adrt_hip_pyr_stencil_up(const float *x, const float *coarse, float *y, const float *coef25,
                             int batch, int h, int w, void *stream);

@@ -107,3 +107,168 @@ class TestGraphPCG:
             assert int((itf - itc).abs().max()) <= 1
             assert float((xf - xc).norm() / xf.norm()) < 1e-5
             assert ac <= af
+
+
+def host_scalar_pcg(op, b, prec, rtol, maxiter):
+    """The batched PCG iteration with its per-image scalar logic in torch
+    operations between the device vector kernels, one host check per
+    iteration: the form the device-side pcg_update / pcg_direction replace.
+    Same float64 scalars, same rounding of alpha and beta to the element
+    type, same breakdown rule."""
+    import torch
+    from adrt_amd import _kernels
+
+    bd = _kernels.batched_dot
+    x = torch.zeros_like(b)
+    r = b.clone()
+    tol = rtol * bd(b, b).sqrt()
+    rr = bd(r, r)
+    use_p = torch.full((b.shape[0],), prec is not None, dtype=torch.bool, device=b.device)
+    if prec is None:
+        z, rz = r, rr.clone()
+    else:
+        z = prec(r)
+        rz = bd(r, z)
+        bad = rz <= 0
+        use_p &= ~bad
+        z = torch.where(bad[:, None, None], r, z)
+        rz = torch.where(bad, rr, rz)
+    p = z.clone()
+    active = rr.sqrt() > tol
+    iters = torch.zeros(b.shape[0], dtype=torch.int64, device=b.device)
+    one = torch.ones_like(rz)
+    zero = torch.zeros_like(rz)
+    for _ in range(maxiter):
+        if not bool(active.any()):
+            break
+        q = op(p)
+        pq = bd(p, q)
+        alpha = torch.where(active & (pq > 0), rz / torch.where(pq > 0, pq, one), zero)
+        rr_new = _kernels.cg_update(x, r, p, q, alpha)
+        iters += active.long()
+        new_active = active & (rr_new.sqrt() > tol)
+        if prec is None:
+            zz, rz_new = r, rr_new
+            beta = rz_new / torch.where(rz != 0, rz, one)
+        else:
+            zz = torch.where(use_p[:, None, None], prec(r), r)
+            rz_new = bd(r, zz)
+            bad = new_active & use_p & ((rz_new <= 0) | (pq <= 0))
+            use_p &= ~bad
+            zz = torch.where(bad[:, None, None], r, zz)
+            rz_new = torch.where(bad, rr_new, rz_new)
+            beta = torch.where(bad, zero, rz_new / torch.where(rz != 0, rz, one))
+        _kernels.cg_direction(p, zz, beta)
+        rz, active = rz_new, new_active
+    return x, iters, ~active
+
+
+class TestPcgStateLayout:
+    def test_views_tile_the_buffer(self):
+        """The state's fields are disjoint views that tile the library's
+        state size in the documented order (no device needed)."""
+        import torch
+        from adrt_amd import _kernels, _lib
+
+        for B in (1, 3, 1000):
+            st = _kernels.PcgState(B, torch.device("cpu"))
+            assert st.buf.numel() == _lib.load().adrt_hip_pcg_state_bytes(B)
+            spans = []
+            for k in _kernels.PcgState.F64 + _kernels.PcgState.I32 + ("count",):
+                v = getattr(st, k)
+                off = v.data_ptr() - st.buf.data_ptr()
+                spans.append((off, off + v.numel() * v.element_size()))
+                assert v.numel() == (1 if k == "count" else B)
+            assert spans[0][0] == 0
+            for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+                assert a1 == b0
+            assert spans[-1][1] <= st.buf.numel()
+
+
+@gpu
+class TestDevicePCG:
+    def test_update_direction_kernels(self, cuda):
+        """pcg_update / pcg_direction on a hand-made state: active and
+        inactive images, a preconditioned breakdown (r.z < 0), an image on
+        plain CG and an image with pq <= 0, against the same arithmetic in
+        float64 torch operations."""
+        import torch
+        from adrt_amd import _kernels
+
+        rng = np.random.default_rng(11)
+        B, n = 6, 32
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+        x, r, p = (t(rng.standard_normal((B, n, n)).astype(np.float32)) for _ in range(3))
+        q = p * 2.0 + t(rng.standard_normal((B, n, n)).astype(np.float32)) * 0.1
+        q[4] = -p[4]                          # pq < 0
+        z = r * 0.5
+        z[1] = -r[1]                          # r.z < 0: breakdown
+        st = _kernels.PcgState(B, cuda)
+        st.rz.copy_(t(rng.uniform(1.0, 2.0, B)))
+        st.rz[5] = 0.0                        # rz == 0 reads as 1
+        st.tol.copy_(t(np.array([1e-3, 1e-3, 1e9, 1e-3, 1e-3, 1e-3])))  # image 2 converges
+        st.active.copy_(t(np.array([1, 1, 1, 0, 1, 1], np.int32)))      # image 3 inactive
+        st.use_p.copy_(t(np.array([1, 1, 1, 1, 1, 0], np.int32)))       # image 5 plain
+        st.iters.copy_(t(np.array([4, 4, 4, 9, 4, 4], np.int32)))
+        x0, r0, p0 = x.clone(), r.clone(), p.clone()
+        rz0 = st.rz.clone()
+        act0 = st.active.bool().clone()
+        ws = _kernels.cg_workspace(x)
+        _kernels.pcg_update(x, r, p, q, st, ws)
+        pq = _kernels.batched_dot(p0, q)
+        alpha = torch.where(act0 & (pq > 0), rz0 / torch.where(pq > 0, pq, 1.0), 0.0)
+        a = alpha.float()[:, None, None]
+        assert torch.equal(x, x0 + a * p0) and torch.equal(r, r0 - a * q)
+        rr = _kernels.batched_dot(r, r)
+        assert torch.equal(st.rr, rr) and torch.equal(st.pq, pq)
+        nxt = act0 & (rr.sqrt() > st.tol)
+        assert st.next.tolist() == nxt.int().tolist()
+        assert st.iters.tolist() == [5, 5, 5, 9, 5, 5]
+        use0 = st.use_p.bool().clone()
+        _kernels.pcg_direction(p, r, z, st, ws)
+        rzd = _kernels.batched_dot(r, z)
+        bad = nxt & use0 & ((rzd <= 0) | (pq <= 0))
+        use1 = use0 & ~bad
+        rzn = torch.where(use1, rzd, rr)
+        denom = torch.where(rz0 != 0, rz0, 1.0)
+        beta = torch.where(bad, 0.0, rzn / denom)
+        assert bool(bad[1]) and bool(bad[4]) and not bool(bad[0])
+        assert st.use_p.tolist() == use1.int().tolist()
+        assert torch.equal(st.rz, rzn) and torch.equal(st.beta, beta)
+        sel = torch.where(use1[:, None, None], z, r)
+        assert torch.equal(p, sel + beta.float()[:, None, None] * p0)
+        assert st.active.tolist() == nxt.int().tolist()
+        assert int(st.count) == int(nxt.sum())
+
+    def test_matches_host_scalar_iteration(self, cuda):
+        """The device-scalar solver gives the host-scalar iteration's
+        iterates bit for bit: plain CG, the pyramid preconditioner, and a
+        preconditioner that turns indefinite on two images at its third call
+        (the restart as plain CG)."""
+        import torch
+        from adrt_amd import learned as LN
+
+        solver = LN.LearnedSolver(n=64, device=cuda)
+        solver.fit_preconditioner(count=4)
+        _, b = solver.data(seed=5, start=0, count=6, noise_seed=6)
+
+        def flipping():
+            calls = [0]
+            sign = torch.ones(6, 1, 1, device=cuda)
+            sign[1] = sign[4] = -1.0
+
+            def f(r):
+                calls[0] += 1
+                out = solver.pyramid(r)
+                return out * sign if calls[0] == 3 else out
+            return f
+
+        for name in ("plain", "pyramid", "flip"):
+            prec = {"plain": None, "pyramid": solver.pyramid}.get(name)
+            pa = flipping() if name == "flip" else prec
+            pb = flipping() if name == "flip" else prec
+            xr, ir, cr = host_scalar_pcg(LN.normal_op, b, pa, 1e-5, 300)
+            xd, idv, cd, _ = LN.pcg(LN.normal_op, b, prec=pb, rtol=1e-5, maxiter=300, compact=0)
+            assert bool(cr.all()), name
+            assert torch.equal(ir, idv) and torch.equal(cr, cd), name
+            assert torch.equal(xr, xd), name

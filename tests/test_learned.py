@@ -394,6 +394,57 @@ class TestPreconditioner:
         want = np_pyramid(r, pyr.coef.cpu().numpy().astype(np.float64))
         assert np.allclose(got, want, rtol=1e-4, atol=1e-4 * np.abs(want).max())
 
+    @pytest.mark.parametrize("shape", [(3, 20, 70), (2, 64, 64), (1, 2, 2), (5, 18, 130),
+                                       (2, 256, 256)])
+    @pytest.mark.parametrize("with_coarse", [False, True])
+    def test_stencil_level_matches_numpy(self, cuda, shape, with_coarse):
+        """One level of the up-sweep, y = h * x (5x5, zero padding) +
+        coarse[parent] / 4, on tiled and partial-tile shapes, against float64
+        numpy (float32 rounding of 26 terms)."""
+        import torch
+        from adrt_amd import _lib
+
+        rng = np.random.default_rng(sum(shape))
+        b, h, w = shape
+        x = rng.standard_normal(shape).astype(np.float32)
+        coef = rng.standard_normal(25).astype(np.float32)
+        coarse = rng.standard_normal((b, h // 2, w // 2)).astype(np.float32)
+        xd, cd = torch.from_numpy(x).to(cuda), torch.from_numpy(coef).to(cuda)
+        kd = torch.from_numpy(coarse).to(cuda)
+        y = torch.full_like(xd, float("nan"))
+        lib = _lib.load()
+        st = lib.adrt_hip_pyr_stencil_up(xd.data_ptr(), kd.data_ptr() if with_coarse else None,
+                                         y.data_ptr(), cd.data_ptr(), b, h, w,
+                                         torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        xp = np.pad(x.astype(np.float64), ((0, 0), (2, 2), (2, 2)))
+        want = np.zeros(shape)
+        for dy in range(5):
+            for dx in range(5):
+                want += coef[dy * 5 + dx] * xp[:, dy:dy + h, dx:dx + w]
+        if with_coarse:
+            want += 0.25 * np.repeat(np.repeat(coarse.astype(np.float64), 2, 1), 2, 2)
+        got = y.cpu().numpy()
+        assert np.all(np.isfinite(got))
+        assert np.allclose(got, want, rtol=1e-5, atol=1e-5 * np.abs(want).max())
+
+    @pytest.mark.parametrize("shape", [(2, 6, 130), (3, 256, 256), (1, 2, 2), (4, 2, 600)])
+    def test_pool_level_matches_numpy(self, cuda, shape):
+        """2 x 2 averaging 0.25 ((a + b) + (c + d)): exact in float32 order."""
+        import torch
+        from adrt_amd import _lib
+
+        x = np.random.default_rng(shape[2]).standard_normal(shape).astype(np.float32)
+        b, h, w = shape
+        xd = torch.from_numpy(x).to(cuda)
+        y = torch.full((b, h // 2, w // 2), float("nan"), device=cuda)
+        st = _lib.load().adrt_hip_pyr_pool(xd.data_ptr(), y.data_ptr(), b, h, w,
+                                           torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        want = np.float32(0.25) * ((x[:, ::2, ::2] + x[:, ::2, 1::2])
+                                   + (x[:, 1::2, ::2] + x[:, 1::2, 1::2]))
+        assert y.cpu().numpy().tobytes() == want.tobytes()
+
     def test_symmetric(self, cuda):
         import torch
 
