@@ -81,6 +81,8 @@ SIGNATURES = {
     "adrt_hip_get_config": (_INT, [_INT, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_SZ)]),
     "adrt_hip_nn_conv3x3": (_INT, [_VP, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _INT, _INT,
                                    _INT, _INT, _INT, _INT, _INT, _VP]),
+    "adrt_hip_nn_conv3x3_upcat": (_INT, [_VP, _INT, _INT, _VP, _INT, _VP, _VP, _VP, _INT, _INT,
+                                         _INT, _INT, _INT, _INT, _INT, _VP]),
     "adrt_hip_nn_wgrad_workspace": (_SZ, [_INT, _INT, _INT, _INT, _INT]),
     "adrt_hip_nn_wgrad": (_INT, [_VP, _INT, _VP, _INT, _VP, _VP, _VP, _SZ, _INT, _INT, _INT,
                                  _INT, _INT, _VP]),

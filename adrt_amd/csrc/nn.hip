@@ -72,6 +72,11 @@ struct ConvArgs {
     long long P;  // pixels = B * H * W
     int x_stride, y_stride, mask_stride;
     int H, W, Cin, Cout;
+    // input channels [c_split, Cin) come from x2 at half resolution (nearest
+    // upsampling: pixel (i, j) reads x2 pixel (i/2, j/2), channel c - c_split);
+    // c_split == Cin: x alone (LDS-tiled kernel only)
+    const bf16 *x2;
+    int x2_stride, c_split;
 };
 
 // Wave = 64 consecutive pixels (4 tiles of 16) x 16*NT output channels.  A
@@ -186,6 +191,7 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
     const int x0 = tx * TW, y0 = ty * kTH;
     const int MT = TW / 16;
     const long long img = b * a.H * a.W;  // first pixel of the image
+    const long long img2 = b * (a.H >> 1) * (a.W >> 1);  // ... in x2
 
     f32x4 acc[kMT][NT];
 #pragma unroll
@@ -204,9 +210,16 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
             const int py = pix / (TW + 2), px = pix - py * (TW + 2);
             const int gy = y0 + py - 1, gx = x0 + px - 1;
             const bool ok = unsigned(gy) < unsigned(a.H) && unsigned(gx) < unsigned(a.W);
-            const long long q = ok ? img + (long long)gy * a.W + gx : img;
-            const bf16x8 val = keep_or_zero(load8(a.x + q * a.x_stride + c0 + 8 * v), ok);
-            *reinterpret_cast<bf16x8 *>(&tile[py][px][8 * v]) = val;
+            const int ch = c0 + 8 * v;
+            const bf16 *src;
+            if (ch < a.c_split) {
+                const long long q = ok ? img + (long long)gy * a.W + gx : img;
+                src = a.x + q * a.x_stride + ch;
+            } else {
+                const long long q = ok ? img2 + (long long)(gy >> 1) * (a.W >> 1) + (gx >> 1) : img2;
+                src = a.x2 + q * a.x2_stride + (ch - a.c_split);
+            }
+            *reinterpret_cast<bf16x8 *>(&tile[py][px][8 * v]) = keep_or_zero(load8(src), ok);
         }
         __syncthreads();
         const int Kc = 9 * ccv;
@@ -243,29 +256,51 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
     }
 
     const long long prow = img + (long long)(y0 + wave) * a.W + x0;
+    if (MODE == kResidual) {
 #pragma unroll
-    for (int mt = 0; mt < kMT; ++mt) {
-        if (mt >= MT) break;
+        for (int mt = 0; mt < kMT; ++mt) {
+            if (mt >= MT) break;
+            if (r == 0 && co0 == 0) {
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int co = co0 + 16 * nt + r;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const long long p = prow + 16 * mt + 4 * g + j;
-                float v = acc[mt][nt][j];
-                if (MODE == kReluBias) {
-                    v += a.bias[co];
-                    v = v > 0.f ? v : 0.f;
-                    a.y[p * a.y_stride + co] = bf16(v);
-                } else if (MODE == kResidual) {
-                    if (co == 0) a.y32[p] = a.resid[p] + (v + a.bias[0]);
-                } else if (MODE == kMaskInput) {
-                    const float m = float(a.mask[p * a.mask_stride + co]);
-                    a.y[p * a.y_stride + co] = bf16(m > 0.f ? v : 0.f);
-                } else {
-                    a.y[p * a.y_stride + co] = bf16(v);
+                for (int j = 0; j < 4; ++j) {
+                    const long long p = prow + 16 * mt + 4 * g + j;
+                    a.y32[p] = a.resid[p] + (acc[mt][0][j] + a.bias[0]);
                 }
             }
+        }
+        return;
+    }
+    // bf16 outputs go through LDS, one 16-channel slice at a time: each wave
+    // writes its row's [16*MT pixels][16 channels] block, then stores it as
+    // 16-byte vectors (8 channels of one pixel per lane).
+    bf16 *ob = &tile[0][0][0] + wave * (kTWmax * 16);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int co = co0 + 16 * nt + r;
+        const float bco = MODE == kReluBias ? a.bias[co] : 0.f;
+        __syncthreads();  // the tile's (or the previous slice's) readers are done
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt) {
+            if (mt >= MT) break;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int px = 16 * mt + 4 * g + j;
+                float v = acc[mt][nt][j];
+                if (MODE == kReluBias) {
+                    v += bco;
+                    v = v > 0.f ? v : 0.f;
+                } else if (MODE == kMaskInput) {
+                    const float m = float(a.mask[(prow + px) * a.mask_stride + co]);
+                    v = m > 0.f ? v : 0.f;
+                }
+                ob[px * 16 + r] = bf16(v);
+            }
+        }
+        __syncthreads();
+        for (int e = lane; e < 32 * MT; e += kWaveLanes) {
+            const int px = e >> 1, h = e & 1;
+            const bf16x8 v = *reinterpret_cast<const bf16x8 *>(&ob[px * 16 + 8 * h]);
+            *reinterpret_cast<bf16x8 *>(&a.y[(prow + px) * a.y_stride + co0 + 16 * nt + 8 * h]) = v;
         }
     }
 }
@@ -770,6 +805,7 @@ int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float 
         if (!y32 || !resid || !bias) return ADRT_HIP_ERR_NULL;
     } else {
         if (!y || y_stride < cout) return ADRT_HIP_ERR_ARG;
+        if (!aligned16(y)) return ADRT_HIP_ERR_NULL;
         if (mode == kReluBias && !bias) return ADRT_HIP_ERR_NULL;
         if (mode == kMaskInput && (!mask || mask_stride < cout)) return ADRT_HIP_ERR_NULL;
     }
@@ -789,6 +825,9 @@ int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float 
     a.W = w_;
     a.Cin = cin;
     a.Cout = cout;
+    a.x2 = nullptr;
+    a.x2_stride = 0;
+    a.c_split = cin;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (cout % 64 == 0)
         launch_conv_nt<4>(a, mode, s);
@@ -796,6 +835,49 @@ int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float 
         launch_conv_nt<2>(a, mode, s);
     else
         launch_conv_nt<1>(a, mode, s);
+    return launch_status();
+}
+
+int adrt_hip_nn_conv3x3_upcat(const void *x, int x_stride, int c_skip, const void *up,
+                              int up_stride, const void *w, const float *bias, void *y,
+                              int y_stride, int batch, int h, int w_, int cin, int cout, int mode,
+                              void *stream) {
+    if (batch < 1 || h < 4 || w_ < 16 || h % kTH != 0 || w_ % 16 != 0 ||
+        (w_ > kTWmax && w_ % kTWmax != 0) || cin < 16 || cin % 8 != 0 || cout < 16 ||
+        cout % 16 != 0 || c_skip < 8 || c_skip % 8 != 0 || c_skip >= cin ||
+        x_stride < c_skip || x_stride % 8 != 0 || up_stride < cin - c_skip ||
+        up_stride % 8 != 0 || y_stride < cout || (mode != kReluBias && mode != kLinear))
+        return ADRT_HIP_ERR_ARG;
+    if (!x || !up || !w || !y || !aligned16(x) || !aligned16(up) || !aligned16(w) ||
+        !aligned16(y))
+        return ADRT_HIP_ERR_NULL;
+    if (mode == kReluBias && !bias) return ADRT_HIP_ERR_NULL;
+    ConvArgs a;
+    a.x = static_cast<const bf16 *>(x);
+    a.w = static_cast<const bf16 *>(w);
+    a.bias = bias;
+    a.y = static_cast<bf16 *>(y);
+    a.y32 = nullptr;
+    a.resid = nullptr;
+    a.mask = nullptr;
+    a.P = (long long)batch * h * w_;
+    a.x_stride = x_stride;
+    a.y_stride = y_stride;
+    a.mask_stride = 0;
+    a.H = h;
+    a.W = w_;
+    a.Cin = cin;
+    a.Cout = cout;
+    a.x2 = static_cast<const bf16 *>(up);
+    a.x2_stride = up_stride;
+    a.c_split = c_skip;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (cout % 64 == 0)
+        launch_conv_lds<4>(a, mode, s);
+    else if (cout % 32 == 0)
+        launch_conv_lds<2>(a, mode, s);
+    else
+        launch_conv_lds<1>(a, mode, s);
     return launch_status();
 }
 

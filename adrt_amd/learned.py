@@ -327,9 +327,9 @@ class UNet:
 
         n1, n2, n3 = n // 2, n // 4, n // 8
         buf = {
-            "x8": t(n, 8), "t0": t(n, 16), "cat0": t(n, 48), "p0": t(n1, 16),
-            "t1": t(n1, 32), "cat1": t(n1, 96), "p1": t(n2, 32),
-            "t2": t(n2, 64), "cat2": t(n2, 192), "p2": t(n3, 64),
+            "x8": t(n, 8), "t0": t(n, 16), "p0": t(n1, 16),
+            "t1": t(n1, 32), "p1": t(n2, 32),
+            "t2": t(n2, 64), "p2": t(n3, 64),
             "t3": t(n3, 128), "b3": t(n3, 128),
             "t2d": t(n2, 64), "d2": t(n2, 64), "t1d": t(n1, 32), "d1": t(n1, 32),
             "t0d": t(n, 16), "d0": t(n, 16),
@@ -339,6 +339,9 @@ class UNet:
             self._buffers[key] = buf
             return buf
         buf.update({
+            # activations kept for backward: skip connections + upsampled
+            # coarse features in one buffer per level
+            "cat0": t(n, 48), "cat1": t(n1, 96), "cat2": t(n2, 192),
             # gradients
             "g_out": t(n, 16), "g16a": t(n, 16), "g16b": t(n, 16), "g_cat0": t(n, 48),
             "g32a": t(n1, 32), "g32b": t(n1, 32), "g_cat1": t(n1, 96),
@@ -385,6 +388,8 @@ class UNet:
         f = self.buffers(B, n, train)
         n1, n2, n3 = n // 2, n // 4, n // 8
         s = _stream()
+        if not train:
+            return self._infer(z, f, B, n)
         cat0, cat1, cat2 = f["cat0"], f["cat1"], f["cat2"]
         _check(lib.adrt_hip_nn_pack_input(_p(z), _p(f["x8"]), B, n, n, 1.0, s), "pack_input")
         self._conv(f["x8"], 8, "e0a", f["t0"], 16, B, n, 0)
@@ -409,6 +414,60 @@ class UNet:
         _check(lib.adrt_hip_nn_upsample(_p(f["d1"]), 32, _p(cat0[..., 16:]), 48, B, n1, n1, 32, s),
                "upsample")
         self._conv(cat0, 48, "d0a", f["t0d"], 16, B, n, 0)
+        self._conv(f["t0d"], 16, "d0b", f["d0"], 16, B, n, 0)
+        self._conv(f["d0"], 16, "out", None, 0, B, n, 1, y32=f["y"], resid=z)
+        return f["y"]
+
+    def _upcat(self, f, skip, cs, up, name, y, B, h):
+        """Decoder convolution over [skip | nearest-upsampled up], read in
+        place (adrt_hip_nn_conv3x3_upcat) where its tiling applies, else
+        through a concatenation buffer."""
+        _, _, cin, cout = self.offsets[name]
+        lib = _lib.load()
+        if h % 16 == 0 and (h <= 64 or h % 64 == 0):
+            _check(lib.adrt_hip_nn_conv3x3_upcat(
+                _p(skip), cs, cs, _p(up), cin - cs, _p(self.weight(name)), _p(self.bias(name)),
+                _p(y), cout, B, h, h, cin, cout, 0, _stream()), f"conv {name}")
+            return
+        torch = _torch()
+        key = "cat_" + name
+        if key not in f:
+            f[key] = torch.empty((B, h, h, cin), dtype=torch.bfloat16, device=self.device)
+        cat = f[key]
+        cat[..., :cs].copy_(skip)
+        _check(lib.adrt_hip_nn_upsample(_p(up), cin - cs, _p(cat[..., cs:]), cin, B, h // 2,
+                                        h // 2, cin - cs, _stream()), "upsample")
+        self._conv(cat, cin, name, y, cout, B, h, 0)
+
+    def _infer(self, z, f, B, n):
+        """Inference form of forward(): the skip connections keep their own
+        buffers and each decoder convolution reads the coarse features
+        through the upsampling in place, so no concatenation buffer is
+        written.  Same arithmetic, bit for bit."""
+        torch = _torch()
+        lib = _lib.load()
+        n1, n2, n3 = n // 2, n // 4, n // 8
+        s = _stream()
+        for key, hw, c in (("s0", n, 16), ("s1", n1, 32), ("s2", n2, 64)):
+            if key not in f:
+                f[key] = torch.empty((B, hw, hw, c), dtype=torch.bfloat16, device=self.device)
+        _check(lib.adrt_hip_nn_pack_input(_p(z), _p(f["x8"]), B, n, n, 1.0, s), "pack_input")
+        self._conv(f["x8"], 8, "e0a", f["t0"], 16, B, n, 0)
+        self._conv(f["t0"], 16, "e0b", f["s0"], 16, B, n, 0)
+        _check(lib.adrt_hip_nn_pool(_p(f["s0"]), 16, _p(f["p0"]), 16, B, n, n, 16, s), "pool")
+        self._conv(f["p0"], 16, "e1a", f["t1"], 32, B, n1, 0)
+        self._conv(f["t1"], 32, "e1b", f["s1"], 32, B, n1, 0)
+        _check(lib.adrt_hip_nn_pool(_p(f["s1"]), 32, _p(f["p1"]), 32, B, n1, n1, 32, s), "pool")
+        self._conv(f["p1"], 32, "e2a", f["t2"], 64, B, n2, 0)
+        self._conv(f["t2"], 64, "e2b", f["s2"], 64, B, n2, 0)
+        _check(lib.adrt_hip_nn_pool(_p(f["s2"]), 64, _p(f["p2"]), 64, B, n2, n2, 64, s), "pool")
+        self._conv(f["p2"], 64, "b3a", f["t3"], 128, B, n3, 0)
+        self._conv(f["t3"], 128, "b3b", f["b3"], 128, B, n3, 0)
+        self._upcat(f, f["s2"], 64, f["b3"], "d2a", f["t2d"], B, n2)
+        self._conv(f["t2d"], 64, "d2b", f["d2"], 64, B, n2, 0)
+        self._upcat(f, f["s1"], 32, f["d2"], "d1a", f["t1d"], B, n1)
+        self._conv(f["t1d"], 32, "d1b", f["d1"], 32, B, n1, 0)
+        self._upcat(f, f["s0"], 16, f["d1"], "d0a", f["t0d"], B, n)
         self._conv(f["t0d"], 16, "d0b", f["d0"], 16, B, n, 0)
         self._conv(f["d0"], 16, "out", None, 0, B, n, 1, y32=f["y"], resid=z)
         return f["y"]

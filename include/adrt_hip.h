@@ -285,13 +285,24 @@ int adrt_hip_get_config(int op, size_t n, int *variant, size_t *group_bytes);
 /* 3x3 convolution with zero padding, on the matrix cores (bf16 in, fp32
  * accumulation).  mode 0: y = relu(conv + bias); 1: y32[p] = resid[p] +
  * conv[p][0] + bias[0] (fp32 output, channel 0); 2: y = conv * (mask > 0);
- * 3: y = conv.  Modes 2 and 3 with the weights of
+ * 3: y = conv (y 16-byte aligned).  Modes 2 and 3 with the weights of
  * adrt_hip_nn_transpose_weights give the input gradient.  cout % 16 == 0,
  * batch*h*w_ % 64 == 0. */
 int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float *bias, void *y,
                         int y_stride, float *y32, const float *resid, const void *mask,
                         int mask_stride, int batch, int h, int w_, int cin, int cout, int mode,
                         void *stream);
+
+/* The same convolution (modes 0 and 3) over the channel concatenation of x
+ * (channels [0, c_skip), h x w_) and the nearest 2x upsampling of up
+ * (channels [c_skip, cin), h/2 x w_/2): the U-Net decoder's input, read in
+ * place instead of materialised.  Needs h % 4 == 0, w_ % 16 == 0 (and
+ * w_ <= 64 or w_ % 64 == 0), cin >= 16, c_skip % 8 == 0; the result equals
+ * adrt_hip_nn_conv3x3 on the concatenated buffer bit for bit. */
+int adrt_hip_nn_conv3x3_upcat(const void *x, int x_stride, int c_skip, const void *up,
+                              int up_stride, const void *w, const float *bias, void *y,
+                              int y_stride, int batch, int h, int w_, int cin, int cout, int mode,
+                              void *stream);
 
 /* Workspace bytes of adrt_hip_nn_wgrad. */
 size_t adrt_hip_nn_wgrad_workspace(int batch, int h, int w, int cin, int cout);

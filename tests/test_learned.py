@@ -370,6 +370,58 @@ class TestUNet:
                 cos = float((gg * gr).sum() / (gg.norm() * gr.norm()))
                 assert err < 6e-2 and cos > 0.998, (name, what, err, cos)
 
+    @pytest.mark.parametrize("n", [32, 64, 256])
+    def test_inference_forward_equals_training_forward(self, cuda, n):
+        """The inference forward (skip buffers + decoder convolutions that
+        read the upsampled coarse features in place) reproduces the training
+        forward (materialised concatenations) bit for bit."""
+        import torch
+
+        net = LN.UNet(seed=3, device=cuda)
+        rng = np.random.default_rng(n)
+        w = net.w.cpu().numpy()
+        for name, _, c in LN.LAYERS:
+            _, bb, _, cc = net.offsets[name]
+            w[bb:bb + cc] = rng.standard_normal(cc) * 0.05
+        o0, b0, cin, cout = net.offsets["out"]
+        w[o0:o0 + cout * 9 * cin] = rng.standard_normal(cout * 9 * cin) * 0.2
+        net.load_state(w)
+        z = torch.from_numpy(rng.standard_normal((4, n, n)).astype(np.float32)).to(cuda)
+        y_train = net.forward(z, train=True).clone()
+        y_inf = net.forward(z, train=False).clone()
+        assert float((y_train - z).abs().max()) > 0
+        assert torch.equal(y_train, y_inf)
+
+    @pytest.mark.parametrize("h,cs,cu,cout", [(64, 16, 32, 16), (128, 32, 64, 32),
+                                              (16, 64, 128, 64), (256, 16, 32, 16)])
+    def test_upcat_conv_equals_concatenated_conv(self, cuda, h, cs, cu, cout):
+        """adrt_hip_nn_conv3x3_upcat over (skip, up) equals the plain
+        convolution over the materialised [skip | upsample(up)] bit for bit
+        (ReLU + bias mode)."""
+        import torch
+        from adrt_amd import _lib
+
+        rng = np.random.default_rng(h + cs)
+        B, cin = 2, cs + cu
+        bf = torch.bfloat16
+        skip = torch.from_numpy(rng.standard_normal((B, h, h, cs)).astype(np.float32)).to(bf).to(cuda)
+        up = torch.from_numpy(rng.standard_normal((B, h // 2, h // 2, cu)).astype(np.float32)).to(bf).to(cuda)
+        wt = torch.from_numpy(rng.standard_normal(cout * 9 * cin).astype(np.float32) * 0.1).to(bf).to(cuda)
+        bias = torch.from_numpy(rng.standard_normal(cout).astype(np.float32)).to(cuda)
+        cat = torch.cat([skip, up.repeat_interleave(2, 1).repeat_interleave(2, 2)], dim=3).contiguous()
+        y0 = torch.full((B, h, h, cout), float("nan"), dtype=bf, device=cuda)
+        y1 = torch.full((B, h, h, cout), float("nan"), dtype=bf, device=cuda)
+        lib = _lib.load()
+        s = torch.cuda.current_stream().cuda_stream
+        assert lib.adrt_hip_nn_conv3x3(cat.data_ptr(), cin, wt.data_ptr(), bias.data_ptr(),
+                                       y0.data_ptr(), cout, None, None, None, 0, B, h, h, cin,
+                                       cout, 0, s) == 0
+        assert lib.adrt_hip_nn_conv3x3_upcat(skip.data_ptr(), cs, cs, up.data_ptr(), cu,
+                                             wt.data_ptr(), bias.data_ptr(), y1.data_ptr(), cout,
+                                             B, h, h, cin, cout, 0, s) == 0
+        assert not bool(torch.isnan(y1.float()).any())
+        assert torch.equal(y0, y1)
+
     def test_training_reduces_loss(self, cuda):
         solver = LN.LearnedSolver(n=64, device=cuda)
         solver.fit_preconditioner(count=8)
