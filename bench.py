@@ -62,8 +62,12 @@ def parse():
     p.add_argument("--calls", type=int, default=4,
                    help="calls in the profiled run (--summarize)")
     p.add_argument("--match", default="adrt_hip",
-                   help="count only kernels whose name contains this (--summarize)")
-    p.add_argument("--op", default="adrt", choices=["adrt", "bdrt", "iadrt", "unet"],
+                   help="count only kernels whose name contains this (--summarize); with "
+                        "--iterations, the kernel that ends each iteration")
+    p.add_argument("--iterations", type=int, default=0,
+                   help="--summarize a kernel trace per solver iteration over the last K "
+                        "iterations (e.g. --op pcg: --match pcg_dir_kernel --iterations 5)")
+    p.add_argument("--op", default="adrt", choices=["adrt", "bdrt", "iadrt", "unet", "pcg"],
                    help="operation a --profile run calls (bdrt/iadrt on the forward output)")
     return p.parse_args()
 
@@ -295,6 +299,31 @@ def summarize(dirs, calls, match="adrt_hip"):
     return "\n".join(lines)
 
 
+def summarize_iterations(dirs, marker, last):
+    """Per-iteration kernel time of the last ``last`` iterations of a traced
+    solver run (rocprofv3 --kernel-trace csv): the dispatches after the
+    ``last + 1``-th from last dispatch whose name contains ``marker`` (a
+    kernel that runs once per iteration, at its end), grouped by kernel."""
+    rows = sorted(_csv_rows(dirs, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+    ends = [i for i, r in enumerate(rows) if marker in r["Kernel_Name"]]
+    if len(ends) < last + 1:
+        return f"fewer than {last + 1} dispatches of {marker}"
+    window = rows[ends[-last - 1] + 1:ends[-1] + 1]
+    agg = {}
+    for r in window:
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        a = agg.setdefault(_short(r["Kernel_Name"]), [0, 0.0])
+        a[0] += 1
+        a[1] += d
+    lines = ["| kernel | dispatches / iteration | µs / iteration |", "|---|---|---|"]
+    total = 0.0
+    for k, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        total += t
+        lines.append(f"| `{k}` | {c / last:.1f} | {t / last:.1f} |")
+    lines.append(f"\nkernel_us_per_iteration: {total / last:.1f}")
+    return "\n".join(lines)
+
+
 def profiled_traffic():
     """HBM bytes per forward call from the committed rocprofv3 summary."""
     try:
@@ -310,7 +339,10 @@ def profiled_traffic():
 def main():
     args = parse()
     if args.summarize:
-        print(summarize(args.summarize, args.calls, args.match))
+        if args.iterations:
+            print(summarize_iterations(args.summarize, args.match, args.iterations))
+        else:
+            print(summarize(args.summarize, args.calls, args.match))
         return
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.extras = 3, 1, True, False
@@ -345,6 +377,21 @@ def main():
             solver.net.forward(z, train=False)
         torch.cuda.synchronize()
         print(json.dumps({"op": "unet", "images": 1000, "n": 256}), flush=True)
+        return
+    if args.profile and args.op == "pcg":
+        # preconditioned CG on the 1000 held-out 256x256 systems of the
+        # learned-CG leg: 2 iterations (warm-up), then 5 iterations without
+        # compaction, the pattern the per-iteration kernel breakdown uses
+        from adrt_amd import batched, learned
+
+        solver = learned.LearnedSolver.load(LEARNED_WEIGHTS)
+        _, b = solver.data(2000, 0, 1000, noise_seed=2000)
+        for op in (_lib.OP_ADRT, _lib.OP_BDRT):
+            batched.tune(op, 256, 1000)
+        learned.pcg(learned.normal_op, b, prec=solver.pyramid, rtol=1e-4, maxiter=2)
+        learned.pcg(learned.normal_op, b, prec=solver.pyramid, rtol=1e-4, maxiter=5, compact=0)
+        torch.cuda.synchronize()
+        print(json.dumps({"op": "pcg", "images": 1000, "n": 256, "iterations": 5}), flush=True)
         return
     n, batch = args.n, args.batch
     s = 4 if args.dtype == "float32" else 8
