@@ -1346,10 +1346,12 @@ struct Tuned {
     std::atomic<int> variant{ADRT_HIP_VARIANT_DEFAULT};
     std::atomic<size_t> group_bytes{0};
 };
-Tuned g_tuned[2][64];
+Tuned g_tuned[3][64];
 
 Tuned *tuned(int op, size_t n) {
-    if ((op != ADRT_HIP_OP_ADRT && op != ADRT_HIP_OP_BDRT) || !is_pow2(n)) return nullptr;
+    if ((op != ADRT_HIP_OP_ADRT && op != ADRT_HIP_OP_BDRT && op != ADRT_HIP_OP_IADRT) ||
+        !is_pow2(n))
+        return nullptr;
     return &g_tuned[op][num_iters(n)];
 }
 
@@ -1374,7 +1376,8 @@ int set_fused_config(int op, size_t n, int variant, size_t group_bytes) {
     if (!t) return ADRT_HIP_ERR_ARG;
     if (variant < ADRT_HIP_VARIANT_DEFAULT || variant > ADRT_HIP_VARIANT_STREAMING)
         return ADRT_HIP_ERR_ARG;
-    if (variant == ADRT_HIP_VARIANT_ALT_SPLIT && !fused_supported(n)) return ADRT_HIP_ERR_ARG;
+    if (variant == ADRT_HIP_VARIANT_ALT_SPLIT && op != ADRT_HIP_OP_IADRT && !fused_supported(n))
+        return ADRT_HIP_ERR_ARG;
     t->variant.store(variant, std::memory_order_relaxed);
     t->group_bytes.store(group_bytes, std::memory_order_relaxed);
     return ADRT_HIP_OK;

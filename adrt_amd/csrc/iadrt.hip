@@ -102,6 +102,97 @@ __host__ __device__ inline bool step(const Wave &w, const Lane &ln, int s, T p0,
     return true;
 }
 
+// Fused tail.  Once a stage's input groups are at most 64 columns wide
+// (Ci <= 64), one wavefront's 64 columns hold whole groups of that stage and
+// of every later one, so the last S stages run in one pass: the first of
+// them (t = 0) reads memory exactly as the stage kernel does (lagged odd
+// chains) and writes its rows into a ring in LDS; each later stage t works on
+// row s + t at step s, reading rows its producer wrote at earlier steps
+// (rows x .. x + Co) from the producer's ring, and the last stage stores its
+// row straight to memory (every lane produces the same row at a step, so the
+// store is whole and coalesced).
+//
+// Ring sizes (rows, a power of two each).  Stage 0 writes rows down to
+// s - maxlag (maxlag = Co_0 - 1) while stage 1 still reads rows up to
+// s + 1 + Co_1, so its ring spans maxlag + Co_1 + 2 = 3 Co_0 / 2 + 1 rows; a
+// later stage t writes row s + t while its consumer reads rows up to
+// s + t + 1 + Co_{t+1}: Co_{t+1} + 2 rows.  The first stage of an S-stage
+// tail has Ci = 2^S columns per group.
+constexpr int kMaxTail = 6;  // stages 64, 32, .., 2 columns wide
+
+__host__ __device__ constexpr int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+// rows of the ring stage t (0 <= t <= S - 2) writes for stage t + 1
+__host__ __device__ constexpr int tail_ring_rows(int S, int t) {
+    return t == 0 ? pow2_at_least(3 * (1 << (S - 1)) / 2 + 1)
+                  : pow2_at_least((1 << (S - t - 2)) + 2);
+}
+// first row of that ring in the kernel's ring buffer
+__host__ __device__ constexpr int tail_ring_base(int S, int t) {
+    int b = 0;
+    for (int u = 0; u < t; ++u) b += tail_ring_rows(S, u);
+    return b;
+}
+__host__ __device__ constexpr int tail_rows_total(int S) { return tail_ring_base(S, S - 1); }
+
+struct TailLane {
+    bool live;
+    bool odd;
+    int col;
+    int pc;  // ring column of the src pair (2col, 2col+1) of the group
+};
+
+__host__ __device__ inline TailLane make_tail_lane(int n, int Ci, int ch0, int lane) {
+    TailLane t;
+    const int ch = ch0 + lane;
+    t.live = ch < n;
+    const int Co = Ci / 2;
+    const int chc = t.live ? ch : ch0;
+    const int l = chc / Co;
+    t.col = chc % Co;
+    t.odd = (l & 1) != 0;
+    t.pc = (l >> 1) * Ci + 2 * t.col - ch0;
+    return t;
+}
+
+// The operands of row x of a later stage's chain in its producer's ring
+// (rows base .. base + mask, pitch kLanes + 1):
+//   even: a = src[2col][x],          b = src[2col+1][x+1]  (b used if x+1 < R)
+//   odd:  a = src[2col+1][x+1+col],  b = src[2col][x+1+col] (both if x+1+col < R)
+// Both slots are read unconditionally (a ring slot exists for every row) and
+// the conditions are applied by selects, so a step has no branches.
+struct TailOperands {
+    int ia, ib;   // ring element indices of a and b
+    bool use_a;   // a enters (even: always)
+    bool use_b;   // b enters
+};
+
+__host__ __device__ __forceinline__ TailOperands tail_operands(const TailLane &tl, int x, int R,
+                                                               int base, int mask) {
+    TailOperands o;
+    const int ro = x + 1 + tl.col;  // the odd chain's row
+    const int ra = tl.odd ? ro : x, rb = tl.odd ? ro : x + 1;
+    o.ia = (base + (ra & mask)) * (kLanes + 1) + tl.pc + (tl.odd ? 1 : 0);
+    o.ib = (base + (rb & mask)) * (kLanes + 1) + tl.pc + (tl.odd ? 0 : 1);
+    o.use_a = tl.odd ? ro < R : true;
+    o.use_b = tl.odd ? ro < R : x + 1 < R;
+    return o;
+}
+
+// val = 0; val += a; val -= b; (if x + 1 < R) val += acc, the reference's
+// order, with each skipped term skipped exactly (selects, not zeros added:
+// -0 + 0 would turn a -0 into +0).
+template <typename T>
+__host__ __device__ __forceinline__ T tail_combine(const TailOperands &o, T a, T b, T acc,
+                                                   bool add_acc) {
+    T v = o.use_a ? T(0) + a : T(0);
+    v = o.use_b ? v - b : v;
+    return add_acc ? v + acc : v;
+}
+
 #ifndef ADRT_HOST_EMULATION
 template <typename T>
 struct Pair;
@@ -172,9 +263,106 @@ __global__ void __launch_bounds__(kLanes)
     for (int k = 0; k < kSteps; ++k)  // tail: steps s..0
         if (k <= s) run_step(s - k, cur[k], acc, prev1);
 }
+
+// The last S stages (the first with Li0 groups, Ci = n / Li0 = 2^S), one
+// single-wave work-group per 64 columns of a plane.
+template <typename T, int S>
+__global__ void __launch_bounds__(kLanes)
+    tail_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li0,
+                int waves_per_plane) {
+    using P2 = typename Pair<T>::type;
+    constexpr int kM0 = tail_ring_rows(S, 0);
+    __shared__ T rbuf[tail_rows_total(S) * (kLanes + 1)];
+    const int lane = threadIdx.x;
+    const size_t plane = blockIdx.x / waves_per_plane;
+    const Wave w = make_wave(R, n, Li0, int(blockIdx.x % waves_per_plane));
+    const Lane ln = make_lane(w, lane);
+    const T *src = in + plane * size_t(R) * n;
+    T *dst = out + plane * size_t(R) * n + w.ch0 + lane;
+    TailLane tl[S - 1];
+#pragma unroll
+    for (int t = 1; t < S; ++t) tl[t - 1] = make_tail_lane(n, n / (Li0 << t), w.ch0, lane);
+
+    auto load = [&](int s) -> P2 {
+        int r = s + ln.roff;
+        r = r < 0 ? 0 : (r > R - 1 ? R - 1 : r);
+        return *reinterpret_cast<const P2 *>(src + size_t(r) * n + (ln.live ? ln.pc : 0));
+    };
+
+    T acc0 = T(0), prev1 = T(0);
+    T acc[S - 1];
+#pragma unroll
+    for (int t = 0; t < S - 1; ++t) acc[t] = T(0);
+    auto run_step = [&](int sk, const P2 &p) {
+        // every later stage's operands first (rows written at earlier steps),
+        // so the step's LDS reads are issued together
+        T a[S - 1], b[S - 1];
+        TailOperands o[S - 1];
+#pragma unroll
+        for (int t = 1; t < S; ++t) {
+            o[t - 1] = tail_operands(tl[t - 1], sk + t, R, tail_ring_base(S, t - 1),
+                                     tail_ring_rows(S, t - 1) - 1);
+            a[t - 1] = rbuf[o[t - 1].ia];
+            b[t - 1] = rbuf[o[t - 1].ib];
+        }
+        // stage 0 (the step function on the loaded pair)
+        T val;
+        int row;
+        if (step<T>(w, ln, sk, p.x, p.y, prev1, acc0, val, row))
+            rbuf[(row & (kM0 - 1)) * (kLanes + 1) + lane] = val;
+        prev1 = p.y;
+        // later stages: row x = sk + t is wave-uniform
+#pragma unroll
+        for (int t = 1; t < S; ++t) {
+            const int x = sk + t;
+            if (x < 0 || x >= R) continue;
+            const T v = tail_combine<T>(o[t - 1], a[t - 1], b[t - 1], acc[t - 1], x + 1 < R);
+            acc[t - 1] = v;
+            if (t < S - 1)
+                rbuf[(tail_ring_base(S, t) + (x & (tail_ring_rows(S, t) - 1))) * (kLanes + 1) +
+                     lane] = v;
+            else if (tl[t - 1].live)
+                dst[size_t(x) * n] = v;
+        }
+    };
+
+    // steps w.top .. -(S-1): the last stage reaches row 0 at step -(S-1);
+    // loads for steps below 0 are clamped into the rows and unused
+    const int s_end = -(S - 1);
+    P2 cur[kSteps];
+    int s = w.top;
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k) cur[k] = load(s - k);
+    for (; s - (kSteps - 1) >= s_end; s -= kSteps) {  // whole blocks
+        P2 nxt[kSteps];
+#pragma unroll
+        for (int k = 0; k < kSteps; ++k) nxt[k] = load(s - kSteps - k);
+#pragma unroll
+        for (int k = 0; k < kSteps; ++k) run_step(s - k, cur[k]);
+#pragma unroll
+        for (int k = 0; k < kSteps; ++k) cur[k] = nxt[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kSteps; ++k)  // tail: steps s .. s_end
+        if (s - k >= s_end) run_step(s - k, cur[k]);
+}
 #endif  // ADRT_HOST_EMULATION
 
 }  // namespace iadrt_native
+
+// Stages in the fused tail for L = log2 n stages and a configuration
+// variant: DEFAULT at most 5 (measured at n = 2048, float32 and float64:
+// 5 stages beat 6, whose first ring needs 64 rows and halves the resident
+// waves, and 4, which leaves one more pass to memory); ALT_SPLIT at most 2;
+// STREAMING none (one kernel per stage).
+constexpr int kDefaultTail = 5;
+
+template <typename T>
+inline int tail_stages(int L, int variant = ADRT_HIP_VARIANT_DEFAULT) {
+    if (variant == ADRT_HIP_VARIANT_STREAMING) return 0;
+    const int cap = variant == ADRT_HIP_VARIANT_ALT_SPLIT ? kDefaultTail / 2 : kDefaultTail;
+    return L < cap ? L : cap;
+}
 
 #ifndef ADRT_HOST_EMULATION
 template <typename T>
@@ -187,18 +375,34 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
             return launch_status();
         return launch_status();
     }
-    // ping-pong so that the last stage writes `out`
-    T *bufs[2] = {(L % 2 == 1) ? out : ws, (L % 2 == 1) ? ws : out};
+    // the last S stages run fused (S = 1: no tail kernel)
+    const int S = tail_stages<T>(L, fused_variant(ADRT_HIP_OP_IADRT, n));
+    const int k0 = L - (S >= 2 ? S : 0);  // stages run one per pass
+    const int passes = k0 + (S >= 2 ? 1 : 0);
+    // ping-pong so that the last pass writes `out`
+    T *bufs[2] = {(passes % 2 == 1) ? out : ws, (passes % 2 == 1) ? ws : out};
     const int waves_per_plane = int((n + iadrt_native::kLanes - 1) / iadrt_native::kLanes);
     const unsigned grid = unsigned(B * 4 * size_t(waves_per_plane));
     const T *src = in;
     int Li = 1;
-    for (int k = 0; k < L; ++k) {
+    for (int k = 0; k < k0; ++k) {
         T *dst = bufs[k & 1];
         hipLaunchKernelGGL(iadrt_native::stage_kernel<T>, dim3(grid), dim3(iadrt_native::kLanes),
                            0, s, src, dst, int(R), int(n), Li, waves_per_plane);
         src = dst;
         Li *= 2;
+    }
+    if (S >= 2) {
+        T *dst = bufs[k0 & 1];
+        const dim3 g(grid), blk(iadrt_native::kLanes);
+        const int r = int(R), nn = int(n);
+        switch (S) {
+            case 2: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 2>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+            case 3: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 3>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+            case 4: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 4>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+            case 5: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 5>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+            default: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 6>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+        }
     }
     return launch_status();
 }

@@ -254,6 +254,11 @@ int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, v
  *              stages, back-projection pass A floor(L/2)); other n: streaming;
  *   ALT_SPLIT  the fused kernels with the other split (differs for odd L);
  *   STREAMING  one kernel per stage.
+ * and of the inverse (ADRT_HIP_OP_IADRT; the group budget does not apply):
+ *   DEFAULT    the last min(L, 5) stages in one fused pass, the others one
+ *              kernel each;
+ *   ALT_SPLIT  a fused tail of min(L, 2) stages;
+ *   STREAMING  one kernel per stage.
  * Every variant gives bit-identical results. */
 #define ADRT_HIP_VARIANT_DEFAULT 0
 #define ADRT_HIP_VARIANT_ALT_SPLIT 1
@@ -261,8 +266,8 @@ int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, v
 
 /* Selects the variant and the per-launch-group workspace budget (bytes of
  * intermediate per group of images; 0 = the default 136 MiB; a group holds
- * at least one image) used from now on for `op` (ADRT_HIP_OP_ADRT or
- * ADRT_HIP_OP_BDRT) at side n.  The setting is process-wide (all devices
+ * at least one image) used from now on for `op` (ADRT_HIP_OP_ADRT,
+ * ADRT_HIP_OP_BDRT or ADRT_HIP_OP_IADRT) at side n.  The setting is process-wide (all devices
  * and threads) and changes adrt_hip_workspace_bytes for that (op, n):
  * query the workspace after configuring.  A launch that meets a workspace
  * sized for another configuration returns ADRT_HIP_ERR_WORKSPACE.  Used by

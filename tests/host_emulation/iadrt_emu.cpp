@@ -1,11 +1,14 @@
-// Host emulation of the iadrt stage kernel (adrt_amd/csrc/iadrt.hip).
+// Host emulation of the iadrt stage and fused-tail kernels
+// (adrt_amd/csrc/iadrt.hip).
 //
 // Runs every wavefront of every stage on the CPU with the kernel's own
 // per-lane step function: lanes advance in lockstep, step by step, through a
 // 64-row ring exactly sized as the kernel's LDS ring (poisoned with NaN), and
 // row s is flushed at step s with the kernel's unconditional stores.  Pair loads use the kernel's clamped rows.
 // Buffers are exactly sized, so an out-of-bounds access aborts under
-// AddressSanitizer.  Usage:  iadrt_emu f32|f64 B n in.bin out.bin
+// AddressSanitizer.  Usage:  iadrt_emu f32|f64 B n in.bin out.bin [tail_stages]
+// (tail_stages: how many final stages the fused tail runs; default: the
+// library's choice; 0: none).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -61,8 +64,104 @@ static void emulate_stage(const std::vector<T> &in, std::vector<T> &out, int pla
     }
 }
 
+// The fused tail kernel: the last S stages, the first with Li0 groups.  Lanes
+// run in lockstep, in the kernel's order within a step: every later stage's
+// ring reads, then stage 0, then the later stages' results.  The ring buffer
+// has exactly the kernel's rows, and every slot carries the row it holds:
+// every ring read that enters a result checks that it reads the row it wants
+// (a stale or unwritten slot aborts).
 template <typename T>
-static int run(int B, int n, const char *fin, const char *fout) {
+static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int planes, int R, int n,
+                         int Li0, int S) {
+    const int waves = (n + kLanes - 1) / kLanes;
+    const int P = kLanes + 1;
+    const int M0 = tail_ring_rows(S, 0);
+    std::vector<T> rbuf(size_t(tail_rows_total(S)) * P);
+    std::vector<int> tag(rbuf.size());
+    auto check = [&](int base, int M, int row, int col) {
+        if (tag.at(size_t(base + (row & (M - 1))) * P + col) != row) {
+            std::fprintf(stderr, "stale ring read: ring base %d row %d col %d\n", base, row, col);
+            std::abort();
+        }
+    };
+    for (int plane = 0; plane < planes; ++plane) {
+        const T *src = in.data() + size_t(plane) * R * n;
+        T *dst = out.data() + size_t(plane) * R * n;
+        for (int wv = 0; wv < waves; ++wv) {
+            const Wave w = make_wave(R, n, Li0, wv);
+            if (w.c0 != 0 || w.maxlag != w.Ci / 2 - 1 || w.Ci != (1 << S)) std::abort();
+            Lane ln[kLanes];
+            std::vector<TailLane> tl(size_t(S) * kLanes);
+            T acc0[kLanes], prev1[kLanes];
+            std::vector<T> acc(size_t(S) * kLanes, T(0));
+            for (int l = 0; l < kLanes; ++l) {
+                ln[l] = make_lane(w, l);
+                acc0[l] = T(0);
+                prev1[l] = T(0);
+                for (int t = 1; t < S; ++t)
+                    tl[size_t(t) * kLanes + l] = make_tail_lane(n, n / (Li0 << t), w.ch0, l);
+            }
+            for (auto &x : rbuf) x = T(NAN);
+            for (auto &x : tag) x = -1 << 20;
+            for (int s = w.top; s >= -(S - 1); --s) {
+                std::vector<TailOperands> ops(size_t(S) * kLanes);
+                std::vector<T> va(size_t(S) * kLanes), vb(size_t(S) * kLanes);
+                for (int t = 1; t < S; ++t) {
+                    const int x = s + t;
+                    const int base = tail_ring_base(S, t - 1), M = tail_ring_rows(S, t - 1);
+                    for (int l = 0; l < kLanes; ++l) {
+                        const TailLane &u = tl[size_t(t) * kLanes + l];
+                        if (u.pc < 0 || u.pc + 1 > kLanes - 1) std::abort();
+                        const TailOperands o = tail_operands(u, x, R, base, M - 1);
+                        const size_t k = size_t(t) * kLanes + l;
+                        ops[k] = o;
+                        va[k] = rbuf.at(size_t(o.ia));
+                        vb[k] = rbuf.at(size_t(o.ib));
+                        if (!u.live || x < 0 || x >= R) continue;
+                        const int ro = x + 1 + u.col;
+                        if (o.use_a) check(base, M, u.odd ? ro : x, u.pc + (u.odd ? 1 : 0));
+                        if (o.use_b) check(base, M, u.odd ? ro : x + 1, u.pc + (u.odd ? 0 : 1));
+                    }
+                }
+                for (int l = 0; l < kLanes; ++l) {
+                    int r = s + ln[l].roff;
+                    r = r < 0 ? 0 : (r > R - 1 ? R - 1 : r);
+                    const size_t at = size_t(r) * n + (ln[l].live ? ln[l].pc : 0);
+                    const T p0 = src[at], p1 = src[at + 1];
+                    T val;
+                    int row;
+                    if (step<T>(w, ln[l], s, p0, p1, prev1[l], acc0[l], val, row)) {
+                        rbuf.at(size_t(row & (M0 - 1)) * P + l) = val;
+                        tag.at(size_t(row & (M0 - 1)) * P + l) = row;
+                    }
+                    prev1[l] = p1;
+                }
+                for (int t = 1; t < S; ++t) {
+                    const int x = s + t;
+                    if (x < 0 || x >= R) continue;
+                    const bool last = t == S - 1;
+                    for (int l = 0; l < kLanes; ++l) {
+                        const size_t k = size_t(t) * kLanes + l;
+                        T &a = acc[k];
+                        const T v = tail_combine<T>(ops[k], va[k], vb[k], a, x + 1 < R);
+                        a = v;
+                        if (!last) {
+                            const size_t at =
+                                size_t(tail_ring_base(S, t) + (x & (tail_ring_rows(S, t) - 1))) * P + l;
+                            rbuf.at(at) = v;
+                            tag.at(at) = x;
+                        } else if (tl[k].live) {
+                            dst[size_t(x) * n + w.ch0 + l] = v;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+static int run(int B, int n, const char *fin, const char *fout, int tail) {
     const int R = 2 * n - 1;
     const size_t elems = size_t(B) * 4 * R * n;
     std::vector<T> a(elems), b(elems, T(NAN));
@@ -70,12 +169,19 @@ static int run(int B, int n, const char *fin, const char *fout) {
     if (!f || std::fread(a.data(), sizeof(T), elems, f) != elems) return 2;
     std::fclose(f);
     const int L = num_iters(size_t(n));
+    // tail < 0: the library's choice; 0 or 1: no fused tail
+    const int S = tail < 0 ? tail_stages<T>(L) : (tail < L ? tail : L);
+    const int k0 = L - (S >= 2 ? S : 0);
     int Li = 1;
-    for (int k = 0; k < L; ++k) {
+    for (int k = 0; k < k0; ++k) {
         emulate_stage<T>(a, b, B * 4, R, n, Li);
         std::swap(a, b);
         for (auto &x : b) x = T(NAN);
         Li *= 2;
+    }
+    if (S >= 2) {
+        emulate_tail<T>(a, b, B * 4, R, n, Li, S);
+        std::swap(a, b);
     }
     f = std::fopen(fout, "wb");
     if (!f || std::fwrite(a.data(), sizeof(T), elems, f) != elems) return 4;
@@ -84,11 +190,12 @@ static int run(int B, int n, const char *fin, const char *fout) {
 }
 
 int main(int argc, char **argv) {
-    if (argc != 6) {
-        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin\n", argv[0]);
+    if (argc != 6 && argc != 7) {
+        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [tail_stages]\n", argv[0]);
         return 1;
     }
     const int B = std::atoi(argv[2]), n = std::atoi(argv[3]);
-    if (std::strcmp(argv[1], "f32") == 0) return run<float>(B, n, argv[4], argv[5]);
-    return run<double>(B, n, argv[4], argv[5]);
+    const int tail = argc == 7 ? std::atoi(argv[6]) : -1;
+    if (std::strcmp(argv[1], "f32") == 0) return run<float>(B, n, argv[4], argv[5], tail);
+    return run<double>(B, n, argv[4], argv[5], tail);
 }

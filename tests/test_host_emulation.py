@@ -165,3 +165,25 @@ def test_iadrt_emulation_bytes_equal(iadrt_emulator, tmp_path, dt, n):
     assert res.returncode == 0, res.stderr[-4000:]
     got = np.fromfile(fout, dtype=dt).reshape(x.shape)
     assert np.array_equal(got.view(np.uint8), O.iadrt(x).view(np.uint8))
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+@pytest.mark.parametrize("n,tail", [(4, 2), (8, 3), (32, 5), (64, 2), (64, 4), (64, 6),
+                                    (128, 3), (128, 6), (256, 5), (256, 6), (256, 0)])
+def test_iadrt_fused_tail_emulation(iadrt_emulator, tmp_path, dt, n, tail):
+    """The fused tail (the last `tail` stages in one pass: lagged first
+    stage, LDS rings between stages, each later stage one row behind) run on
+    the CPU with every ring read checked against the row it must hold, equals
+    the oracle byte for byte, for every tail length either dtype allows."""
+    rng = np.random.default_rng(n + tail)
+    b = 2 if n <= 16 else 1
+    x = (rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)
+    x.flat[:3] = [-0.0, 0.0, -0.0]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(fin)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([iadrt_emulator, "f32" if dt == "float32" else "f64", str(b), str(n),
+                          str(fin), str(fout), str(tail)], capture_output=True, text=True, env=env)
+    assert res.returncode == 0, res.stderr[-4000:]
+    got = np.fromfile(fout, dtype=dt).reshape(x.shape)
+    assert np.array_equal(got.view(np.uint8), O.iadrt(x).view(np.uint8))

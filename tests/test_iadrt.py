@@ -74,6 +74,32 @@ class TestIAdrtCompute:
         y = rng.standard_normal((b, 4, 2 * n - 1, n)).astype(dt)
         assert_bitwise(adrt_amd.iadrt(y), O.iadrt(y), f"iadrt {dt} n={n}")
 
+    @pytest.mark.parametrize("dt", ["float32", "float64"])
+    @pytest.mark.parametrize("n", [2, 8, 64, 256, 2048])
+    def test_every_variant_bitwise(self, cuda, dt, n):
+        """The fused tail (default and half length) and the one-kernel-per-
+        stage path give the oracle's bytes, on device tensors."""
+        import torch
+        from adrt_amd import _kernels, _lib, batched
+
+        rng = np.random.default_rng(n + 11)
+        b = 1 if n == 2048 else 3
+        y = rng.standard_normal((b, 4, 2 * n - 1, n)).astype(dt)
+        y.flat[:2] = [-0.0, 0.0]
+        want = O.iadrt(y) if n < 2048 else None
+        yd = torch.from_numpy(y).to(cuda)
+        outs = []
+        try:
+            for v in (_lib.VARIANT_DEFAULT, _lib.VARIANT_ALT_SPLIT, _lib.VARIANT_STREAMING):
+                batched.set_config(_lib.OP_IADRT, n, v)
+                outs.append(_kernels.iadrt(yd).cpu().numpy())
+        finally:
+            batched.set_config(_lib.OP_IADRT, n, _lib.VARIANT_DEFAULT)
+        for o in outs[1:]:
+            assert o.tobytes() == outs[0].tobytes()
+        if want is not None:
+            assert_bitwise(outs[0], want, f"iadrt {dt} n={n}")
+
     def test_integer_round_trip_exact(self, cuda):
         x = np.random.default_rng(5).integers(-9, 9, size=(3, 64, 64)).astype(np.float64)
         inv = adrt_amd.utils.truncate(adrt_amd.iadrt(adrt_amd.adrt(x))).mean(axis=-3)
