@@ -102,6 +102,71 @@ __host__ __device__ inline bool step(const Wave &w, const Lane &ln, int s, T p0,
     return true;
 }
 
+// Paired stages (Co >= 32).  Output chain (l odd, col) reads its pair at
+// source row r + 1 + col; its rows r >= R - 1 - col have no source term and
+// are 0 + 0 = +0.  Indexing the odd chain by t = R - 2 - col - r, it reads
+// source row R - 1 - t: the row the even chain (l - 1, col) reads at the
+// same t (output row R - 1 - t).  So a wave holds the 32 even chains of
+// columns c0 .. c0 + 31 of group 2pl in lanes 0-31 and the 32 odd chains of
+// the same columns of group 2pl + 1 in lanes 32-63, and at step t every lane
+// loads its pair from source row R - 1 - t: lanes L and L + 32 load the same
+// 8 bytes, so each pair leaves memory once per stage.  Even lanes store their
+// row directly (one 128-byte row segment per step); an odd lane's row lags
+// by col - c0, so odd rows pass through a 32-row LDS ring of each lane's own
+// column and row f = R - 2 - c0 - t is stored whole at step t.  The odd
+// chains' +0 rows above R - 2 - c0 are filled after the sweep; the ring
+// starts at +0 for the rows a lane skips before its first source row.
+constexpr int kHalf = 32;
+// Steps per prefetch block of the paired kernel: 16 pair loads in flight per
+// lane (and up to 32 stores issued between a block's loads and their use,
+// within the 63 outstanding memory operations the counter tracks).
+constexpr int kPairSteps = 16;
+
+struct PairLane {
+    bool odd;
+    int col;  // column within the group
+    int pc;   // source column of the (2col, 2col+1) pair
+    int oc;   // output column
+};
+
+// wave_in_plane -> the lane's chain; c0 = the wave's first column in its group
+__host__ __device__ inline PairLane make_pair_lane(int n, int Li, int wave_in_plane, int lane,
+                                                   int &c0) {
+    const int Ci = n / Li, Co = Ci / 2;
+    const int wpp = Co / kHalf;  // waves per pair of groups
+    const int pl = wave_in_plane / wpp;
+    c0 = (wave_in_plane % wpp) * kHalf;
+    PairLane p;
+    p.odd = lane >= kHalf;
+    p.col = c0 + (lane & (kHalf - 1));
+    p.pc = pl * Ci + 2 * p.col;
+    p.oc = (2 * pl + (p.odd ? 1 : 0)) * Co + p.col;
+    return p;
+}
+
+// Step at source row y = R - 1 - t: even lanes produce row y, odd lanes row
+// y - 1 - col (if >= 0), in the reference's order of additions.
+template <typename T>
+__host__ __device__ inline bool pair_step(const PairLane &pp, int y, int R, T p0, T p1, T prev1,
+                                          T &acc, T &val, int &row) {
+    T v = T(0);
+    if (!pp.odd) {
+        row = y;
+        v += p0;                          // src[2col][row]
+        if (y + 1 < R) v -= prev1;        // src[2col+1][row+1]
+        if (y + 1 < R) v += acc;
+    } else {
+        row = y - 1 - pp.col;
+        if (row < 0) return false;
+        v += p1;                          // src[2col+1][row+1+col], row+1+col = y < R
+        v -= p0;                          // src[2col][row+1+col]
+        v += acc;                         // row + 1 < R
+    }
+    acc = v;
+    val = v;
+    return true;
+}
+
 // Fused tail.  Once a stage's input groups are at most 64 columns wide
 // (Ci <= 64), one wavefront's 64 columns hold whole groups of that stage and
 // of every later one, so the last S stages run in one pass: the first of
@@ -264,6 +329,66 @@ __global__ void __launch_bounds__(kLanes)
         if (k <= s) run_step(s - k, cur[k], acc, prev1);
 }
 
+// A paired stage (Co >= 32): grid = planes * (n / 64) single-wave
+// work-groups.
+template <typename T>
+__global__ void __launch_bounds__(kLanes)
+    pair_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
+                int waves_per_plane) {
+    using P2 = typename Pair<T>::type;
+    __shared__ T ring[kHalf][kHalf];  // [row & 31][odd lane]: bank = lane
+    const int lane = threadIdx.x;
+    const int hl = lane & (kHalf - 1);
+    const size_t plane = blockIdx.x / waves_per_plane;
+    int c0;
+    const PairLane pp = make_pair_lane(n, Li, int(blockIdx.x % waves_per_plane), lane, c0);
+    const T *src = in + plane * size_t(R) * n + pp.pc;
+    T *dst = out + plane * size_t(R) * n + pp.oc;
+    if (pp.odd)
+        for (int r = 0; r < kHalf; ++r) ring[r][hl] = T(0);
+
+    auto load = [&](int t) -> P2 {
+        const int y = R - 1 - t;
+        return *reinterpret_cast<const P2 *>(src + size_t(y < 0 ? 0 : y) * n);
+    };
+    T acc = T(0), prev1 = T(0);
+    auto run_step = [&](int t, const P2 &p) {
+        const int y = R - 1 - t;
+        T val;
+        int row;
+        if (pair_step<T>(pp, y, R, p.x, p.y, prev1, acc, val, row)) {
+            if (!pp.odd)
+                dst[size_t(row) * n] = val;
+            else
+                ring[row & (kHalf - 1)][hl] = val;
+        }
+        prev1 = p.y;
+        if (pp.odd) {
+            const int f = y - 1 - c0;  // every odd lane has produced row f
+            if (f >= 0) dst[size_t(f) * n] = ring[f & (kHalf - 1)][hl];
+        }
+    };
+
+    P2 cur[kPairSteps];
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < kPairSteps; ++k) cur[k] = load(k);
+    for (; t + kPairSteps <= R; t += kPairSteps) {  // whole blocks
+        P2 nxt[kPairSteps];
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) nxt[k] = load(t + kPairSteps + k);
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) run_step(t + k, cur[k]);
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) cur[k] = nxt[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kPairSteps; ++k)  // tail: steps t .. R-1
+        if (t + k < R) run_step(t + k, cur[k]);
+    if (pp.odd)  // the odd chains' +0 rows
+        for (int r = R - 1 - c0; r < R; ++r) dst[size_t(r) * n] = T(0);
+}
+
 // The last S stages (the first with Li0 groups, Ci = n / Li0 = 2^S), one
 // single-wave work-group per 64 columns of a plane.
 template <typename T, int S>
@@ -387,8 +512,14 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
     int Li = 1;
     for (int k = 0; k < k0; ++k) {
         T *dst = bufs[k & 1];
-        hipLaunchKernelGGL(iadrt_native::stage_kernel<T>, dim3(grid), dim3(iadrt_native::kLanes),
-                           0, s, src, dst, int(R), int(n), Li, waves_per_plane);
+        if (int(n) / Li / 2 >= iadrt_native::kHalf)  // Co >= 32: even and odd chains paired
+            hipLaunchKernelGGL(iadrt_native::pair_kernel<T>, dim3(grid),
+                               dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n), Li,
+                               waves_per_plane);
+        else
+            hipLaunchKernelGGL(iadrt_native::stage_kernel<T>, dim3(grid),
+                               dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n), Li,
+                               waves_per_plane);
         src = dst;
         Li *= 2;
     }

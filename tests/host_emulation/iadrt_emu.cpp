@@ -64,6 +64,67 @@ static void emulate_stage(const std::vector<T> &in, std::vector<T> &out, int pla
     }
 }
 
+// The paired stage kernel (Co >= 32): lanes in lockstep, the odd lanes'
+// 32-row rings of their own columns, tagged with the row each slot holds
+// (the +0 start is its own tag); every flush checks that it reads its row,
+// or a +0 row the lane skipped.
+template <typename T>
+static void emulate_pair_stage(const std::vector<T> &in, std::vector<T> &out, int planes, int R,
+                               int n, int Li) {
+    const int waves = n / kLanes;
+    const int kZero = -(1 << 30);
+    for (int plane = 0; plane < planes; ++plane) {
+        const T *src = in.data() + size_t(plane) * R * n;
+        T *dst = out.data() + size_t(plane) * R * n;
+        for (int wv = 0; wv < waves; ++wv) {
+            PairLane pp[kLanes];
+            int c0 = 0;
+            for (int l = 0; l < kLanes; ++l) pp[l] = make_pair_lane(n, Li, wv, l, c0);
+            std::vector<T> ring(size_t(kHalf) * kHalf, T(0));
+            std::vector<int> tag(ring.size(), kZero);
+            T acc[kLanes], prev1[kLanes];
+            for (int l = 0; l < kLanes; ++l) {
+                acc[l] = T(0);
+                prev1[l] = T(0);
+                if (pp[l].pc < 0 || pp[l].pc + 1 >= n || pp[l].oc < 0 || pp[l].oc >= n)
+                    std::abort();
+            }
+            for (int t = 0; t < R; ++t) {
+                const int y = R - 1 - t;
+                for (int l = 0; l < kLanes; ++l) {
+                    const size_t at = size_t(y) * n + pp[l].pc;
+                    const T p0 = src[at], p1 = src[at + 1];
+                    T val;
+                    int row;
+                    if (pair_step<T>(pp[l], y, R, p0, p1, prev1[l], acc[l], val, row)) {
+                        if (!pp[l].odd) {
+                            dst[size_t(row) * n + pp[l].oc] = val;
+                        } else {
+                            const size_t slot = size_t(row & (kHalf - 1)) * kHalf + (l & (kHalf - 1));
+                            ring[slot] = val;
+                            tag[slot] = row;
+                        }
+                    }
+                    prev1[l] = p1;
+                }
+                const int f = y - 1 - c0;
+                if (f < 0) continue;
+                for (int l = kHalf; l < kLanes; ++l) {
+                    const size_t slot = size_t(f & (kHalf - 1)) * kHalf + (l & (kHalf - 1));
+                    const bool skipped = tag[slot] == kZero && f > R - 2 - pp[l].col;
+                    if (tag[slot] != f && !skipped) {
+                        std::fprintf(stderr, "stale pair flush: row %d lane %d\n", f, l);
+                        std::abort();
+                    }
+                    dst[size_t(f) * n + pp[l].oc] = ring[slot];
+                }
+            }
+            for (int l = kHalf; l < kLanes; ++l)
+                for (int r = R - 1 - c0; r < R; ++r) dst[size_t(r) * n + pp[l].oc] = T(0);
+        }
+    }
+}
+
 // The fused tail kernel: the last S stages, the first with Li0 groups.  Lanes
 // run in lockstep, in the kernel's order within a step: every later stage's
 // ring reads, then stage 0, then the later stages' results.  The ring buffer
@@ -174,7 +235,10 @@ static int run(int B, int n, const char *fin, const char *fout, int tail) {
     const int k0 = L - (S >= 2 ? S : 0);
     int Li = 1;
     for (int k = 0; k < k0; ++k) {
-        emulate_stage<T>(a, b, B * 4, R, n, Li);
+        if (n / Li / 2 >= kHalf)
+            emulate_pair_stage<T>(a, b, B * 4, R, n, Li);
+        else
+            emulate_stage<T>(a, b, B * 4, R, n, Li);
         std::swap(a, b);
         for (auto &x : b) x = T(NAN);
         Li *= 2;

@@ -169,12 +169,17 @@ def test_iadrt_emulation_bytes_equal(iadrt_emulator, tmp_path, dt, n):
 
 @pytest.mark.parametrize("dt", ["float32", "float64"])
 @pytest.mark.parametrize("n,tail", [(4, 2), (8, 3), (32, 5), (64, 2), (64, 4), (64, 6),
-                                    (128, 3), (128, 6), (256, 5), (256, 6), (256, 0)])
+                                    (128, 3), (128, 6), (256, 5), (256, 6), (256, 0),
+                                    (1024, 5), (1024, 0)])
 def test_iadrt_fused_tail_emulation(iadrt_emulator, tmp_path, dt, n, tail):
     """The fused tail (the last `tail` stages in one pass: lagged first
     stage, LDS rings between stages, each later stage one row behind) run on
     the CPU with every ring read checked against the row it must hold, equals
-    the oracle byte for byte, for every tail length either dtype allows."""
+    the oracle byte for byte, for every tail length either dtype allows.  At
+    n = 1024 the planes have 16 waves, so the early stages run with the
+    paired (XCD) block order, checked to be a bijection."""
+    if n == 1024 and dt == "float64":
+        pytest.skip("one dtype suffices for the block order at this size")
     rng = np.random.default_rng(n + tail)
     b = 2 if n <= 16 else 1
     x = (rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)
