@@ -184,6 +184,7 @@ __host__ __device__ inline bool pair_step(const PairLane &pp, int y, int R, T p0
 // s + t + 1 + Co_{t+1}: Co_{t+1} + 2 rows.  The first stage of an S-stage
 // tail has Ci = 2^S columns per group.
 constexpr int kMaxTail = 6;  // stages 64, 32, .., 2 columns wide
+constexpr int kTailSteps = 16;  // steps per prefetch block (1 load, <= 1 store a step)
 
 __host__ __device__ constexpr int pow2_at_least(int v) {
     int p = 1;
@@ -454,21 +455,21 @@ __global__ void __launch_bounds__(kLanes)
     // steps w.top .. -(S-1): the last stage reaches row 0 at step -(S-1);
     // loads for steps below 0 are clamped into the rows and unused
     const int s_end = -(S - 1);
-    P2 cur[kSteps];
+    P2 cur[kTailSteps];
     int s = w.top;
 #pragma unroll
-    for (int k = 0; k < kSteps; ++k) cur[k] = load(s - k);
-    for (; s - (kSteps - 1) >= s_end; s -= kSteps) {  // whole blocks
-        P2 nxt[kSteps];
+    for (int k = 0; k < kTailSteps; ++k) cur[k] = load(s - k);
+    for (; s - (kTailSteps - 1) >= s_end; s -= kTailSteps) {  // whole blocks
+        P2 nxt[kTailSteps];
 #pragma unroll
-        for (int k = 0; k < kSteps; ++k) nxt[k] = load(s - kSteps - k);
+        for (int k = 0; k < kTailSteps; ++k) nxt[k] = load(s - kTailSteps - k);
 #pragma unroll
-        for (int k = 0; k < kSteps; ++k) run_step(s - k, cur[k]);
+        for (int k = 0; k < kTailSteps; ++k) run_step(s - k, cur[k]);
 #pragma unroll
-        for (int k = 0; k < kSteps; ++k) cur[k] = nxt[k];
+        for (int k = 0; k < kTailSteps; ++k) cur[k] = nxt[k];
     }
 #pragma unroll
-    for (int k = 0; k < kSteps; ++k)  // tail: steps s .. s_end
+    for (int k = 0; k < kTailSteps; ++k)  // tail: steps s .. s_end
         if (s - k >= s_end) run_step(s - k, cur[k]);
 }
 #endif  // ADRT_HOST_EMULATION
