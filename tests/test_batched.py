@@ -40,7 +40,7 @@ def test_candidates_budgets_distinct_groups():
 @pytest.fixture
 def reset_config():
     yield
-    for op in (_lib.OP_ADRT, _lib.OP_BDRT):
+    for op in (_lib.OP_ADRT, _lib.OP_BDRT, _lib.OP_IADRT):
         for L in range(0, 13):
             batched.set_config(op, 1 << L, _lib.VARIANT_DEFAULT, 0)
 
@@ -53,10 +53,18 @@ class TestConfig:
         batched.set_config(_lib.OP_ADRT, 128, _lib.VARIANT_DEFAULT, 0)
         assert batched.get_config(_lib.OP_ADRT, 128) == (_lib.VARIANT_DEFAULT, 136 * MIB)
 
+    def test_iadrt_variants_roundtrip(self, cuda, reset_config):
+        """The inverse's fused-tail variants are set per size like the
+        forward's (the group budget does not apply to it)."""
+        for v in (_lib.VARIANT_STREAMING, _lib.VARIANT_ALT_SPLIT, _lib.VARIANT_DEFAULT):
+            batched.set_config(_lib.OP_IADRT, 32, v, 0)
+            assert batched.get_config(_lib.OP_IADRT, 32)[0] == v
+
     def test_invalid_config(self, cuda, reset_config):
         lib = _lib.load()
         assert lib.adrt_hip_set_config(_lib.OP_ADRT, 32, _lib.VARIANT_ALT_SPLIT, 0) == _lib.ERR_ARG
-        assert lib.adrt_hip_set_config(_lib.OP_IADRT, 64, 0, 0) == _lib.ERR_ARG
+        assert lib.adrt_hip_set_config(7, 64, 0, 0) == _lib.ERR_ARG
+        assert lib.adrt_hip_set_config(_lib.OP_IADRT, 48, 0, 0) == _lib.ERR_ARG
         assert lib.adrt_hip_set_config(_lib.OP_ADRT, 96, 0, 0) == _lib.ERR_ARG
         assert lib.adrt_hip_set_config(_lib.OP_ADRT, 64, 3, 0) == _lib.ERR_ARG
 
