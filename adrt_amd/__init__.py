@@ -23,9 +23,11 @@ __version__ = "0.1.0"
 
 def _residual_norm(x, d_a):
     """||adrt(x) - a||_2: difference in the data dtype, norm accumulated in
-    float64 on the device."""
-    diff = _kernels.adrt(x) - d_a
-    return float(diff.double().square().sum().sqrt())
+    float64 on the device (the batched dot kernel, whose per-image sums do
+    not depend on the batch, so core.iadrt_fmg_batched decides each image
+    exactly as this function does)."""
+    diff = (_kernels.adrt(x) - d_a).reshape(1, -1)
+    return float(_kernels.batched_dot(diff, diff)[0].sqrt())
 
 
 def iadrt_fmg(a, /, *, max_iters=None):

@@ -30,6 +30,7 @@ __all__ = [
     "threading_enabled",
     "iadrt_fmg_step",
     "iadrt_fmg_iter",
+    "iadrt_fmg_batched",
 ]
 
 
@@ -169,3 +170,44 @@ def iadrt_fmg_iter(a, /, *, copy=True):
     d_a, restore = dev.to_device(arr)
     for inv in _fmg_iter_device(d_a):
         yield _freeze(restore(inv), copy)
+
+
+def iadrt_fmg_batched(a, /, *, max_iters=None):
+    r"""Batched :func:`adrt_amd.iadrt_fmg`: ``(B, 4, 2N-1, N)`` ->
+    ``(B, N, N)``.
+
+    An extension: the reference's ``iadrt_fmg`` takes one image
+    (``__init__.py:122-125``).  Every image follows that function's rule on
+    its own: its FMG estimates are refined (``iadrt_fmg_iter``) until its
+    residual norm ``||adrt(x) - a||`` fails to decrease, or ``max_iters``
+    estimates were made, and the last estimate whose residual decreased is
+    returned.  The refinement runs on the whole batch on the GPU until every
+    image has stopped; image ``b`` of the result equals ``iadrt_fmg(a[b])``
+    bit for bit (the estimates and the float64 residual norms of an image do
+    not depend on the rest of the batch).
+    """
+    import itertools
+
+    if a.ndim != 4:
+        raise ValueError(f"array must have 4 dimensions (a batch), but had {a.ndim}")
+    if max_iters is not None and max_iters < 1:
+        raise ValueError(f"must allow at least one iteration, but specified {max_iters}")
+    arr, _ = _fmg_validate(a)
+    d_a, restore = dev.to_device(arr)
+    torch = dev.torch()
+    B = d_a.shape[0]
+    best = best_res = None
+    done = torch.zeros(B, dtype=torch.bool, device=d_a.device)
+    for x in itertools.islice(_fmg_iter_device(d_a), max_iters):
+        diff = (_kernels.adrt(x) - d_a).reshape(B, -1)
+        res = _kernels.batched_dot(diff, diff).sqrt()
+        if best is None:
+            best, best_res = x.clone(), res
+            continue
+        better = ~done & (res < best_res)
+        done |= ~better
+        best[better] = x[better]
+        best_res = torch.where(better, res, best_res)
+        if bool(done.all()):
+            break
+    return restore(best)

@@ -8,8 +8,9 @@
 //
 // with a_b, c_b the float64 scalars rounded to the element type, so each
 // update is the same two roundings as the unfused expression x + a * p.
-// Sums are deterministic: each block reduces a fixed chunk in a fixed order
-// and a second pass adds the chunks of an image in order.
+// Sums are deterministic and independent of the batch size: each block
+// reduces a fixed 16 K-element chunk of one image in a fixed order and a
+// second pass adds the chunks of an image in order.
 //
 // The pcg_* entry points run a whole (preconditioned) CG iteration's scalar
 // logic on the device, over a per-image state buffer, so that one iteration
@@ -115,15 +116,17 @@ struct Split {
     int chunks;
 };
 
-// About 2048 blocks in all, each chunk at least 16 K elements.
+// Chunks of 16 K elements whatever the batch size, so an image's sums (and
+// everything computed from them) do not depend on the other images in the
+// batch: a solver that drops converged images, or a caller that splits a
+// batch, gets the same values bit for bit.
+constexpr size_t kChunk = 16384;
+
 inline Split split(size_t B, size_t m) {
-    size_t want = (2048 + B - 1) / B;
-    size_t chunk = (m + want - 1) / want;
-    if (chunk < 16384) chunk = 16384;
-    chunk = (chunk + kThreads - 1) / kThreads * kThreads;
+    (void)B;
     Split s;
-    s.chunk = chunk;
-    s.chunks = int((m + chunk - 1) / chunk);
+    s.chunk = kChunk;
+    s.chunks = int((m + kChunk - 1) / kChunk);
     return s;
 }
 

@@ -205,7 +205,9 @@ int adrt_hip_truncate_combine(const void *in, void *out, int dtype, size_t batch
 
 /* ---- batched CG vector operations (no reference counterpart) ------------- *
  * For B independent systems of m unknowns (C-contiguous, B <= 65535), with
- * float64 scalars per system and deterministic float64 sums:
+ * float64 scalars per system and deterministic float64 sums (fixed
+ * 16384-element chunks: a system's sums do not depend on B or on the other
+ * systems):
  *   adrt_hip_batched_dot:  out[b] = sum_i u[b][i] * v[b][i]
  *   adrt_hip_cg_update:    x += a_b p, r -= a_b q (a_b = alpha[b] rounded to
  *                          the element type), rr[b] = sum_i r[b][i]^2 after

@@ -199,3 +199,43 @@ def test_fmg_iter_batch(cuda, dt):
     ref = list(itertools.islice(O.iadrt_fmg_iter(a), 3))
     for x, r in zip(its, ref):
         assert_bitwise(x, r)
+
+
+def test_iadrt_fmg_batched_rejects():
+    with pytest.raises(ValueError, match="4 dimensions"):
+        adrt_amd.core.iadrt_fmg_batched(np.zeros((4, 15, 8), dtype="float32"))
+    with pytest.raises(ValueError, match="at least one iteration"):
+        adrt_amd.core.iadrt_fmg_batched(np.zeros((2, 4, 15, 8), dtype="float32"), max_iters=0)
+    with pytest.raises(ValueError, match="valid ADRT output shape"):
+        adrt_amd.core.iadrt_fmg_batched(np.zeros((2, 4, 14, 8), dtype="float32"))
+
+
+@gpu
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+@pytest.mark.parametrize("n,max_iters", [(16, None), (64, None), (64, 2), (8, 1)])
+def test_iadrt_fmg_batched_equals_per_image(cuda, dt, n, max_iters):
+    """Each image of the batched FMG inverse is the unbatched iadrt_fmg of
+    that image, bit for bit (its own stopping point included); images that
+    stop at different iterations are mixed in one batch."""
+    rng = np.random.default_rng(n)
+    imgs = rng.random((4, n, n)).astype(dt)
+    imgs[1] = 0.5                              # a flat image stops at once
+    imgs[2] = np.round(imgs[2] * 8)            # an integer image
+    a = adrt_amd.adrt(imgs)
+    a[3] += rng.standard_normal(a[3].shape).astype(dt) * 0.01  # inconsistent data
+    got = adrt_amd.core.iadrt_fmg_batched(a, max_iters=max_iters)
+    assert got.shape == (4, n, n) and got.dtype == np.dtype(dt)
+    for b in range(4):
+        assert_bitwise(got[b], adrt_amd.iadrt_fmg(a[b], max_iters=max_iters), f"image {b}")
+
+
+@gpu
+def test_iadrt_fmg_batched_device_tensors(cuda):
+    """Device tensors in, device tensor out, same values as NumPy in."""
+    import torch
+
+    imgs = np.random.default_rng(3).random((3, 32, 32)).astype("float32")
+    a = adrt_amd.adrt(imgs)
+    got = adrt_amd.core.iadrt_fmg_batched(torch.from_numpy(a).to(cuda))
+    assert got.is_cuda and tuple(got.shape) == (3, 32, 32)
+    assert_bitwise(got.cpu().numpy(), adrt_amd.core.iadrt_fmg_batched(a))
