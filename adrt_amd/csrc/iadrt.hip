@@ -117,9 +117,9 @@ __host__ __device__ inline bool step(const Wave &w, const Lane &ln, int s, T p0,
 // chains' +0 rows above R - 2 - c0 are filled after the sweep; the ring
 // starts at +0 for the rows a lane skips before its first source row.
 constexpr int kHalf = 32;
-// Steps per prefetch block of the paired kernel: 16 pair loads in flight per
-// lane (and up to 32 stores issued between a block's loads and their use,
-// within the 63 outstanding memory operations the counter tracks).
+// Steps per prefetch block of the paired kernel: pair loads in flight per
+// lane (with one store per step, a block's loads and stores stay within the
+// 63 outstanding memory operations the counter tracks).
 constexpr int kPairSteps = 16;
 
 struct PairLane {
@@ -357,17 +357,15 @@ __global__ void __launch_bounds__(kLanes)
         const int y = R - 1 - t;
         T val;
         int row;
-        if (pair_step<T>(pp, y, R, p.x, p.y, prev1, acc, val, row)) {
-            if (!pp.odd)
-                dst[size_t(row) * n] = val;
-            else
-                ring[row & (kHalf - 1)][hl] = val;
-        }
+        const bool made = pair_step<T>(pp, y, R, p.x, p.y, prev1, acc, val, row);
+        if (pp.odd && made) ring[row & (kHalf - 1)][hl] = val;
         prev1 = p.y;
-        if (pp.odd) {
-            const int f = y - 1 - c0;  // every odd lane has produced row f
-            if (f >= 0) dst[size_t(f) * n] = ring[f & (kHalf - 1)][hl];
-        }
+        // one store instruction per step: even lanes their row y, odd lanes
+        // row f = y - 1 - c0, which every odd lane has produced
+        const int f = y - 1 - c0;
+        const int orow = pp.odd ? f : y;
+        const T ov = pp.odd ? ring[f & (kHalf - 1)][hl] : val;
+        if (orow >= 0) dst[size_t(orow) * n] = ov;
     };
 
     P2 cur[kPairSteps];
