@@ -204,14 +204,24 @@ __host__ __device__ constexpr int tail_ring_base(int S, int t) {
 }
 __host__ __device__ constexpr int tail_rows_total(int S) { return tail_ring_base(S, S - 1); }
 
+// The operands of row x of a later stage's chain in its producer's ring
+// (rows base .. base + mask, pitch kLanes + 1):
+//   even: a = src[2col][x],          b = src[2col+1][x+1]  (b used if x+1 < R)
+//   odd:  a = src[2col+1][x+1+col],  b = src[2col][x+1+col] (both if x+1+col < R)
+// As per-lane constants: a sits at ring row x + offa, column ca, and enters
+// while x < lima (b likewise), so a step computes each index with one add,
+// one mask and one multiply-add and each condition with one compare.  Both
+// slots are read unconditionally (a ring slot exists for every row) and the
+// conditions are applied by selects, so a step has no branches.
 struct TailLane {
     bool live;
     bool odd;
     int col;
     int pc;  // ring column of the src pair (2col, 2col+1) of the group
+    int offa, offb, ca, cb, lima, limb;
 };
 
-__host__ __device__ inline TailLane make_tail_lane(int n, int Ci, int ch0, int lane) {
+__host__ __device__ inline TailLane make_tail_lane(int n, int Ci, int ch0, int lane, int R) {
     TailLane t;
     const int ch = ch0 + lane;
     t.live = ch < n;
@@ -221,30 +231,28 @@ __host__ __device__ inline TailLane make_tail_lane(int n, int Ci, int ch0, int l
     t.col = chc % Co;
     t.odd = (l & 1) != 0;
     t.pc = (l >> 1) * Ci + 2 * t.col - ch0;
+    t.offa = t.odd ? 1 + t.col : 0;
+    t.offb = t.odd ? 1 + t.col : 1;
+    t.ca = t.pc + (t.odd ? 1 : 0);
+    t.cb = t.pc + (t.odd ? 0 : 1);
+    t.lima = t.odd ? R - 1 - t.col : R;  // odd: x + 1 + col < R; even: always (x < R)
+    t.limb = t.odd ? R - 1 - t.col : R - 1;  // even: x + 1 < R
     return t;
 }
 
-// The operands of row x of a later stage's chain in its producer's ring
-// (rows base .. base + mask, pitch kLanes + 1):
-//   even: a = src[2col][x],          b = src[2col+1][x+1]  (b used if x+1 < R)
-//   odd:  a = src[2col+1][x+1+col],  b = src[2col][x+1+col] (both if x+1+col < R)
-// Both slots are read unconditionally (a ring slot exists for every row) and
-// the conditions are applied by selects, so a step has no branches.
 struct TailOperands {
     int ia, ib;   // ring element indices of a and b
     bool use_a;   // a enters (even: always)
     bool use_b;   // b enters
 };
 
-__host__ __device__ __forceinline__ TailOperands tail_operands(const TailLane &tl, int x, int R,
+__host__ __device__ __forceinline__ TailOperands tail_operands(const TailLane &tl, int x,
                                                                int base, int mask) {
     TailOperands o;
-    const int ro = x + 1 + tl.col;  // the odd chain's row
-    const int ra = tl.odd ? ro : x, rb = tl.odd ? ro : x + 1;
-    o.ia = (base + (ra & mask)) * (kLanes + 1) + tl.pc + (tl.odd ? 1 : 0);
-    o.ib = (base + (rb & mask)) * (kLanes + 1) + tl.pc + (tl.odd ? 0 : 1);
-    o.use_a = tl.odd ? ro < R : true;
-    o.use_b = tl.odd ? ro < R : x + 1 < R;
+    o.ia = (base + ((x + tl.offa) & mask)) * (kLanes + 1) + tl.ca;
+    o.ib = (base + ((x + tl.offb) & mask)) * (kLanes + 1) + tl.cb;
+    o.use_a = x < tl.lima;
+    o.use_b = x < tl.limb;
     return o;
 }
 
@@ -405,7 +413,7 @@ __global__ void __launch_bounds__(kLanes)
     T *dst = out + plane * size_t(R) * n + w.ch0 + lane;
     TailLane tl[S - 1];
 #pragma unroll
-    for (int t = 1; t < S; ++t) tl[t - 1] = make_tail_lane(n, n / (Li0 << t), w.ch0, lane);
+    for (int t = 1; t < S; ++t) tl[t - 1] = make_tail_lane(n, n / (Li0 << t), w.ch0, lane, R);
 
     auto load = [&](int s) -> P2 {
         int r = s + ln.roff;
@@ -424,7 +432,7 @@ __global__ void __launch_bounds__(kLanes)
         TailOperands o[S - 1];
 #pragma unroll
         for (int t = 1; t < S; ++t) {
-            o[t - 1] = tail_operands(tl[t - 1], sk + t, R, tail_ring_base(S, t - 1),
+            o[t - 1] = tail_operands(tl[t - 1], sk + t, tail_ring_base(S, t - 1),
                                      tail_ring_rows(S, t - 1) - 1);
             a[t - 1] = rbuf[o[t - 1].ia];
             b[t - 1] = rbuf[o[t - 1].ib];

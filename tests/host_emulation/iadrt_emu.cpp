@@ -160,7 +160,7 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
                 acc0[l] = T(0);
                 prev1[l] = T(0);
                 for (int t = 1; t < S; ++t)
-                    tl[size_t(t) * kLanes + l] = make_tail_lane(n, n / (Li0 << t), w.ch0, l);
+                    tl[size_t(t) * kLanes + l] = make_tail_lane(n, n / (Li0 << t), w.ch0, l, R);
             }
             for (auto &x : rbuf) x = T(NAN);
             for (auto &x : tag) x = -1 << 20;
@@ -173,7 +173,7 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
                     for (int l = 0; l < kLanes; ++l) {
                         const TailLane &u = tl[size_t(t) * kLanes + l];
                         if (u.pc < 0 || u.pc + 1 > kLanes - 1) std::abort();
-                        const TailOperands o = tail_operands(u, x, R, base, M - 1);
+                        const TailOperands o = tail_operands(u, x, base, M - 1);
                         const size_t k = size_t(t) * kLanes + l;
                         ops[k] = o;
                         va[k] = rbuf.at(size_t(o.ia));
