@@ -24,6 +24,8 @@
 // The lagging outputs go through a 64-row LDS ring per wave and row s is
 // stored, whole and coalesced, at step s, when every lane has produced it.
 // Pair loads are issued a block of kSteps steps ahead of their use.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace adrt_hip {
@@ -425,7 +427,11 @@ __global__ void __launch_bounds__(kLanes)
     T acc[S - 1];
 #pragma unroll
     for (int t = 0; t < S - 1; ++t) acc[t] = T(0);
-    auto run_step = [&](int sk, const P2 &p) {
+    // kChecked: some later stage's row sk + t may lie outside [0, R) (the
+    // first and last steps); a block of steps with every stage inside runs
+    // without the per-stage range branches
+    auto run_step = [&](int sk, const P2 &p, auto checked) {
+        constexpr bool kChecked = decltype(checked)::value;
         // every later stage's operands first (rows written at earlier steps),
         // so the step's LDS reads are issued together
         T a[S - 1], b[S - 1];
@@ -447,7 +453,7 @@ __global__ void __launch_bounds__(kLanes)
 #pragma unroll
         for (int t = 1; t < S; ++t) {
             const int x = sk + t;
-            if (x < 0 || x >= R) continue;
+            if (kChecked && (x < 0 || x >= R)) continue;
             const T v = tail_combine<T>(o[t - 1], a[t - 1], b[t - 1], acc[t - 1], x + 1 < R);
             acc[t - 1] = v;
             if (t < S - 1)
@@ -465,18 +471,25 @@ __global__ void __launch_bounds__(kLanes)
     int s = w.top;
 #pragma unroll
     for (int k = 0; k < kTailSteps; ++k) cur[k] = load(s - k);
+    using Checked = std::true_type;
+    using Unchecked = std::false_type;
     for (; s - (kTailSteps - 1) >= s_end; s -= kTailSteps) {  // whole blocks
         P2 nxt[kTailSteps];
 #pragma unroll
         for (int k = 0; k < kTailSteps; ++k) nxt[k] = load(s - kTailSteps - k);
+        if (s <= R - S && s - (kTailSteps - 1) >= 0) {  // rows s+1 .. s+S-1 inside
 #pragma unroll
-        for (int k = 0; k < kTailSteps; ++k) run_step(s - k, cur[k]);
+            for (int k = 0; k < kTailSteps; ++k) run_step(s - k, cur[k], Unchecked{});
+        } else {
+#pragma unroll
+            for (int k = 0; k < kTailSteps; ++k) run_step(s - k, cur[k], Checked{});
+        }
 #pragma unroll
         for (int k = 0; k < kTailSteps; ++k) cur[k] = nxt[k];
     }
 #pragma unroll
     for (int k = 0; k < kTailSteps; ++k)  // tail: steps s .. s_end
-        if (s - k >= s_end) run_step(s - k, cur[k]);
+        if (s - k >= s_end) run_step(s - k, cur[k], Checked{});
 }
 #endif  // ADRT_HOST_EMULATION
 

@@ -164,6 +164,14 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
             }
             for (auto &x : rbuf) x = T(NAN);
             for (auto &x : tag) x = -1 << 20;
+            // the kernel's blocks of kTailSteps steps: a whole block that it
+            // runs without range checks must have every later stage's row
+            // inside [0, R)
+            for (int sb = w.top; sb - (kTailSteps - 1) >= -(S - 1); sb -= kTailSteps)
+                if (sb <= R - S && sb - (kTailSteps - 1) >= 0)
+                    for (int k = 0; k < kTailSteps; ++k)
+                        for (int t = 1; t < S; ++t)
+                            if (sb - k + t < 0 || sb - k + t >= R) std::abort();
             for (int s = w.top; s >= -(S - 1); --s) {
                 std::vector<TailOperands> ops(size_t(S) * kLanes);
                 std::vector<T> va(size_t(S) * kLanes), vb(size_t(S) * kLanes);
