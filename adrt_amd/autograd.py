@@ -100,12 +100,19 @@ def _apply_adjoint(x, r):
     return g
 
 
-def lsq_cg(b, x0=None, *, iters=200, callback=None):
+def lsq_cg(b, x0=None, *, iters=200, callback=None, check_every=16):
     """Least squares min_x 0.5 * ||adrt(x) - b||^2 by CGLS for a batch of
-    ADRT data b (B, 4, 2n-1, n) (or unbatched).  Runs exactly ``iters``
-    iterations (stopping early only if an image's gradient vanishes).
-    Returns ``(x, loss_history)``; the history holds per-image losses (float64
-    tensors) before the first and after every iteration."""
+    ADRT data b (B, 4, 2n-1, n) (or unbatched).  Runs ``iters`` iterations;
+    every ``check_every`` iterations the host checks whether every image's
+    gradient has vanished and stops early if so (an image whose gradient
+    vanished meanwhile takes zero steps: alpha = beta = 0).  Returns
+    ``(x, loss_history)``; the history holds per-image losses (float64
+    tensors) before the first and after every iteration run.
+
+    Each iteration applies A once and A^T once: the autograd graph of
+    ``q = adrt(p)`` also gives ``A^T r`` for the updated residual (the
+    backward of a linear map does not depend on the point), so no forward is
+    run only to reach its backward."""
     torch = _torch()
     dev.require_gpu()
     single = b.dim() == 3
@@ -118,20 +125,23 @@ def lsq_cg(b, x0=None, *, iters=200, callback=None):
     p = s.clone()
     gamma = _bdot(s, s)
     history = [0.5 * _bdot(r, r)]
-    for _ in range(iters):
-        if not bool((gamma > 0).any()):
+    one = torch.ones_like(gamma)
+    zero = torch.zeros_like(gamma)
+    for k in range(iters):
+        if k % max(int(check_every), 1) == 0 and not bool((gamma > 0).any()):
             break
-        q = adrt(p).detach()
+        pv = p.detach().requires_grad_(True)
+        with torch.enable_grad():
+            y = adrt(pv)
+        q = y.detach()
         qq = _bdot(q, q)
-        alpha = torch.where(qq > 0, gamma / torch.where(qq > 0, qq, torch.ones_like(qq)),
-                            torch.zeros_like(qq))
+        alpha = torch.where(qq > 0, gamma / torch.where(qq > 0, qq, one), zero)
         a = alpha.to(bb.dtype)
         x = x + a[:, None, None] * p
         r = r - a[:, None, None, None] * q
-        s = _apply_adjoint(x, r)
+        (s,) = torch.autograd.grad(y, pv, grad_outputs=r)  # A^T r
         gamma_new = _bdot(s, s)
-        beta = torch.where(gamma > 0, gamma_new / torch.where(gamma > 0, gamma, torch.ones_like(gamma)),
-                           torch.zeros_like(gamma))
+        beta = torch.where(gamma > 0, gamma_new / torch.where(gamma > 0, gamma, one), zero)
         p = s + beta.to(bb.dtype)[:, None, None] * p
         gamma = gamma_new
         history.append(0.5 * _bdot(r, r))
