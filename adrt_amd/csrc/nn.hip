@@ -177,7 +177,8 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_kernel(ConvArgs a) {
 constexpr int kTH = 4, kCC = 32, kCCP = kCC + 8, kTWmax = 64;
 
 template <int NT, int MODE>
-__global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int TW) {
+__global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int TW,
+                                                                 int tw2_recip) {
     __shared__ __align__(16) bf16 tile[kTH + 2][kTWmax + 2][kCCP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r = lane & 15;
@@ -205,9 +206,12 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
         const int nvec = ccv / 8;
         const int npix = (kTH + 2) * (TW + 2);
         __syncthreads();  // the previous chunk's readers are done
+        // nvec is 1, 2 or 4 (shifts); pix / (TW + 2) by the exact reciprocal
+        // multiply (TW in {16, 32, 48, 64}, pix < 6 (TW + 2))
+        const int lg = nvec == 4 ? 2 : (nvec == 2 ? 1 : 0);
         for (int i = threadIdx.x; i < npix * nvec; i += kWgThreads) {
-            const int v = i % nvec, pix = i / nvec;
-            const int py = pix / (TW + 2), px = pix - py * (TW + 2);
+            const int v = i & (nvec - 1), pix = i >> lg;
+            const int py = (pix * tw2_recip) >> 16, px = pix - py * (TW + 2);
             const int gy = y0 + py - 1, gx = x0 + px - 1;
             const bool ok = unsigned(gy) < unsigned(a.H) && unsigned(gx) < unsigned(a.W);
             const int ch = c0 + 8 * v;
@@ -738,19 +742,24 @@ inline unsigned blocks_for(long long work, int block = kWgThreads) {
 template <int NT>
 static void launch_conv_lds(const ConvArgs &a, int mode, hipStream_t s) {
     const int TW = a.W < kTWmax ? a.W : kTWmax;
+    // floor(p / (TW + 2)) == (p * recip) >> 16 for every tile pixel index p
+    // (checked for TW in {16, 32, 48, 64})
+    const int recip = (65536 + TW + 1) / (TW + 2);
     const long long tiles = (a.P / a.W / kTH) * (a.W / TW);
     const dim3 grid(unsigned(tiles), unsigned(a.Cout / (16 * NT)));
     switch (mode) {
-        case kReluBias: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kReluBias>), grid, dim3(kWgThreads), 0, s, a, TW); break;
-        case kResidual: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kResidual>), grid, dim3(kWgThreads), 0, s, a, TW); break;
-        case kMaskInput: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kMaskInput>), grid, dim3(kWgThreads), 0, s, a, TW); break;
-        default: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kLinear>), grid, dim3(kWgThreads), 0, s, a, TW); break;
+        case kReluBias: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kReluBias>), grid, dim3(kWgThreads), 0, s, a, TW, recip); break;
+        case kResidual: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kResidual>), grid, dim3(kWgThreads), 0, s, a, TW, recip); break;
+        case kMaskInput: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kMaskInput>), grid, dim3(kWgThreads), 0, s, a, TW, recip); break;
+        default: hipLaunchKernelGGL((conv3x3_lds_kernel<NT, kLinear>), grid, dim3(kWgThreads), 0, s, a, TW, recip); break;
     }
 }
 
 template <int NT>
 static void launch_conv_nt(const ConvArgs &a, int mode, hipStream_t s) {
-    if (a.W % 16 == 0 && a.H % kTH == 0 && a.Cin >= 16 && (a.W <= kTWmax || a.W % kTWmax == 0)) {
+    // the tile fill takes 8, 16 or 32 channels per chunk (power-of-two vectors)
+    if (a.W % 16 == 0 && a.H % kTH == 0 && a.Cin >= 16 && a.Cin % kCC != 24 &&
+        (a.W <= kTWmax || a.W % kTWmax == 0)) {
         launch_conv_lds<NT>(a, mode, s);
         return;
     }
@@ -843,7 +852,8 @@ int adrt_hip_nn_conv3x3_upcat(const void *x, int x_stride, int c_skip, const voi
                               int y_stride, int batch, int h, int w_, int cin, int cout, int mode,
                               void *stream) {
     if (batch < 1 || h < 4 || w_ < 16 || h % kTH != 0 || w_ % 16 != 0 ||
-        (w_ > kTWmax && w_ % kTWmax != 0) || cin < 16 || cin % 8 != 0 || cout < 16 ||
+        (w_ > kTWmax && w_ % kTWmax != 0) || cin < 16 || cin % 8 != 0 || cin % kCC == 24 ||
+        cout < 16 ||
         cout % 16 != 0 || c_skip < 8 || c_skip % 8 != 0 || c_skip >= cin ||
         x_stride < c_skip || x_stride % 8 != 0 || up_stride < cin - c_skip ||
         up_stride % 8 != 0 || y_stride < cout || (mode != kReluBias && mode != kLinear))

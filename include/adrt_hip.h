@@ -304,7 +304,8 @@ int adrt_hip_nn_conv3x3(const void *x, int x_stride, const void *w, const float 
  * (channels [0, c_skip), h x w_) and the nearest 2x upsampling of up
  * (channels [c_skip, cin), h/2 x w_/2): the U-Net decoder's input, read in
  * place instead of materialised.  Needs h % 4 == 0, w_ % 16 == 0 (and
- * w_ <= 64 or w_ % 64 == 0), cin >= 16, c_skip % 8 == 0; the result equals
+ * w_ <= 64 or w_ % 64 == 0), cin >= 16 with cin % 32 != 24, c_skip % 8 == 0;
+ * the result equals
  * adrt_hip_nn_conv3x3 on the concatenated buffer bit for bit. */
 int adrt_hip_nn_conv3x3_upcat(const void *x, int x_stride, int c_skip, const void *up,
                               int up_stride, const void *w, const float *bias, void *y,
