@@ -170,8 +170,34 @@ class UnevenBatch:
             self.back[n] = torch.empty_like(self.fwd[n])
             self.ws_f[n] = _workspace(_lib.OP_ADRT, self.code, b, n, self.device)
             self.ws_b[n] = _workspace(_lib.OP_BDRT, self.code, b, n, self.device)
-        self.streams = {n: torch.cuda.Stream(self.device) for n in self.classes}
+        self.lanes = self._assign_lanes()
+        self.streams = [torch.cuda.Stream(self.device) for _ in self.lanes]
         self.graph = None
+
+    # The runtime maps streams onto 4 hardware queues (HIP's default), so
+    # more than 4 streams share queues and serialise whatever lands
+    # together.  The size classes are spread over at most 4 streams by
+    # longest-processing-time-first: each class (costliest first) goes to the
+    # stream with the least work so far.  A class's cost is its tuned
+    # forward + back-projection time, or, untuned, its bytes plus a fixed
+    # per-launch latency (small sides are launch-bound).
+    HW_QUEUES = 4
+
+    def _class_cost(self, n):
+        ms = [t["choice"]["ms"] for t in self.tuning if t["n"] == n and "ms" in t["choice"]]
+        if len(ms) == 2:
+            return sum(ms)
+        launches = 4 if fused_range(n) else 2 * (n.bit_length() + 2)
+        return self.count[n] * n * n * 100 / 5e9 * 1e3 + launches * 0.02  # ms
+
+    def _assign_lanes(self):
+        lanes = [[] for _ in range(min(self.HW_QUEUES, len(self.classes)))]
+        load = [0.0] * len(lanes)
+        for n in sorted(self.classes, key=self._class_cost, reverse=True):
+            i = min(range(len(lanes)), key=lambda j: load[j])
+            lanes[i].append(n)
+            load[i] += self._class_cost(n)
+        return lanes
 
     def load(self, images):
         """Copies ``images`` (a sequence of (n, n) tensors in plan order)."""
@@ -185,15 +211,14 @@ class UnevenBatch:
     def _enqueue(self):
         torch = _torch()
         main = torch.cuda.current_stream(self.device)
-        # largest classes first: they fill the GPU while the small ones overlap
-        for n in reversed(self.classes):
-            s = self.streams[n]
+        for lane, s in zip(self.lanes, self.streams):
             s.wait_stream(main)
-            b = self.count[n]
-            _launch(_lib.OP_ADRT, self.images[n], self.fwd[n], self.ws_f[n], self.code, b, n, s)
-            _launch(_lib.OP_BDRT, self.fwd[n], self.back[n], self.ws_b[n], self.code, b, n, s)
-        for n in self.classes:
-            main.wait_stream(self.streams[n])
+            for n in lane:  # costliest first
+                b = self.count[n]
+                _launch(_lib.OP_ADRT, self.images[n], self.fwd[n], self.ws_f[n], self.code, b, n, s)
+                _launch(_lib.OP_BDRT, self.fwd[n], self.back[n], self.ws_b[n], self.code, b, n, s)
+        for s in self.streams:
+            main.wait_stream(s)
 
     def run(self):
         """Enqueues the whole pipeline on the current stream (eager)."""

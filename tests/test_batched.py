@@ -163,3 +163,27 @@ class TestUnevenBatch:
             plan.load([])
         with pytest.raises(RuntimeError, match="capture"):
             plan.replay()
+
+
+def test_lane_assignment_longest_first():
+    """Size classes go to at most 4 streams (the hardware queues), costliest
+    first onto the least-loaded stream; every class exactly once."""
+    plan = batched.UnevenBatch.__new__(batched.UnevenBatch)
+    plan.classes = [32, 64, 128, 256, 512, 1024]
+    plan.count = {32: 479, 64: 267, 128: 142, 256: 65, 512: 32, 1024: 15}
+    ms = {32: (0.25, 0.25), 64: (0.06, 0.1), 128: (0.1, 0.15), 256: (0.15, 0.3),
+          512: (0.3, 0.5), 1024: (0.5, 1.0)}
+    plan.tuning = [{"op": op, "n": n, "choice": {"ms": ms[n][i]}}
+                   for n in plan.classes for i, op in enumerate(("adrt", "bdrt"))]
+    lanes = plan._assign_lanes()
+    assert len(lanes) == 4
+    assert sorted(n for lane in lanes for n in lane) == plan.classes
+    assert [1024] in lanes                      # the costliest class runs alone
+    costs = [sum(sum(ms[n]) for n in lane) for lane in lanes]
+    assert max(costs) == sum(ms[1024])
+    for lane in lanes:                          # costliest first within a stream
+        assert [sum(ms[n]) for n in lane] == sorted((sum(ms[n]) for n in lane), reverse=True)
+    # untuned: bytes plus launch latency, same properties
+    plan.tuning = []
+    lanes = plan._assign_lanes()
+    assert sorted(n for lane in lanes for n in lane) == plan.classes and len(lanes) == 4
