@@ -17,17 +17,20 @@ Two learned parts, both applied on the GPU by this package's own kernels
   per level, ReLU, 2x2 average pooling, nearest upsampling, skip
   concatenation), BF16 weights and activations with FP32 accumulation on the
   matrix cores.  Input z = P b; output x0 = z + net(z), trained with Adam on
-  ||x0 - x*||^2 against the solution x* of the training pair's normal
-  equations (or against the clean phantom).  Forward and backward are
-  hand-written kernels; the gradients are checked against a float32 CPU
-  autograd model in the tests.
+  ||x0 - target||^2 against the clean phantom (``train_learned``'s default;
+  the solution of the training pair's normal equations is the other
+  target).  Forward and backward are hand-written kernels; the gradients are
+  checked against a float32 CPU autograd model in the tests.
 
-``pcg`` runs preconditioned CG from x0 with float64 scalars per image (fused
-device kernels for the dot products and updates, one iteration captured as a
-hipGraph); an image whose preconditioned step is not a descent step (P not
-positive definite there) continues as plain CG from its current iterate (the
-fallback), and ``evaluate`` reports every image's convergence and its error
-against plain CG's solution and the phantom.
+``pcg`` runs preconditioned CG from x0 with float64 scalars per image: one
+iteration is the operator, the preconditioner and the device kernels that
+hold every per-image scalar (adrt_hip_pcg_update / adrt_hip_pcg_direction),
+with the host checking convergence one iteration behind (optionally the
+iteration is captured as a hipGraph); an image whose preconditioned step is
+not a descent step (P not positive definite there) continues as plain CG
+from its current iterate (the fallback), and ``evaluate`` reports every
+image's convergence and its error against plain CG's solution and the
+phantom.
 """
 
 import ctypes
@@ -612,13 +615,13 @@ class LearnedSolver:
         return pcg(normal_op, b, prec=self.pyramid, rtol=rtol, maxiter=maxiter, graph=False)[0]
 
     def train(self, steps, batch=64, seed=1000, pairs=50000, lr=1e-3, log_every=0, log=None,
-              lr_fn=None, start=0, target="solution"):
+              lr_fn=None, start=0, target="phantom"):
         """Adam on 0.5 * mean ||x0 - t||^2 over seeded phantom pairs (pair i =
         phantom i of ``seed`` with its batch's noise seed); batches cycle
-        through ``pairs`` pairs.  The target t is the normal equations'
-        solution for the pair's noisy data (``target='solution'``: the point
-        CG converges to, so the initial guess is judged by what is left for
-        CG to do), the clean phantom (``target='phantom'``), or none: with
+        through ``pairs`` pairs.  The target t is the clean phantom
+        (``target='phantom'``, the default: measured to give the fewest CG
+        iterations), the normal equations' solution for the pair's noisy data
+        (``target='solution'``: the point CG converges to), or none: with
         ``target='residual'`` the loss is sum_i ||M x0_i - b_i||^2 / ||b_i||^2,
         the quantity CG's stopping test measures (its gradient is
         2 M (M x0 - b) / ||b||^2, M being symmetric); the logged value is then
