@@ -31,12 +31,20 @@ ERR_ARG = 8
 VARIANT_DEFAULT = 0
 VARIANT_ALT_SPLIT = 1
 VARIANT_STREAMING = 2
+VARIANT_SINGLE = 3
 
 _SZ = ctypes.c_size_t
 _VP = ctypes.c_void_p
 _INT = ctypes.c_int
 _SHAPE3 = _SZ * 3
 _SHAPE4 = _SZ * 4
+
+
+class Batch(ctypes.Structure):
+    """``adrt_hip_batch``: one entry of adrt_hip_adrt_bdrt_small."""
+    _fields_ = [("images", ctypes.c_void_p), ("adrt", ctypes.c_void_p),
+                ("bdrt", ctypes.c_void_p), ("batch", ctypes.c_size_t), ("n", ctypes.c_size_t)]
+
 
 # name -> (restype, argtypes); mirrors include/adrt_hip.h
 SIGNATURES = {
@@ -56,6 +64,7 @@ SIGNATURES = {
     "adrt_hip_adrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
     "adrt_hip_bdrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_bdrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
+    "adrt_hip_adrt_bdrt_small": (_INT, [ctypes.POINTER(Batch), _INT, _INT, _INT, _VP]),
     "adrt_hip_iadrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_fmg_restriction": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_fmg_prolongation": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _SZ, _VP]),

@@ -2,11 +2,12 @@
 
 ``UnevenBatch`` takes a list of image sides (powers of two) and holds, per
 distinct side n, one stacked device buffer of the images of that side, their
-forward ADRTs and back-projections, and the workspaces.  ``run()`` enqueues,
-per size class, one batched forward call and one batched back-projection
-call, each size class on its own stream (so small classes overlap), and
-``capture()`` records the whole pipeline once as a hipGraph that
-``replay()`` relaunches with no host work.
+forward ADRTs and back-projections, and the workspaces.  ``run()`` enqueues
+all sides n <= 64 as one persistent launch of the LDS kernel (every stage of
+both transforms in LDS, ``adrt_hip_adrt_bdrt_small``) and, per larger side,
+one batched forward call and one batched back-projection call; these units
+go onto at most 4 streams, and ``capture()`` records the whole pipeline once
+as a hipGraph that ``replay()`` relaunches with no host work.
 
 ``tune()`` is the autotuner: for one (op, n, batch) it times every kernel
 variant the library offers for that size (the fused two-pass kernels with
@@ -64,14 +65,79 @@ def fused_range(n):
     return 6 <= L <= 12
 
 
+SMALL_MAX = 64  # sides served by the one-launch LDS kernel
+SMALL_CLASSES = 8  # size classes per launch of that kernel
+
+
+def small_range(n):
+    return 1 <= n <= SMALL_MAX
+
+
+def _batch_array(entries):
+    """ctypes array of adrt_hip_batch for (images, adrt, bdrt) tensor triples."""
+    arr = (_lib.Batch * len(entries))()
+    for e, (img, fwd, back) in zip(arr, entries):
+        e.images = img.data_ptr()
+        e.adrt = fwd.data_ptr() if fwd is not None else None
+        e.bdrt = back.data_ptr() if back is not None else None
+        e.batch = img.shape[0]
+        e.n = img.shape[-1]
+    return arr
+
+
+def _launch_small(arr, code, stream, max_groups=0):
+    _lib.check(_lib.load().adrt_hip_adrt_bdrt_small(arr, len(arr), code, int(max_groups),
+                                                    ctypes.c_void_p(stream.cuda_stream)),
+               "adrt_bdrt_small")
+
+
+def adrt_bdrt_small(stacks, *, adrt=True, bdrt=True):
+    """Forward ADRT and back-projection of several stacks of small images in
+    one persistent launch.
+
+    ``stacks``: up to 8 tensors (B, n, n), n a power of two <= 64, one dtype
+    (float32 or float64) and device.  Returns one (adrt, bdrt) pair per stack,
+    each (B, 4, 2n-1, n) or None where not asked; equal bit for bit to
+    ``adrt(x)`` and ``bdrt(adrt(x))``."""
+    torch = _torch()
+    dev.require_gpu()
+    if not stacks or len(stacks) > SMALL_CLASSES:
+        raise ValueError(f"expected 1 to {SMALL_CLASSES} stacks, got {len(stacks)}")
+    if not (adrt or bdrt):
+        raise ValueError("ask for the forward result, the back-projection or both")
+    dtype, device = stacks[0].dtype, stacks[0].device
+    code = _code(dtype)
+    entries = []
+    for x in stacks:
+        if x.dim() != 3 or x.shape[1] != x.shape[2] or not small_range(x.shape[-1]) \
+                or x.shape[-1] & (x.shape[-1] - 1):
+            raise ValueError(f"stack of shape {tuple(x.shape)}: expected (B, n, n), "
+                             f"n a power of two <= {SMALL_MAX}")
+        if x.dtype != dtype or x.device != device or x.device.type != "cuda":
+            raise ValueError("stacks must share one dtype and one GPU device")
+        x = x.contiguous()
+        b, n = x.shape[0], x.shape[-1]
+        shape = (b, 4, 2 * n - 1, n)
+        f = torch.empty(shape, dtype=dtype, device=device) if adrt else None
+        k = torch.empty(shape, dtype=dtype, device=device) if bdrt else None
+        entries.append((x, f, k))
+    with torch.cuda.device(device):
+        _launch_small(_batch_array(entries), code, torch.cuda.current_stream(device))
+    return [(f, k) for _, f, k in entries]
+
+
 def candidates(op, n, batch, per_image_bytes):
     """Distinct (variant, group_bytes) configurations for (op, n, batch).
 
-    The alternative stage split exists for odd log2 n in the fused range;
+    The one-launch LDS kernel exists for n <= 64 (the default below 64, the
+    SINGLE variant at 64); the alternative stage split exists for odd log2 n
+    in the fused range;
     group budgets are kept only when they change the images per launch group
     (``per_image_bytes``: the fused path's intermediate per image)."""
     L = int(n).bit_length() - 1
     out = [(_lib.VARIANT_STREAMING, 0)]
+    if small_range(n):  # below the fused range the default is the LDS kernel
+        out.append((_lib.VARIANT_SINGLE if fused_range(n) else _lib.VARIANT_DEFAULT, 0))
     if not fused_range(n):
         return out
     seen = set()
@@ -156,9 +222,12 @@ class UnevenBatch:
             self.slot.append((s, count[s]))
             count[s] += 1
         self.count = count
+        # sides <= 64 run together in one launch (at most 7 classes: 1 .. 64)
+        self.small = [n for n in self.classes if small_range(n)]
+        self.large = [n for n in self.classes if not small_range(n)]
         self.tuning = []
         if autotune:
-            for n in self.classes:
+            for n in self.large:
                 for op in (_lib.OP_ADRT, _lib.OP_BDRT):
                     self.tuning.append(tune(op, n, count[n], dtype=self.dtype,
                                             device=self.device, reps=reps))
@@ -168,36 +237,69 @@ class UnevenBatch:
             self.images[n] = torch.zeros((b, n, n), dtype=self.dtype, device=self.device)
             self.fwd[n] = torch.empty((b, 4, 2 * n - 1, n), dtype=self.dtype, device=self.device)
             self.back[n] = torch.empty_like(self.fwd[n])
-            self.ws_f[n] = _workspace(_lib.OP_ADRT, self.code, b, n, self.device)
-            self.ws_b[n] = _workspace(_lib.OP_BDRT, self.code, b, n, self.device)
+            if n in self.large:
+                self.ws_f[n] = _workspace(_lib.OP_ADRT, self.code, b, n, self.device)
+                self.ws_b[n] = _workspace(_lib.OP_BDRT, self.code, b, n, self.device)
+        self.small_batches = _batch_array(
+            [(self.images[n], self.fwd[n], self.back[n]) for n in self.small]
+        ) if self.small else None
+        self.small_ms = None
+        if autotune and self.small:  # time the small launch like the tuner
+            stream = torch.cuda.current_stream(self.device)
+            _launch_small(self.small_batches, self.code, stream)
+            start, end = (torch.cuda.Event(enable_timing=True),
+                          torch.cuda.Event(enable_timing=True))
+            start.record(stream)
+            for _ in range(reps):
+                _launch_small(self.small_batches, self.code, stream)
+            end.record(stream)
+            end.synchronize()
+            self.small_ms = start.elapsed_time(end) / reps
         self.lanes = self._assign_lanes()
         self.streams = [torch.cuda.Stream(self.device) for _ in self.lanes]
         self.graph = None
 
     # The runtime maps streams onto 4 hardware queues (HIP's default), so
     # more than 4 streams share queues and serialise whatever lands
-    # together.  The size classes are spread over at most 4 streams by
-    # longest-processing-time-first: each class (costliest first) goes to the
-    # stream with the least work so far.  A class's cost is its tuned
+    # together.  The units (each large side, and the one launch of all small
+    # sides) are spread over at most 4 streams by
+    # longest-processing-time-first: each unit (costliest first) goes to the
+    # stream with the least work so far.  A large side's cost is its tuned
     # forward + back-projection time, or, untuned, its bytes plus a fixed
-    # per-launch latency (small sides are launch-bound).
+    # per-launch latency; the small launch's cost is its timed duration
+    # (untuned: its bytes at 1 TB/s plus one launch).
     HW_QUEUES = 4
+    SMALL = "small"  # the unit of all sides <= 64
+    SMALL_GROUPS = 0  # grid cap of the small launch (0: every resident work-group)
+
+    def _bytes(self, n):
+        elem = 4 if self.code == _lib.F32 else 8
+        return self.count[n] * elem * (n * n + 3 * 4 * n * (2 * n - 1))
 
     def _class_cost(self, n):
+        if n == self.SMALL:
+            if getattr(self, "small_ms", None):
+                return self.small_ms
+            return sum(self._bytes(m) for m in self.small) / 1e9 + 0.01  # ms
         ms = [t["choice"]["ms"] for t in self.tuning if t["n"] == n and "ms" in t["choice"]]
         if len(ms) == 2:
             return sum(ms)
         launches = 4 if fused_range(n) else 2 * (n.bit_length() + 2)
-        return self.count[n] * n * n * 100 / 5e9 * 1e3 + launches * 0.02  # ms
+        return self._bytes(n) / 3e9 + launches * 0.01  # ms
 
     def _assign_lanes(self):
-        lanes = [[] for _ in range(min(self.HW_QUEUES, len(self.classes)))]
+        units = list(self.large) + ([self.SMALL] if self.small else [])
+        lanes = [[] for _ in range(min(self.HW_QUEUES, len(units)))]
         load = [0.0] * len(lanes)
-        for n in sorted(self.classes, key=self._class_cost, reverse=True):
+        for u in sorted(units, key=self._class_cost, reverse=True):
             i = min(range(len(lanes)), key=lambda j: load[j])
-            lanes[i].append(n)
-            load[i] += self._class_cost(n)
-        return lanes
+            lanes[i].append(u)
+            load[i] += self._class_cost(u)
+        # The small launch is bound by instruction issue, the large sides by
+        # HBM bandwidth: it goes first on its stream, beside the large sides'
+        # first kernels, rather than last and alone.
+        return [[u for u in lane if u == self.SMALL] + [u for u in lane if u != self.SMALL]
+                for lane in lanes]
 
     def load(self, images):
         """Copies ``images`` (a sequence of (n, n) tensors in plan order)."""
@@ -211,13 +313,21 @@ class UnevenBatch:
     def _enqueue(self):
         torch = _torch()
         main = torch.cuda.current_stream(self.device)
-        for lane, s in zip(self.lanes, self.streams):
+        if len(self.lanes) > len(self.streams):
+            raise RuntimeError("more lanes than streams")
+        # only the streams that carry work fork from and join `main`: under
+        # graph capture a wait on a stream outside the capture is invalid
+        used = self.streams[:len(self.lanes)]
+        for lane, s in zip(self.lanes, used):
             s.wait_stream(main)
-            for n in lane:  # costliest first
+            for n in lane:
+                if n == self.SMALL:
+                    _launch_small(self.small_batches, self.code, s, self.SMALL_GROUPS)
+                    continue
                 b = self.count[n]
                 _launch(_lib.OP_ADRT, self.images[n], self.fwd[n], self.ws_f[n], self.code, b, n, s)
                 _launch(_lib.OP_BDRT, self.fwd[n], self.back[n], self.ws_b[n], self.code, b, n, s)
-        for s in self.streams:
+        for s in used:
             main.wait_stream(s)
 
     def run(self):

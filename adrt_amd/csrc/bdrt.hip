@@ -93,6 +93,7 @@ static int run_streaming(const T *in, T *out, T *ws, size_t B, size_t n, hipStre
 }
 
 size_t bdrt_workspace(int dtype, size_t B, size_t n) {
+    if (use_small(ADRT_HIP_OP_BDRT, B, n)) return 0;
     if (use_fused(ADRT_HIP_OP_BDRT, n)) return fused_bdrt_workspace(dtype, B, n);
     const size_t s = dtype == ADRT_HIP_F64 ? 8 : 4;
     return B * 4 * n * (2 * n - 1) * s;
@@ -102,6 +103,8 @@ size_t bdrt_workspace(int dtype, size_t B, size_t n) {
 // other sizes stream one stage per kernel.
 int launch_bdrt(const void *in, void *out, void *ws, size_t, int dtype, size_t B, size_t n,
                 hipStream_t s) {
+    if (use_small(ADRT_HIP_OP_BDRT, B, n))
+        return launch_small(in, out, ADRT_HIP_OP_BDRT, dtype, B, n, s);
     if (use_fused(ADRT_HIP_OP_BDRT, n)) return run_fused_bdrt(in, out, ws, dtype, B, n, s);
     if (dtype == ADRT_HIP_F32)
         return run_streaming(static_cast<const float *>(in), static_cast<float *>(out),

@@ -79,6 +79,12 @@ bool use_fused(int op, size_t n);
 int set_fused_config(int op, size_t n, int variant, size_t group_bytes);
 int get_fused_config(int op, size_t n, int *variant, size_t *group_bytes);
 size_t fused_workspace(int dtype, size_t B, size_t n);
+// One-launch LDS path (small.hip) for power-of-two n <= 64: supported sizes,
+// whether (op, n) is configured to use it, and one op over one batch.
+bool small_supported(size_t B, size_t n);
+bool use_small(int op, size_t B, size_t n);
+int launch_small(const void *src, void *out, int op, int dtype, size_t B, size_t n,
+                 hipStream_t s);
 int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, hipStream_t s);
 size_t fused_bdrt_workspace(int dtype, size_t B, size_t n);
 int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,

@@ -1390,15 +1390,28 @@ int fused_variant(int op, size_t n) {
 }
 
 bool use_fused(int op, size_t n) {
-    return fused_supported(n) && fused_variant(op, n) != ADRT_HIP_VARIANT_STREAMING;
+    const int v = fused_variant(op, n);
+    return fused_supported(n) && v != ADRT_HIP_VARIANT_STREAMING && v != ADRT_HIP_VARIANT_SINGLE;
+}
+
+// The one-launch LDS path: the default below the fused range, selectable up
+// to n = 64.
+bool use_small(int op, size_t B, size_t n) {
+    if (op != ADRT_HIP_OP_ADRT && op != ADRT_HIP_OP_BDRT) return false;
+    const int v = fused_variant(op, n);
+    const bool chosen = v == ADRT_HIP_VARIANT_SINGLE ||
+                        (v == ADRT_HIP_VARIANT_DEFAULT && !fused_supported(n));
+    return chosen && small_supported(B, n);
 }
 
 int set_fused_config(int op, size_t n, int variant, size_t group_bytes) {
     Tuned *t = tuned(op, n);
     if (!t) return ADRT_HIP_ERR_ARG;
-    if (variant < ADRT_HIP_VARIANT_DEFAULT || variant > ADRT_HIP_VARIANT_STREAMING)
+    if (variant < ADRT_HIP_VARIANT_DEFAULT || variant > ADRT_HIP_VARIANT_SINGLE)
         return ADRT_HIP_ERR_ARG;
     if (variant == ADRT_HIP_VARIANT_ALT_SPLIT && op != ADRT_HIP_OP_IADRT && !fused_supported(n))
+        return ADRT_HIP_ERR_ARG;
+    if (variant == ADRT_HIP_VARIANT_SINGLE && (op == ADRT_HIP_OP_IADRT || !small_supported(1, n)))
         return ADRT_HIP_ERR_ARG;
     t->variant.store(variant, std::memory_order_relaxed);
     t->group_bytes.store(group_bytes, std::memory_order_relaxed);

@@ -104,6 +104,7 @@ static int run_streaming(const T *in, T *out, T *ws, size_t B, size_t n, hipStre
 }
 
 size_t adrt_workspace(int dtype, size_t B, size_t n) {
+    if (use_small(ADRT_HIP_OP_ADRT, B, n)) return 0;
     if (use_fused(ADRT_HIP_OP_ADRT, n)) return fused_workspace(dtype, B, n);
     const size_t s = dtype == ADRT_HIP_F64 ? 8 : 4;
     return B * 4 * n * (2 * n - 1) * s;
@@ -111,6 +112,8 @@ size_t adrt_workspace(int dtype, size_t B, size_t n) {
 
 int launch_adrt(const void *in, void *out, void *ws, size_t, int dtype, size_t B, size_t n,
                 hipStream_t s) {
+    if (use_small(ADRT_HIP_OP_ADRT, B, n))
+        return launch_small(in, out, ADRT_HIP_OP_ADRT, dtype, B, n, s);
     if (use_fused(ADRT_HIP_OP_ADRT, n)) return run_fused(in, out, ws, dtype, B, n, s);
     if (dtype == ADRT_HIP_F32)
         return run_streaming(static_cast<const float *>(in), static_cast<float *>(out),

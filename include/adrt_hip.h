@@ -113,6 +113,27 @@ int adrt_hip_adrt_step(const void *in, void *out, int dtype, size_t batch, size_
 int adrt_hip_bdrt(const void *in, void *out, void *workspace, size_t workspace_bytes,
                   int dtype, size_t batch, size_t n, void *stream);
 
+/* Forward ADRT and back-projection of several batches of small sides in one
+ * persistent launch (no reference counterpart: the reference runs one call
+ * per array; this is the uneven-size batch of many small images).  For each
+ * entry, images (batch, n, n) give adrt = adrt_hip_adrt(images) and
+ * bdrt = adrt_hip_bdrt(adrt), both (batch, 4, 2n-1, n), bit-identical to
+ * those calls; either output may be NULL (not stored), not both.  n a power
+ * of two <= 64; 1 <= count <= 8; no workspace, no allocation, capturable in a
+ * graph.  Entries of batch 0 are skipped.  max_groups > 0 caps the persistent
+ * grid (rounded down to a multiple of 32, at least 32) to leave CUs to other
+ * streams; 0 = every work-group that fits at once. */
+typedef struct adrt_hip_batch {
+    const void *images;
+    void *adrt;
+    void *bdrt;
+    size_t batch;
+    size_t n;
+} adrt_hip_batch;
+
+int adrt_hip_adrt_bdrt_small(const adrt_hip_batch *batches, int count, int dtype,
+                             int max_groups, void *stream);
+
 /* One back-projection stage.  Replaces adrt._adrt_cdefs.bdrt_step
  * (adrt_cdefs_py.cpp:550-619 -> bdrt_step, adrt_cdefs_bdrt.hpp:189-244). */
 int adrt_hip_bdrt_step(const void *in, void *out, int dtype, size_t batch, size_t n,
@@ -253,9 +274,11 @@ int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, v
 /* Variants of the forward and back-projection paths for one size:
  *   DEFAULT    log2 n in [6, 12]: the fused two-pass kernels with the default
  *              stage split (forward pass 1 takes ceil(L/2) of the L = log2 n
- *              stages, back-projection pass A floor(L/2)); other n: streaming;
+ *              stages, back-projection pass A floor(L/2)); n < 64: SINGLE;
+ *              n > 4096: streaming;
  *   ALT_SPLIT  the fused kernels with the other split (differs for odd L);
- *   STREAMING  one kernel per stage.
+ *   STREAMING  one kernel per stage;
+ *   SINGLE     n <= 64: one launch with every stage in LDS (no workspace).
  * and of the inverse (ADRT_HIP_OP_IADRT; the group budget does not apply):
  *   DEFAULT    the last min(L, 5) stages in one fused pass, the others one
  *              kernel each;
@@ -265,6 +288,7 @@ int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, v
 #define ADRT_HIP_VARIANT_DEFAULT 0
 #define ADRT_HIP_VARIANT_ALT_SPLIT 1
 #define ADRT_HIP_VARIANT_STREAMING 2
+#define ADRT_HIP_VARIANT_SINGLE 3
 
 /* Selects the variant and the per-launch-group workspace budget (bytes of
  * intermediate per group of images; 0 = the default 136 MiB; a group holds
@@ -274,7 +298,7 @@ int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, v
  * query the workspace after configuring.  A launch that meets a workspace
  * sized for another configuration returns ADRT_HIP_ERR_WORKSPACE.  Used by
  * the autotuner; ALT_SPLIT is rejected (ADRT_HIP_ERR_ARG) where n has no
- * fused path. */
+ * fused path, SINGLE for n > 64 and for ADRT_HIP_OP_IADRT. */
 int adrt_hip_set_config(int op, size_t n, int variant, size_t group_bytes);
 
 /* Reads the configuration of (op, n) (group_bytes: the budget in effect). */

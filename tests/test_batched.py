@@ -16,9 +16,14 @@ gpu = pytest.mark.gpu
 MIB = 1 << 20
 
 
-def test_candidates_streaming_only_outside_fused_range():
-    assert batched.candidates(_lib.OP_ADRT, 32, 100, 1) == [(_lib.VARIANT_STREAMING, 0)]
+def test_candidates_outside_fused_range():
+    # below 64 the default is the one-launch LDS kernel; above 4096 streaming only
+    assert batched.candidates(_lib.OP_ADRT, 32, 100, 1) == [(_lib.VARIANT_STREAMING, 0),
+                                                           (_lib.VARIANT_DEFAULT, 0)]
     assert batched.candidates(_lib.OP_BDRT, 8192, 2, 1) == [(_lib.VARIANT_STREAMING, 0)]
+    at64 = batched.candidates(_lib.OP_ADRT, 64, 100, 1)
+    assert (_lib.VARIANT_SINGLE, 0) in at64 and (_lib.VARIANT_STREAMING, 0) in at64
+    assert all(v != _lib.VARIANT_SINGLE for v, _ in batched.candidates(_lib.OP_ADRT, 128, 9, 1))
 
 
 def test_candidates_alt_split_for_odd_log2():
@@ -66,7 +71,12 @@ class TestConfig:
         assert lib.adrt_hip_set_config(7, 64, 0, 0) == _lib.ERR_ARG
         assert lib.adrt_hip_set_config(_lib.OP_IADRT, 48, 0, 0) == _lib.ERR_ARG
         assert lib.adrt_hip_set_config(_lib.OP_ADRT, 96, 0, 0) == _lib.ERR_ARG
-        assert lib.adrt_hip_set_config(_lib.OP_ADRT, 64, 3, 0) == _lib.ERR_ARG
+        assert lib.adrt_hip_set_config(_lib.OP_ADRT, 64, 4, 0) == _lib.ERR_ARG
+        # the one-launch LDS kernel exists for n <= 64 and not for the inverse
+        assert lib.adrt_hip_set_config(_lib.OP_ADRT, 128, _lib.VARIANT_SINGLE, 0) == _lib.ERR_ARG
+        assert lib.adrt_hip_set_config(_lib.OP_IADRT, 32, _lib.VARIANT_SINGLE, 0) == _lib.ERR_ARG
+        assert lib.adrt_hip_set_config(_lib.OP_BDRT, 64, _lib.VARIANT_SINGLE, 0) == _lib.OK
+        assert lib.adrt_hip_workspace_bytes(_lib.OP_BDRT, _lib.F32, 4, 64) == 0
 
     def test_workspace_follows_config(self, cuda, reset_config):
         import torch
@@ -84,7 +94,7 @@ class TestConfig:
                                8, 256, torch.cuda.current_stream().cuda_stream)
         assert st == _lib.ERR_WORKSPACE
 
-    @pytest.mark.parametrize("n", [64, 128, 512, 2048])
+    @pytest.mark.parametrize("n", [1, 2, 16, 32, 64, 128, 512, 2048])
     def test_every_variant_matches_oracle(self, cuda, reset_config, n):
         import torch
         from adrt_amd import _kernels
@@ -170,20 +180,159 @@ def test_lane_assignment_longest_first():
     first onto the least-loaded stream; every class exactly once."""
     plan = batched.UnevenBatch.__new__(batched.UnevenBatch)
     plan.classes = [32, 64, 128, 256, 512, 1024]
+    plan.small, plan.large = [32, 64], [128, 256, 512, 1024]
+    plan.code = _lib.F32
     plan.count = {32: 479, 64: 267, 128: 142, 256: 65, 512: 32, 1024: 15}
-    ms = {32: (0.25, 0.25), 64: (0.06, 0.1), 128: (0.1, 0.15), 256: (0.15, 0.3),
-          512: (0.3, 0.5), 1024: (0.5, 1.0)}
+    ms = {128: (0.1, 0.15), 256: (0.15, 0.3), 512: (0.3, 0.5), 1024: (0.5, 1.0)}
     plan.tuning = [{"op": op, "n": n, "choice": {"ms": ms[n][i]}}
-                   for n in plan.classes for i, op in enumerate(("adrt", "bdrt"))]
+                   for n in plan.large for i, op in enumerate(("adrt", "bdrt"))]
+    cost = dict((n, sum(ms[n])) for n in plan.large)
+    plan.small_ms = 0.17                     # timed duration of the small launch
+    cost["small"] = plan._class_cost("small")
+    assert cost["small"] == 0.17
     lanes = plan._assign_lanes()
     assert len(lanes) == 4
-    assert sorted(n for lane in lanes for n in lane) == plan.classes
+    units = [u for lane in lanes for u in lane]
+    assert sorted(map(str, units)) == sorted(map(str, plan.large + ["small"]))
     assert [1024] in lanes                      # the costliest class runs alone
-    costs = [sum(sum(ms[n]) for n in lane) for lane in lanes]
-    assert max(costs) == sum(ms[1024])
-    for lane in lanes:                          # costliest first within a stream
-        assert [sum(ms[n]) for n in lane] == sorted((sum(ms[n]) for n in lane), reverse=True)
+    costs = [sum(cost[u] for u in lane) for lane in lanes]
+    assert max(costs) == cost[1024]
+    for lane in lanes:   # the small launch first, then the costliest first
+        rest = [u for u in lane if u != "small"]
+        assert lane[:len(lane) - len(rest)] in ([], ["small"])
+        assert [cost[u] for u in rest] == sorted((cost[u] for u in rest), reverse=True)
     # untuned: bytes plus launch latency, same properties
-    plan.tuning = []
+    plan.tuning, plan.small_ms = [], None
+    assert 0 < plan._class_cost("small") < plan._class_cost(1024)
     lanes = plan._assign_lanes()
-    assert sorted(n for lane in lanes for n in lane) == plan.classes and len(lanes) == 4
+    units = [u for lane in lanes for u in lane]
+    assert sorted(map(str, units)) == sorted(map(str, plan.large + ["small"])) and len(lanes) == 4
+
+
+def test_small_launch_argument_checks():
+    """adrt_hip_adrt_bdrt_small validates before touching the device."""
+    import ctypes
+
+    lib = _lib.load()
+    arr = (_lib.Batch * 9)()
+    for e in arr:
+        e.images, e.adrt, e.bdrt, e.batch, e.n = 16, 16, 16, 0, 32
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 0, _lib.F32, 0, None) == _lib.ERR_ARG
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 9, _lib.F32, 0, None) == _lib.ERR_ARG
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 1, 7, 0, None) == _lib.ERR_DTYPE
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 1, _lib.F32, -1, None) == _lib.ERR_ARG
+    assert lib.adrt_hip_adrt_bdrt_small(ctypes.cast(None, ctypes.POINTER(_lib.Batch)), 1,
+                                        _lib.F32, 0, None) == _lib.ERR_NULL
+    arr[1].n = 128
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 2, _lib.F32, 0, None) == _lib.ERR_SHAPE
+    arr[1].n = 48
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 2, _lib.F32, 0, None) == _lib.ERR_SHAPE
+    arr[1].n, arr[1].adrt, arr[1].bdrt = 8, None, None
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 2, _lib.F32, 0, None) == _lib.ERR_ARG
+    arr[1].adrt, arr[1].batch, arr[1].images = 16, 3, None
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 2, _lib.F32, 0, None) == _lib.ERR_NULL
+    arr[1].images, arr[1].batch = 16, 1 << 40
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 2, _lib.F32, 0, None) == _lib.ERR_TOO_BIG
+    # nothing to do: every batch empty
+    assert lib.adrt_hip_adrt_bdrt_small(arr, 1, _lib.F32, 0, None) == _lib.OK
+
+
+@gpu
+class TestSmallLaunch:
+    """All sides <= 64 in one persistent launch of the LDS kernel."""
+
+    SIDES = [64, 32, 16, 8, 4, 2, 1]
+    COUNTS = [5, 9, 3, 7, 2, 3, 4]
+
+    def _stacks(self, cuda, dtype):
+        import torch
+
+        g = torch.Generator(device=cuda)
+        g.manual_seed(5)
+        return [torch.randn((b, n, n), generator=g, device=cuda, dtype=dtype)
+                for n, b in zip(self.SIDES, self.COUNTS)]
+
+    @pytest.mark.parametrize("dt", ["float32", "float64"])
+    def test_matches_oracle_and_default_path(self, cuda, dt):
+        import torch
+        from adrt_amd import _kernels
+
+        stacks = self._stacks(cuda, getattr(torch, dt))
+        stacks[0][1, 2, :3] = torch.tensor([-0.0, float("inf"), -float("inf")], dtype=stacks[0].dtype)
+        got = batched.adrt_bdrt_small(stacks)
+        for x, (f, k) in zip(stacks, got):
+            xn = x.cpu().numpy()
+            want = O.adrt(xn)
+            assert_bitwise(f.cpu().numpy(), want, f"adrt {tuple(x.shape)}")
+            assert_bitwise(k.cpu().numpy(), O.bdrt(want), f"bdrt {tuple(x.shape)}")
+            # the library's single-op calls (fused kernels at 64) agree bit for
+            # bit (inf - inf makes NaNs here, which torch.equal calls unequal)
+            assert_bitwise(f.cpu().numpy(), _kernels.adrt(x).cpu().numpy())
+            assert_bitwise(k.cpu().numpy(), _kernels.bdrt(f).cpu().numpy())
+
+    def test_one_output(self, cuda):
+        import torch
+
+        stacks = self._stacks(cuda, torch.float32)[:3]
+        both = batched.adrt_bdrt_small(stacks)
+        only_f = batched.adrt_bdrt_small(stacks, bdrt=False)
+        only_b = batched.adrt_bdrt_small(stacks, adrt=False)
+        for (f, k), (f1, k1), (f2, k2) in zip(both, only_f, only_b):
+            assert k1 is None and f2 is None
+            assert torch.equal(f, f1) and torch.equal(k, k2)
+
+    def test_graph_replays(self, cuda):
+        import torch
+
+        stacks = self._stacks(cuda, torch.float32)
+        entries = []
+        for x in stacks:
+            b, n = x.shape[0], x.shape[-1]
+            entries.append((x, torch.empty((b, 4, 2 * n - 1, n), device=cuda),
+                            torch.empty((b, 4, 2 * n - 1, n), device=cuda)))
+        arr = batched._batch_array(entries)
+        want = batched.adrt_bdrt_small(stacks)
+        s = torch.cuda.Stream(cuda)
+        s.wait_stream(torch.cuda.current_stream(cuda))
+        with torch.cuda.stream(s):
+            batched._launch_small(arr, _lib.F32, s)
+        torch.cuda.current_stream(cuda).wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            batched._launch_small(arr, _lib.F32, torch.cuda.current_stream(cuda))
+        for rep in range(3):
+            for _, f, k in entries:
+                f.fill_(float("nan"))
+                k.fill_(float("nan"))
+            g.replay()
+            torch.cuda.synchronize()
+            _ = batched.adrt_bdrt_small(stacks[:1])  # eager work between replays
+            for (_, f, k), (wf, wk) in zip(entries, want):
+                assert torch.equal(f, wf) and torch.equal(k, wk), rep
+
+    def test_variant_single_at_64(self, cuda, reset_config):
+        """adrt/bdrt at n = 64 configured to the LDS kernel: no workspace and
+        the fused path's results."""
+        import torch
+        from adrt_amd import _kernels
+
+        x = self._stacks(cuda, torch.float32)[0]
+        f0 = _kernels.adrt(x)
+        k0 = _kernels.bdrt(f0)
+        batched.set_config(_lib.OP_ADRT, 64, _lib.VARIANT_SINGLE, 0)
+        batched.set_config(_lib.OP_BDRT, 64, _lib.VARIANT_SINGLE, 0)
+        assert _lib.load().adrt_hip_workspace_bytes(_lib.OP_ADRT, _lib.F32, 5, 64) == 0
+        assert torch.equal(_kernels.adrt(x), f0)
+        assert torch.equal(_kernels.bdrt(f0), k0)
+
+    def test_rejects_bad_stacks(self, cuda):
+        import torch
+
+        with pytest.raises(ValueError, match="power of two"):
+            batched.adrt_bdrt_small([torch.zeros((1, 128, 128), device=cuda)])
+        with pytest.raises(ValueError, match="one dtype"):
+            batched.adrt_bdrt_small([torch.zeros((1, 8, 8), device=cuda),
+                                     torch.zeros((1, 4, 4), device=cuda, dtype=torch.float64)])
+        with pytest.raises(ValueError, match="1 to 8"):
+            batched.adrt_bdrt_small([torch.zeros((1, 8, 8), device=cuda)] * 9)

@@ -192,3 +192,91 @@ def test_iadrt_fused_tail_emulation(iadrt_emulator, tmp_path, dt, n, tail):
     assert res.returncode == 0, res.stderr[-4000:]
     got = np.fromfile(fout, dtype=dt).reshape(x.shape)
     assert np.array_equal(got.view(np.uint8), O.iadrt(x).view(np.uint8))
+
+
+SMALL_SRC = os.path.join(HERE, "host_emulation", "small_emu.cpp")
+
+
+@pytest.fixture(scope="module")
+def small_emulator(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None or not os.path.isdir(ROCM_INCLUDE):
+        pytest.skip("g++ or ROCm headers not available")
+    exe = str(tmp_path_factory.mktemp("emu") / "small_emu")
+    cmd = [gxx, "-std=c++17", "-O1", "-g", "-ffp-contract=off", "-fsanitize=address",
+           "-D__HIP_PLATFORM_AMD__", f"-I{ROCM_INCLUDE}", f"-I{os.path.join(ROOT, 'include')}",
+           "-x", "c++", SMALL_SRC, "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+def run_small(exe, arrays, tmp_path, mode="fused", grid=64):
+    """Runs the one-launch kernel's schedule over `arrays` (one per class:
+    images (B, n, n), or ADRT data for mode 'bdrt'); returns per class
+    (adrt, bdrt) with None for what the mode does not write."""
+    args = []
+    for i, a in enumerate(arrays):
+        n = a.shape[-1]
+        fin = tmp_path / f"in{i}.bin"
+        a.tofile(fin)
+        args += [str(n), str(a.shape[0]), str(fin), str(tmp_path / f"f{i}.bin"),
+                 str(tmp_path / f"b{i}.bin")]
+    tag = "f32" if arrays[0].dtype == np.float32 else "f64"
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([exe, tag, mode, str(grid), *args], capture_output=True, text=True,
+                         env=env)
+    assert res.returncode == 0, res.stderr[-4000:]
+    out = []
+    for i, a in enumerate(arrays):
+        b, n = a.shape[0], a.shape[-1]
+        shape = (b, 4, 2 * n - 1, n)
+        f = None if mode == "bdrt" else np.fromfile(tmp_path / f"f{i}.bin", a.dtype).reshape(shape)
+        k = None if mode == "fwd" else np.fromfile(tmp_path / f"b{i}.bin", a.dtype).reshape(shape)
+        out.append((f, k))
+    return out
+
+
+def _same(a, b):
+    """Bitwise equal, signs of zero included; NaNs match by position only
+    (their payload and sign depend on the order of two NaN operands)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return (a.shape == b.shape and np.array_equal(na, nb)
+            and np.array_equal(np.where(na, 0, a).view(np.uint8), np.where(nb, 0, b).view(np.uint8)))
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+@pytest.mark.parametrize("grid", [32, 64, 512])
+def test_small_kernel_emulation_classes(small_emulator, tmp_path, dt, grid):
+    """Every side 1..64 in one persistent launch (forward, then the
+    back-projection of the forward result from LDS): bit-identical to the
+    oracle's adrt and bdrt(adrt), signs of zero included, NaN where the
+    oracle has NaN."""
+    rng = np.random.default_rng(grid)
+    sides = [64, 32, 16, 8, 4, 2, 1]
+    counts = [3, 5, 2, 7, 1, 3, 2]
+    arrays = [(rng.standard_normal((b, n, n)) * 4).astype(dt) for n, b in zip(sides, counts)]
+    arrays[0][0, 0, :4] = [-0.0, np.inf, -np.inf, np.nan]
+    arrays[1][1, :, 3] = -0.0
+    got = run_small(small_emulator, arrays, tmp_path, grid=grid)
+    for a, (f, k) in zip(arrays, got):
+        want = O.adrt(a)
+        assert _same(f, want), a.shape
+        assert _same(k, O.bdrt(want)), a.shape
+
+
+@pytest.mark.parametrize("mode", ["fwd", "bdrt"])
+@pytest.mark.parametrize("n", [64, 32, 8])
+def test_small_kernel_emulation_single_op(small_emulator, tmp_path, mode, n):
+    """One op over one batch, as adrt_hip_adrt / adrt_hip_bdrt run it below
+    the fused range (a batch that leaves the last chunk partly empty)."""
+    rng = np.random.default_rng(n)
+    b = 5
+    if mode == "fwd":
+        x = rng.standard_normal((b, n, n)).astype(np.float32)
+        (f, _), = run_small(small_emulator, [x], tmp_path, mode="fwd", grid=32)
+        assert _same(f, O.adrt(x))
+    else:
+        y = rng.standard_normal((b, 4, 2 * n - 1, n))
+        y[0, 2, -1, :2] = [-0.0, np.nan]
+        (_, k), = run_small(small_emulator, [y], tmp_path, mode="bdrt", grid=32)
+        assert _same(k, O.bdrt(y))
