@@ -222,77 +222,95 @@ __host__ __device__ __forceinline__ void store_planes(const T *buf, T *dst, cons
     }
 }
 
+// Stage units: one 64-element segment of a pair of output lines that read
+// the same two input lines, so the pair's index math is done once; a chunk
+// has kMaxUnits / 2 of them per stage.
+constexpr int kPairUnits = kUnits / 2;
+
 // Forward stage with S = 2^log2s angles per line group (fwd_stage_kernel):
 // out line (bqr, a): out[c] = x[c] (c < ceil(a/2)) else x[c] + y[c - ceil(a/2)],
-// x = input line (2 bqr, a/2), y = input line (2 bqr + 1, a/2).  A unit is
-// one 64-column segment of one output line.
+// x = input line (2 bqr, a/2), y = input line (2 bqr + 1, a/2).  Angles
+// a = 2k and 2k + 1 share x and y (shifts k and k + 1): a unit reads x[c],
+// y[c - k] and y[c - k - 1] and writes both lines.
 template <typename T>
 __host__ __device__ __forceinline__ void fwd_stage(const T *in, T *out, const Chunk &k, int log2s,
                                                    int wave, int lane) {
     const int C = k.C, seg = k.seg;
     const int S_in = (1 << log2s) >> 1;
-    const int units = (k.np * k.n) << seg;
-    T v[kUnits];
+    const int units = ((k.np * k.n) >> 1) << seg;
+    T ve[kPairUnits], vo[kPairUnits];
 #pragma unroll
-    for (int t = 0; t < kUnits; ++t) {
+    for (int t = 0; t < kPairUnits; ++t) {
         const int u = imin(wave + t * kWaves, units - 1);
-        const int line = u >> seg;
-        const int a = line & ((1 << log2s) - 1);
-        const int bqr = line >> log2s;
-        const int xo = ((2 * bqr) * S_in + (a >> 1)) * C;
+        const int pair = u >> seg;  // = bqr * S_in + kk
+        const int kk = pair & (S_in - 1);
+        const int bqr = pair >> (log2s - 1);
+        const int xo = ((2 * bqr) * S_in + kk) * C;
         const int yo = xo + S_in * C;
-        const int sh = (a + 1) >> 1;
         const int c = ((u & ((1 << seg) - 1)) << 6) + lane;
         const T xv = in[xo + imin(c, C - 1)];
-        const T yv = in[yo + imin(c >= sh ? c - sh : 0, C - 1)];
-        v[t] = c >= sh ? xv + yv : xv;
+        const T y0 = in[yo + imin(c >= kk ? c - kk : 0, C - 1)];
+        const T y1 = in[yo + imin(c > kk ? c - kk - 1 : 0, C - 1)];
+        ve[t] = c >= kk ? xv + y0 : xv;
+        vo[t] = c > kk ? xv + y1 : xv;
     }
 #pragma unroll
-    for (int t = 0; t < kUnits; ++t) {
+    for (int t = 0; t < kPairUnits; ++t) {
         const int u = wave + t * kWaves;
         if (u >= units) break;
         const int c = ((u & ((1 << seg) - 1)) << 6) + lane;
-        if (c < C) out[(u >> seg) * C + c] = v[t];
+        if (c < C) {
+            T *o = out + 2 * (u >> seg) * C + c;
+            o[0] = ve[t];
+            o[C] = vo[t];
+        }
     }
 }
 
 // Back-projection stage with R = 2^(L-log2s) angle pairs and S = 2^log2s
 // sections (bdrt_stage_kernel): left sections x[r] + y[r]; right sections
 // x[r+si] + y[r+si+1], a copy of x[C-1] at r + si == C-1 and zeros below.
-// A unit is one 64-row segment of one output line.
+// The left and right sections of one section pair read the same x and y
+// lines: a unit writes both.
 template <typename T>
 __host__ __device__ __forceinline__ void bdrt_stage(const T *in, T *out, const Chunk &k, int log2s,
                                                     int wave, int lane) {
     const int C = k.C, seg = k.seg;
     const int S_in = (1 << log2s) >> 1;
     const int Rm = (1 << (k.L - log2s)) - 1;
-    const int units = (k.np * k.n) << seg;
-    T v[kUnits];
+    const int units = ((k.np * k.n) >> 1) << seg;
+    T vl[kPairUnits], vr[kPairUnits];
 #pragma unroll
-    for (int t = 0; t < kUnits; ++t) {
+    for (int t = 0; t < kPairUnits; ++t) {
         const int u = imin(wave + t * kWaves, units - 1);
-        const int line = u >> seg;
-        const int sp = line & ((1 << log2s) - 1);
-        const int bqsi = line >> log2s;
-        const bool right = (sp & 1) != 0;
-        const int si = right ? (bqsi & Rm) : 0;
-        const int xo = ((2 * bqsi) * S_in + (sp >> 1)) * C;
+        const int pair = u >> seg;  // = bqsi * S_in + sec
+        const int sec = pair & (S_in - 1);
+        const int bqsi = pair >> (log2s - 1);
+        const int si = bqsi & Rm;
+        const int xo = ((2 * bqsi) * S_in + sec) * C;
         const int yo = xo + S_in * C;
         const int r = ((u & ((1 << seg) - 1)) << 6) + lane;
+        const int rc = imin(r, C - 1);
         const int row = r + si;
-        const int ya = right ? row + 1 : row;
-        const T xv = in[xo + imin(row, C - 1)];
-        const T yv = in[yo + imin(ya, C - 1)];
-        T w = xv + yv;
-        if (right && row >= C - 1) w = row == C - 1 ? xv : T(0);
-        v[t] = w;
+        const T xl = in[xo + rc];
+        const T yl = in[yo + rc];
+        const T xr = in[xo + imin(row, C - 1)];
+        const T yr = in[yo + imin(row + 1, C - 1)];
+        vl[t] = xl + yl;
+        T w = xr + yr;
+        if (row >= C - 1) w = row == C - 1 ? xr : T(0);
+        vr[t] = w;
     }
 #pragma unroll
-    for (int t = 0; t < kUnits; ++t) {
+    for (int t = 0; t < kPairUnits; ++t) {
         const int u = wave + t * kWaves;
         if (u >= units) break;
         const int r = ((u & ((1 << seg) - 1)) << 6) + lane;
-        if (r < C) out[(u >> seg) * C + r] = v[t];
+        if (r < C) {
+            T *o = out + 2 * (u >> seg) * C + r;
+            o[0] = vl[t];
+            o[C] = vr[t];
+        }
     }
 }
 

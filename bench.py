@@ -67,7 +67,8 @@ def parse():
     p.add_argument("--iterations", type=int, default=0,
                    help="--summarize a kernel trace per solver iteration over the last K "
                         "iterations (e.g. --op pcg: --match pcg_dir_kernel --iterations 5)")
-    p.add_argument("--op", default="adrt", choices=["adrt", "bdrt", "iadrt", "unet", "pcg"],
+    p.add_argument("--op", default="adrt",
+                   choices=["adrt", "bdrt", "iadrt", "unet", "pcg", "uneven"],
                    help="operation a --profile run calls (bdrt/iadrt on the forward output)")
     return p.parse_args()
 
@@ -377,6 +378,31 @@ def main():
             solver.net.forward(z, train=False)
         torch.cuda.synchronize()
         print(json.dumps({"op": "unet", "images": 1000, "n": 256}), flush=True)
+        return
+    if args.profile and args.op == "uneven":
+        # the uneven-size pipeline of 1000 images (autotuned, captured):
+        # 3 graph replays after the capture's eager warm-up
+        import numpy as np
+        from adrt_amd import batched
+
+        sides = np.array(UNEVEN_SIDES)
+        p = 1.0 / sides
+        sizes = np.random.default_rng(0).choice(sides, 1000, p=p / p.sum()).tolist()
+        plan = batched.UnevenBatch(sizes)
+        g = torch.Generator(device=plan.device)
+        g.manual_seed(0)
+        for n in plan.classes:
+            plan.images[n].uniform_(0.0, 1.0, generator=g)
+        plan.capture()
+        for _ in range(3):
+            plan.replay()
+        torch.cuda.synchronize()
+        a = torch.empty(CALIB_BYTES, device=device, dtype=torch.uint8)  # calibration copy
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"op": "uneven", "images": 1000, "lanes": [list(map(str, lane))
+                                                                    for lane in plan.lanes]}),
+              flush=True)
         return
     if args.profile and args.op == "pcg":
         # preconditioned CG on the 1000 held-out 256x256 systems of the
