@@ -213,26 +213,46 @@ def cpu_baseline(host_imgs, budget_s):
     fn = R.adrt if kind == "reference" else O.adrt
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     one = np.ascontiguousarray(host_imgs[:1])
+    fn(one)  # warm-up (first touch of the library and the thread pool)
     t0 = time.perf_counter()
-    fn(one)  # warm-up (also first-touch of the library)
+    fn(one)
     est = time.perf_counter() - t0
-    reps = max(1, min(len(host_imgs), int(budget_s / max(est, 1e-3) / 3)))
+    runs = 5
+    reps = max(1, min(len(host_imgs), int(budget_s / max(est, 1e-3) / runs)))
     sample = np.ascontiguousarray(host_imgs[:reps])
     times = []
-    for _ in range(3):
+    for _ in range(runs):
         t0 = time.perf_counter()
         fn(sample)
         times.append(time.perf_counter() - t0)
-    med = sorted(times)[1]
+    med = sorted(times)[runs // 2]
     n = host_imgs.shape[-1]
-    return {
+    out = {
         "value": reps / med,
         "unit": "imgs/s",
         "cores": cores,
         "kind": kind,
         "sample": f"adrt of {reps} of the same {n}x{n} {host_imgs.dtype} phantoms, "
-                  f"median of 3 runs, OpenMP threads={cores}",
+                  f"median of {runs} runs after a warm-up, OpenMP threads={cores}",
     }
+    if kind == "reference":
+        # the same call on one thread (the reference's OpenMP runtime is
+        # libgomp, shared with this process): one image, median of 3
+        import ctypes
+
+        try:
+            gomp = ctypes.CDLL("libgomp.so.1")
+            gomp.omp_set_num_threads(1)
+            t1 = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                fn(one)
+                t1.append(time.perf_counter() - t0)
+            gomp.omp_set_num_threads(cores)
+            out["single_thread_imgs_per_s"] = 1.0 / sorted(t1)[1]
+        except OSError:
+            pass
+    return out
 
 
 def _csv_rows(dirs, suffix):
