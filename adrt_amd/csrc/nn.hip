@@ -175,6 +175,9 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_kernel(ConvArgs a) {
 // kernel above.  Pixel rows in LDS are padded to 40 channels (80 bytes), so
 // 16 lanes reading 16 consecutive pixels hit distinct banks.
 constexpr int kTH = 4, kCC = 32, kCCP = kCC + 8, kTWmax = 64;
+// 16-byte tile vectors per thread at most: (kTH + 2) (kTWmax + 2) kCC / 8
+// over kWgThreads threads
+constexpr int kFill = ((kTH + 2) * (kTWmax + 2) * (kCC / 8) + kWgThreads - 1) / kWgThreads;
 
 template <int NT, int MODE>
 __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int TW,
@@ -207,9 +210,18 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
         const int npix = (kTH + 2) * (TW + 2);
         __syncthreads();  // the previous chunk's readers are done
         // nvec is 1, 2 or 4 (shifts); pix / (TW + 2) by the exact reciprocal
-        // multiply (TW in {16, 32, 48, 64}, pix < 6 (TW + 2))
+        // multiply (TW in {16, 32, 48, 64}, pix < 6 (TW + 2)).  A thread's
+        // (at most kFill) vectors are all loaded before any is stored, so
+        // their loads are in flight together; vectors past the tile load
+        // the image's first pixel and are not stored.
         const int lg = nvec == 4 ? 2 : (nvec == 2 ? 1 : 0);
-        for (int i = threadIdx.x; i < npix * nvec; i += kWgThreads) {
+        const int nfill = npix * nvec;
+        bf16x8 fv[kFill];
+        int fdst[kFill];
+#pragma unroll
+        for (int j = 0; j < kFill; ++j) {
+            const int i0 = int(threadIdx.x) + j * kWgThreads;
+            const int i = i0 < nfill ? i0 : 0;
             const int v = i & (nvec - 1), pix = i >> lg;
             const int py = (pix * tw2_recip) >> 16, px = pix - py * (TW + 2);
             const int gy = y0 + py - 1, gx = x0 + px - 1;
@@ -223,39 +235,59 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
                 const long long q = ok ? img2 + (long long)(gy >> 1) * (a.W >> 1) + (gx >> 1) : img2;
                 src = a.x2 + q * a.x2_stride + (ch - a.c_split);
             }
-            *reinterpret_cast<bf16x8 *>(&tile[py][px][8 * v]) = keep_or_zero(load8(src), ok);
+            fv[j] = keep_or_zero(load8(src), ok);
+            fdst[j] = (py * (kTWmax + 2) + px) * kCCP + 8 * v;
         }
+#pragma unroll
+        for (int j = 0; j < kFill; ++j)
+            if (int(threadIdx.x) + j * kWgThreads < nfill)
+                *reinterpret_cast<bf16x8 *>(&tile[0][0][0] + fdst[j]) = fv[j];
         __syncthreads();
         const int Kc = 9 * ccv;
         int k = 8 * g;
         int tap = k / ccv, ci = k - tap * ccv;
+        // weight fragments of k (zero past Kc, read from tap 4, channel 0)
+        auto load_b = [&](int kk, int tp, int cn, bf16x8 (&Bo)[NT]) {
+            const bool ok = kk < Kc;
+            const int tc = ok ? tp : 4, cc = ok ? cn : 0;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                Bo[nt] = keep_or_zero(
+                    load8(a.w + (size_t)(co0 + 16 * nt + r) * K + tc * a.Cin + c0 + cc), ok);
+        };
+        bf16x8 Bf[NT];
+        load_b(k, tap, ci, Bf);
         for (int k0 = 0; k0 < Kc; k0 += 32) {
             const bool kok = k < Kc;
             const int tc = kok ? tap : 4;
             const int dy = tc / 3, dx = tc - (tc / 3) * 3;
             const int cc = kok ? ci : 0;
-            bf16x8 A[kMT], Bf[NT];
+            bf16x8 A[kMT];
 #pragma unroll
             for (int mt = 0; mt < kMT; ++mt)
                 if (mt < MT)
                     A[mt] = keep_or_zero(
                         *reinterpret_cast<const bf16x8 *>(&tile[wave + dy][16 * mt + r + dx][cc]),
                         kok);
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                Bf[nt] = keep_or_zero(
-                    load8(a.w + (size_t)(co0 + 16 * nt + r) * K + tc * a.Cin + c0 + cc), kok);
+            // the next step's weights are loaded before this step's products
+            const int kn = k + 32;
+            int tapn = tap, cin = ci + 32;
+            while (cin >= ccv) {
+                cin -= ccv;
+                ++tapn;
+            }
+            bf16x8 Bn[NT];
+            load_b(kn, tapn, cin, Bn);
 #pragma unroll
             for (int mt = 0; mt < kMT; ++mt)
                 if (mt < MT)
 #pragma unroll
                     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(A[mt], Bf[nt], acc[mt][nt]);
-            k += 32;
-            ci += 32;
-            while (ci >= ccv) {
-                ci -= ccv;
-                ++tap;
-            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) Bf[nt] = Bn[nt];
+            k = kn;
+            tap = tapn;
+            ci = cin;
         }
     }
 
