@@ -129,15 +129,15 @@ def adrt_bdrt_small(stacks, *, adrt=True, bdrt=True):
 def candidates(op, n, batch, per_image_bytes):
     """Distinct (variant, group_bytes) configurations for (op, n, batch).
 
-    The one-launch LDS kernel exists for n <= 64 (the default below 64, the
-    SINGLE variant at 64); the alternative stage split exists for odd log2 n
+    The one-launch LDS kernel is the default for n <= 64 (the fused kernels
+    are ALT_SPLIT there); the alternative stage split exists for odd log2 n
     in the fused range;
     group budgets are kept only when they change the images per launch group
     (``per_image_bytes``: the fused path's intermediate per image)."""
     L = int(n).bit_length() - 1
     out = [(_lib.VARIANT_STREAMING, 0)]
-    if small_range(n):  # below the fused range the default is the LDS kernel
-        out.append((_lib.VARIANT_SINGLE if fused_range(n) else _lib.VARIANT_DEFAULT, 0))
+    if small_range(n):  # up to 64 the default is the LDS kernel
+        out.append((_lib.VARIANT_DEFAULT, 0))
     if not fused_range(n):
         return out
     seen = set()
@@ -147,7 +147,10 @@ def candidates(op, n, batch, per_image_bytes):
         if g not in seen:
             seen.add(g)
             budgets.append(b)
-    variants = [_lib.VARIANT_DEFAULT] + ([_lib.VARIANT_ALT_SPLIT] if L % 2 == 1 else [])
+    if small_range(n):  # the fused kernels at 64: ALT_SPLIT (the same split)
+        variants = [_lib.VARIANT_ALT_SPLIT]
+    else:
+        variants = [_lib.VARIANT_DEFAULT] + ([_lib.VARIANT_ALT_SPLIT] if L % 2 == 1 else [])
     out += [(v, b) for v in variants for b in budgets]
     return out
 

@@ -1389,19 +1389,23 @@ int fused_variant(int op, size_t n) {
     return t ? t->variant.load(std::memory_order_relaxed) : ADRT_HIP_VARIANT_DEFAULT;
 }
 
-bool use_fused(int op, size_t n) {
-    const int v = fused_variant(op, n);
-    return fused_supported(n) && v != ADRT_HIP_VARIANT_STREAMING && v != ADRT_HIP_VARIANT_SINGLE;
-}
-
-// The one-launch LDS path: the default below the fused range, selectable up
-// to n = 64.
-bool use_small(int op, size_t B, size_t n) {
+// The one-launch LDS path: the default up to n = 64 (at 64 it measured as
+// fast as the fused kernels or faster: 40 / 37 us against 42 / 45 us for
+// forward / bdrt of 256 fp32 images, 55 / 61 against 75 / 77 us in fp64).
+static bool small_chosen(int op, size_t n) {
     if (op != ADRT_HIP_OP_ADRT && op != ADRT_HIP_OP_BDRT) return false;
     const int v = fused_variant(op, n);
-    const bool chosen = v == ADRT_HIP_VARIANT_SINGLE ||
-                        (v == ADRT_HIP_VARIANT_DEFAULT && !fused_supported(n));
-    return chosen && small_supported(B, n);
+    return v == ADRT_HIP_VARIANT_SINGLE || (v == ADRT_HIP_VARIANT_DEFAULT && n <= 64);
+}
+
+bool use_small(int op, size_t B, size_t n) { return small_chosen(op, n) && small_supported(B, n); }
+
+// The fused kernels: log2 n in [6, 12] unless streaming or the LDS path is
+// configured (at n = 64 they are reached through ALT_SPLIT, whose split
+// equals the default one for even log2 n).
+bool use_fused(int op, size_t n) {
+    const int v = fused_variant(op, n);
+    return fused_supported(n) && v != ADRT_HIP_VARIANT_STREAMING && !small_chosen(op, n);
 }
 
 int set_fused_config(int op, size_t n, int variant, size_t group_bytes) {

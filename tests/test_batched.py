@@ -22,7 +22,8 @@ def test_candidates_outside_fused_range():
                                                            (_lib.VARIANT_DEFAULT, 0)]
     assert batched.candidates(_lib.OP_BDRT, 8192, 2, 1) == [(_lib.VARIANT_STREAMING, 0)]
     at64 = batched.candidates(_lib.OP_ADRT, 64, 100, 1)
-    assert (_lib.VARIANT_SINGLE, 0) in at64 and (_lib.VARIANT_STREAMING, 0) in at64
+    assert (_lib.VARIANT_DEFAULT, 0) in at64 and (_lib.VARIANT_STREAMING, 0) in at64
+    assert any(v == _lib.VARIANT_ALT_SPLIT for v, _ in at64)  # the fused kernels
     assert all(v != _lib.VARIANT_SINGLE for v, _ in batched.candidates(_lib.OP_ADRT, 128, 9, 1))
 
 

@@ -99,9 +99,10 @@ def test_entry_points_reject_before_launch():
     assert lib.adrt_hip_fmg_highpass(z, z, 0, 1, 1, 5, z) == _lib.ERR_SHAPE
     assert lib.adrt_hip_truncate_combine(z, z, 0, 1, 4, 1.0, 3, z) == _lib.ERR_ARG
     one = ctypes.c_void_p(1)
-    need = lib.adrt_hip_workspace_bytes(_lib.OP_ADRT, 0, 2, 64)
+    need = lib.adrt_hip_workspace_bytes(_lib.OP_ADRT, 0, 2, 128)
     assert need > 0
-    assert lib.adrt_hip_adrt(one, one, z, 0, 0, 2, 64, z) == _lib.ERR_WORKSPACE
+    assert lib.adrt_hip_adrt(one, one, z, 0, 0, 2, 128, z) == _lib.ERR_WORKSPACE
+    assert lib.adrt_hip_workspace_bytes(_lib.OP_ADRT, 0, 2, 64) == 0  # LDS kernel
 
 
 def test_workspace_sizes_scale():
