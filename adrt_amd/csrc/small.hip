@@ -19,10 +19,11 @@
 // chunks of up to kMaxClasses size classes (the small sides of an uneven
 // batch), largest side first.  Chunk sizes are chosen so that every chunk
 // has the same 128 line segments of 64 elements per stage: one plane at
-// n = 64, four (one image) at n = 32, eight at n = 16, and so on.  Within every 32
-// consecutive slots the four planes of one n = 64 image are given to work-groups
-// of one XCD (work-group ids are dealt round-robin over the 8 XCDs), so the
-// image is read from HBM once and from that XCD's L2 three times.
+// n = 64, four (one image) at n = 32, eight at n = 16, and so on.  Within
+// every 32 consecutive slots the four planes of one n = 64 image are given
+// to work-groups of one XCD (work-group ids are dealt round-robin over the 8
+// XCDs), so the image is read from HBM once and from that XCD's L2 three
+// times.
 //
 // With ADRT_HOST_EMULATION defined only the phase functions are compiled, so
 // tests/host_emulation can run every chunk on the CPU.

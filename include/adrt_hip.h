@@ -119,8 +119,8 @@ int adrt_hip_bdrt(const void *in, void *out, void *workspace, size_t workspace_b
  * entry, images (batch, n, n) give adrt = adrt_hip_adrt(images) and
  * bdrt = adrt_hip_bdrt(adrt), both (batch, 4, 2n-1, n), bit-identical to
  * those calls; either output may be NULL (not stored), not both.  n a power
- * of two <= 64; 1 <= count <= 8; no workspace, no allocation, capturable in a
- * graph.  Entries of batch 0 are skipped.  max_groups > 0 caps the persistent
+ * of two <= 64; 1 <= count <= 8; outputs must not overlap any input or each
+ * other; no workspace, no allocation, capturable in a graph.  Entries of batch 0 are skipped.  max_groups > 0 caps the persistent
  * grid (rounded down to a multiple of 32, at least 32) to leave CUs to other
  * streams; 0 = every work-group that fits at once. */
 typedef struct adrt_hip_batch {
