@@ -1239,8 +1239,23 @@ bdrt_passB(const T *__restrict__ Z_base, T *__restrict__ out_base, int n, int kA
 // k1 = floor(L/2) (the same for even L).  Back-projection: pass A takes
 // kA = floor(L/2) (variant 0) or ceil(L/2) (variant 1).  For L in [6, 12]
 // every pass keeps 3..6 stages.
-inline int fwd_pass1_stages(int L, int variant) { return variant == 1 ? L / 2 : (L + 1) / 2; }
-inline int bdrt_passA_stages(int L, int variant) { return variant == 1 ? (L + 1) / 2 : L / 2; }
+// Stage splits (they differ only for odd L).  The default is the split that
+// measured faster for the size and element type (MI355X, ms per call):
+//   forward pass 1: floor(L/2) stages for fp64 or L <= 9 (fp64 2048^2:
+//     2.39 against 2.92; fp32 128^2: 0.081 against 0.132), else ceil(L/2)
+//     (fp32 2048^2: equal);
+//   bdrt pass A: ceil(L/2) for L <= 9 (fp32 128^2: 0.175 against 0.191;
+//     fp64 512^2: 1.01 against 1.05), else floor(L/2) (2048^2: 4.38
+//     against 4.83 in fp64, 1.62 against 1.86 in fp32).
+// ALT_SPLIT (variant 1) selects the other split.
+inline int fwd_pass1_stages(int L, int variant, int elem = 4) {
+    const bool low = elem == 8 || L <= 9;
+    return (variant == 1) != low ? L / 2 : (L + 1) / 2;
+}
+inline int bdrt_passA_stages(int L, int variant) {
+    const bool high = L <= 9;
+    return (variant == 1) != high ? (L + 1) / 2 : L / 2;
+}
 
 // Largest k1 of either variant: sizes X for both.
 inline int pass1_stages(int L) { return (L + 1) / 2; }
@@ -1440,7 +1455,7 @@ size_t fused_workspace(int dtype, size_t B, size_t n) {
 template <typename T>
 static int run_fused_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
     const int L = num_iters(n);
-    const int k1 = fused::fwd_pass1_stages(L, fused_variant(ADRT_HIP_OP_ADRT, n));
+    const int k1 = fused::fwd_pass1_stages(L, fused_variant(ADRT_HIP_OP_ADRT, n), int(sizeof(T)));
     const size_t per = fused::x_bytes_per_image(int(n), L, sizeof(T));
     const size_t G = fused::group_images(B, per, group_budget(ADRT_HIP_OP_ADRT, n));
     for (size_t first = 0; first < B; first += G) {

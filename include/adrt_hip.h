@@ -273,11 +273,13 @@ int adrt_hip_pcg_direction(void *p, const void *r, const void *z, void *state, v
 
 /* Variants of the forward and back-projection paths for one size:
  *   DEFAULT    n <= 64: SINGLE; log2 n in [7, 12]: the fused two-pass kernels
- *              with the default stage split (forward pass 1 takes ceil(L/2)
- *              of the L = log2 n stages, back-projection pass A floor(L/2));
+ *              with the default stage split of the L = log2 n stages (they
+ *              differ for odd L): forward pass 1 takes floor(L/2) stages for
+ *              float64 or L <= 9, else ceil(L/2); back-projection pass A
+ *              takes ceil(L/2) for L <= 9, else floor(L/2);
  *              n > 4096: streaming;
  *   ALT_SPLIT  log2 n in [6, 12]: the fused kernels with the other split
- *              (differs for odd L; at n = 64 the fused kernels' only split);
+ *              (at n = 64 the fused kernels' only split);
  *   STREAMING  one kernel per stage;
  *   SINGLE     n <= 64: one launch with every stage in LDS (no workspace).
  * and of the inverse (ADRT_HIP_OP_IADRT; the group budget does not apply):
