@@ -644,21 +644,39 @@ __global__ void __launch_bounds__(kWgThreads)
     const int ty = tt / tiles_x, tx = tt - ty * tiles_x;
     const int x0 = tx * kPyrTW, y0 = ty * kPyrTH;
     const float *img = x + size_t(b) * H * W;
-    for (int i = threadIdx.x; i < kPyrLH * kPyrLW; i += kWgThreads) {
+    // All of a thread's tile loads are issued before any is stored (points
+    // outside the image load the image's first pixel and store 0).
+    constexpr int kN = kPyrLH * kPyrLW, kFillPyr = (kN + kWgThreads - 1) / kWgThreads;
+    float fv[kFillPyr];
+#pragma unroll
+    for (int q = 0; q < kFillPyr; ++q) {
+        const int i0 = int(threadIdx.x) + q * kWgThreads;
+        const int i = i0 < kN ? i0 : 0;
         const int r = i / kPyrLW, c = i - r * kPyrLW;
         const int gy = y0 - 2 + r, gx = x0 - 2 + c;
-        t[r][c] = (unsigned(gy) < unsigned(H) && unsigned(gx) < unsigned(W))
-                      ? img[size_t(gy) * W + gx]
-                      : 0.f;
+        const bool ok = unsigned(gy) < unsigned(H) && unsigned(gx) < unsigned(W);
+        const float v = img[ok ? size_t(gy) * W + gx : 0];
+        fv[q] = ok ? v : 0.f;
     }
+    // thread -> columns 4j .. 4j+3, rows 2i, 2i+1 of the tile; rows 2i and
+    // 2i + 1 share their coarse parent row, columns 4j .. 4j+3 two parents
+    const int j = threadIdx.x & 15, i = threadIdx.x >> 4;
+    const int cx = x0 + 4 * j, ry = y0 + 2 * i;
+    const bool inside = cx < W && ry < H;
+    float par[2] = {0.f, 0.f};
+    if (coarse && inside) {
+        const float *cr = coarse + (size_t(b) * (H / 2) + ry / 2) * (W / 2) + cx / 2;
+        par[0] = cr[0];
+        if (cx + 2 < W) par[1] = cr[1];
+    }
+#pragma unroll
+    for (int q = 0; q < kFillPyr; ++q)
+        if (int(threadIdx.x) + q * kWgThreads < kN) (&t[0][0])[int(threadIdx.x) + q * kWgThreads] = fv[q];
     float cf[25];
 #pragma unroll
     for (int k = 0; k < 25; ++k) cf[k] = coef[k];
     __syncthreads();
-    // thread -> columns 4j .. 4j+3, rows 2i, 2i+1 of the tile
-    const int j = threadIdx.x & 15, i = threadIdx.x >> 4;
-    const int cx = x0 + 4 * j, ry = y0 + 2 * i;
-    if (cx >= W || ry >= H) return;
+    if (!inside) return;
     float win[6][8];
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
@@ -688,10 +706,9 @@ __global__ void __launch_bounds__(kWgThreads)
             o[ox] = s;
         }
         if (coarse) {
-            const float *cr = coarse + (size_t(b) * (H / 2) + yy / 2) * (W / 2) + cx / 2;
 #pragma unroll
             for (int ox = 0; ox < 4; ++ox)
-                if (cx + ox < W) o[ox] += 0.25f * cr[ox / 2];
+                if (cx + ox < W) o[ox] += 0.25f * par[ox / 2];
         }
         float *dst = y + (size_t(b) * H + yy) * W + cx;
         if (vec_store && cx + 4 <= W) {
