@@ -883,19 +883,31 @@ struct BdrtA {
     }
 
     // stg[a][rr] = y[min(c0 + rr, C - 1)][p * 2^kA + a], rr < 256, a < 2^kA.
+    // A thread's vectors are all loaded before any is stored, so its loads
+    // are in flight together.
     __host__ __device__ __forceinline__ static void stage(const Item &it, T *lds, int tid) {
         constexpr int vecs = kSA / 4;  // 4-element vectors per input row
-        for (int f = tid; f < kLanes * kW * vecs; f += kThreads) {
+        constexpr int per = kLanes * kW * vecs / kThreads;
+        static_assert(per * kThreads == kLanes * kW * vecs, "whole vectors per thread");
+        T v[per][4];
+#pragma unroll
+        for (int q = 0; q < per; ++q) {
+            const int f = tid + q * kThreads;
             const int rr = f / vecs, vq = f - rr * vecs;
             const int r = it.c0 + rr;
             const int rc = r < it.C ? r : it.C - 1;
-            T a, b, c, d;
-            load_vec4(it.y + unsigned(rc * it.n + (it.p << kA) + 4 * vq), a, b, c, d);
+            load_vec4(it.y + unsigned(rc * it.n + (it.p << kA) + 4 * vq), v[q][0], v[q][1],
+                      v[q][2], v[q][3]);
+        }
+#pragma unroll
+        for (int q = 0; q < per; ++q) {
+            const int f = tid + q * kThreads;
+            const int rr = f / vecs, vq = f - rr * vecs;
             T *col = lds + (4 * vq) * kSP + rr;
-            col[0] = a;
-            col[kSP] = b;
-            col[2 * kSP] = c;
-            col[3 * kSP] = d;
+            col[0] = v[q][0];
+            col[kSP] = v[q][1];
+            col[2 * kSP] = v[q][2];
+            col[3 * kSP] = v[q][3];
         }
     }
 
