@@ -627,6 +627,25 @@ __global__ void __launch_bounds__(kWgThreads)
     y[size_t(R) * Wc + xq] = 0.25f * ((a.x + a.y) + (c.x + c.y));
 }
 
+// The same average, two outputs per thread from 16-byte loads, for w / 4 a
+// power of two (2^lq pairs per output row) and 16-byte aligned x, 8-byte
+// aligned y; thread p -> output row p >> lq, pair p & (2^lq - 1).
+__global__ void __launch_bounds__(kWgThreads)
+    pyr_pool4_kernel(const float *__restrict__ x, float *__restrict__ y, int W, int lq,
+                     long long pairs) {
+    const long long p = (long long)blockIdx.x * kWgThreads + threadIdx.x;
+    if (p >= pairs) return;
+    const long long R = p >> lq;
+    const int xp = int(p & ((1LL << lq) - 1));
+    const float *top = x + size_t(2 * R) * W + 4 * xp;
+    const float4 a = *reinterpret_cast<const float4 *>(top);
+    const float4 c = *reinterpret_cast<const float4 *>(top + W);
+    float2 o;
+    o.x = 0.25f * ((a.x + a.y) + (c.x + c.y));
+    o.y = 0.25f * ((a.z + a.w) + (c.z + c.w));
+    *reinterpret_cast<float2 *>(y + size_t(R) * (W / 2) + 2 * xp) = o;
+}
+
 // y = h * x (5x5, zero padding) + coarse[parent] / 4 (coarse may be null),
 // one block per 64 x 32 tile of one image.  The tile and its 2-pixel halo
 // are staged in LDS (zeros outside the image); each thread computes a 4 x 2
@@ -1091,6 +1110,18 @@ int adrt_hip_pyr_pool(const float *x, float *y, int batch, int h, int w, void *s
     if (batch < 1 || h < 2 || w < 2 || h % 2 || w % 2) return ADRT_HIP_ERR_ARG;
     if (!x || !y) return ADRT_HIP_ERR_NULL;
     if (reinterpret_cast<uintptr_t>(x) % 8) return ADRT_HIP_ERR_ARG;  // float2 loads
+    const int q = w / 4;
+    if (w % 4 == 0 && (q & (q - 1)) == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(y) % 8 == 0) {
+        int lq = 0;
+        while ((1 << lq) < q) ++lq;
+        const long long pairs = (long long)batch * (h / 2) * q;
+        const long long nb = (pairs + kWgThreads - 1) / kWgThreads;
+        if (nb > 0x7fffffffLL) return ADRT_HIP_ERR_TOO_BIG;
+        hipLaunchKernelGGL(pyr_pool4_kernel, dim3(unsigned(nb), 1, 1), dim3(kWgThreads), 0,
+                           static_cast<hipStream_t>(stream), x, y, w, lq, pairs);
+        return launch_status();
+    }
     const int chunks = (w / 2 + kWgThreads - 1) / kWgThreads;
     const long long blocks = (long long)batch * (h / 2) * chunks;
     if (blocks > 0x7fffffffLL) return ADRT_HIP_ERR_TOO_BIG;

@@ -480,7 +480,8 @@ class TestPreconditioner:
         assert np.all(np.isfinite(got))
         assert np.allclose(got, want, rtol=1e-5, atol=1e-5 * np.abs(want).max())
 
-    @pytest.mark.parametrize("shape", [(2, 6, 130), (3, 256, 256), (1, 2, 2), (4, 2, 600)])
+    @pytest.mark.parametrize("shape", [(2, 6, 130), (3, 256, 256), (1, 2, 2), (4, 2, 600),
+                                       (5, 4, 4), (2, 8, 64), (1000, 2, 8)])
     def test_pool_level_matches_numpy(self, cuda, shape):
         """2 x 2 averaging 0.25 ((a + b) + (c + d)): exact in float32 order."""
         import torch
