@@ -10,10 +10,10 @@ go onto at most 4 streams, and ``capture()`` records the whole pipeline once
 as a hipGraph that ``replay()`` relaunches with no host work.
 
 ``tune()`` is the autotuner: for one (op, n, batch) it times every kernel
-variant the library offers for that size (the fused two-pass kernels with
-either stage split, i.e. the number of stages fused in LDS per pass, and the
-streaming one-kernel-per-stage path) and every launch-group size that makes a
-difference, with HIP events on data of that shape, then leaves the fastest
+variant the library offers for that size (the one-launch LDS kernel for
+n <= 64, the fused two-pass kernels with either stage split, i.e. the number
+of stages fused in LDS per pass, and the streaming one-kernel-per-stage
+path) and every launch-group size that makes a difference, with HIP events on data of that shape, then leaves the fastest
 configuration selected in the library.  Results are bit-identical for every
 configuration; only speed differs.
 
