@@ -238,6 +238,14 @@ def test_small_launch_argument_checks():
     assert lib.adrt_hip_adrt_bdrt_small(arr, 1, _lib.F32, 0, None) == _lib.OK
 
 
+def _same_signed(a, b):
+    """Bitwise equal including signs of zero; NaNs match by position (their
+    payload depends on the order of two NaN operands)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return (a.shape == b.shape and np.array_equal(na, nb)
+            and np.array_equal(np.where(na, 0, a).view(np.uint8), np.where(nb, 0, b).view(np.uint8)))
+
+
 @gpu
 class TestSmallLaunch:
     """All sides <= 64 in one persistent launch of the LDS kernel."""
@@ -266,6 +274,9 @@ class TestSmallLaunch:
             want = O.adrt(xn)
             assert_bitwise(f.cpu().numpy(), want, f"adrt {tuple(x.shape)}")
             assert_bitwise(k.cpu().numpy(), O.bdrt(want), f"bdrt {tuple(x.shape)}")
+            # the LDS kernel also keeps the oracle's signs of zero (NaNs by position)
+            assert _same_signed(f.cpu().numpy(), want), f"adrt {tuple(x.shape)}"
+            assert _same_signed(k.cpu().numpy(), O.bdrt(want)), f"bdrt {tuple(x.shape)}"
             # the library's single-op calls (fused kernels at 64) agree bit for
             # bit (inf - inf makes NaNs here, which torch.equal calls unequal)
             assert_bitwise(f.cpu().numpy(), _kernels.adrt(x).cpu().numpy())
