@@ -323,6 +323,31 @@ class TestSmallLaunch:
             for (_, f, k), (wf, wk) in zip(entries, want):
                 assert torch.equal(f, wf) and torch.equal(k, wk), rep
 
+    def test_capped_grid_strides_over_chunks(self, cuda):
+        """A grid capped at 32 work-groups runs ~15 chunks per work-group
+        through the persistent loop (and the slot permutation's partial last
+        group): the same bytes as the default grid and as the oracle."""
+        import torch
+
+        g = torch.Generator(device=cuda)
+        g.manual_seed(9)
+        stacks = [torch.randn((101, 64, 64), generator=g, device=cuda),
+                  torch.randn((37, 32, 32), generator=g, device=cuda),
+                  torch.randn((3, 8, 8), generator=g, device=cuda)]
+        want = batched.adrt_bdrt_small(stacks)
+        entries = []
+        for x in stacks:
+            b, n = x.shape[0], x.shape[-1]
+            entries.append((x, torch.full((b, 4, 2 * n - 1, n), float("nan"), device=cuda),
+                            torch.full((b, 4, 2 * n - 1, n), float("nan"), device=cuda)))
+        arr = batched._batch_array(entries)
+        batched._launch_small(arr, _lib.F32, torch.cuda.current_stream(cuda), max_groups=32)
+        torch.cuda.synchronize()
+        for (x, f, k), (wf, wk) in zip(entries, want):
+            assert torch.equal(f, wf) and torch.equal(k, wk), tuple(x.shape)
+        xn = stacks[1][:2].cpu().numpy()
+        assert _same_signed(entries[1][1][:2].cpu().numpy(), O.adrt(xn))
+
     def test_variant_single_at_64(self, cuda, reset_config):
         """adrt/bdrt at n = 64 configured to the LDS kernel: no workspace and
         the fused path's results."""
