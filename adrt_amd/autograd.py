@@ -25,6 +25,48 @@ def _torch():
     return dev.torch()
 
 
+# Argument checks in the reference's order (adrt_cdefs_py.cpp:79-154,
+# 264-289): array type, ndim, zero dimensions, shape, then device and dtype;
+# the messages are the reference's.  Nothing reaches a kernel unchecked, so a
+# wrong shape cannot make the library read past a buffer.
+def _check_dims(t, lo, hi):
+    if not dev.is_tensor(t):
+        raise TypeError(f"array must be a torch tensor, but got {type(t).__name__}")
+    if not lo <= t.dim() <= hi:
+        raise ValueError(f"array must have between {lo} and {hi} dimensions, but had {t.dim()}")
+    for i, size in enumerate(t.shape):
+        if size == 0:
+            raise ValueError(
+                f"all array dimensions must be nonzero, but found zero in dimension {i}")
+
+
+def _check_device_dtype(t):
+    if not t.is_cuda:
+        raise TypeError("array must be a HIP device tensor, but got a host tensor")
+    if dev.dtype_code(t) is None:
+        raise TypeError(f"unsupported array dtype {t.dtype}")
+
+
+def _check_image(x):
+    """(n, n) or (B, n, n), n a power of two: the shapes adrt accepts
+    (adrt_cdefs_py.cpp:275-289)."""
+    _check_dims(x, 2, 3)
+    n = int(x.shape[-1])
+    if int(x.shape[-2]) != n or n & (n - 1):
+        raise ValueError("array must be square with a power of two shape")
+    _check_device_dtype(x)
+
+
+def _check_adrt_data(y):
+    """(4, 2n-1, n) or (B, 4, 2n-1, n), n a power of two: the shapes bdrt
+    accepts (adrt_cdefs_py.cpp:482-495, adrt_cdefs_common.cpp:185-192)."""
+    _check_dims(y, 3, 4)
+    n = int(y.shape[-1])
+    if int(y.shape[-3]) != 4 or int(y.shape[-2]) != 2 * n - 1 or n & (n - 1):
+        raise ValueError("array must have a valid ADRT output shape")
+    _check_device_dtype(y)
+
+
 def _adjoint(g):
     """A^T g = sum over quadrants of truncate(bdrt(g)): (..., 4, 2n-1, n) -> (..., n, n)."""
     return _kernels.truncate_combine(_kernels.bdrt(g.contiguous()), 1.0, _kernels.SUM)
@@ -74,13 +116,19 @@ def _get_classes():
 
 
 def adrt(x):
-    """Differentiable forward ADRT of a device tensor (..., n, n)."""
+    """Differentiable forward ADRT of a device tensor (n, n) or (B, n, n)
+    (n a power of two, float32 or float64).  Other shapes raise ValueError
+    and other dtypes or host tensors TypeError, before any device work."""
+    _check_image(x)
     dev.require_gpu()
     return _get_classes()[0].apply(x)
 
 
 def bdrt(y):
-    """Differentiable back-projection of a device tensor (..., 4, 2n-1, n)."""
+    """Differentiable back-projection of a device tensor (4, 2n-1, n) or
+    (B, 4, 2n-1, n) (n a power of two, float32 or float64).  Other shapes
+    raise ValueError and other dtypes or host tensors TypeError."""
+    _check_adrt_data(y)
     dev.require_gpu()
     return _get_classes()[1].apply(y)
 
@@ -114,6 +162,7 @@ def lsq_cg(b, x0=None, *, iters=200, callback=None, check_every=16):
     backward of a linear map does not depend on the point), so no forward is
     run only to reach its backward."""
     torch = _torch()
+    _check_adrt_data(b)
     dev.require_gpu()
     single = b.dim() == 3
     bb = (b.unsqueeze(0) if single else b).contiguous()

@@ -33,6 +33,7 @@
 // with ADRT_HOST_EMULATION defined only those and the whole-strip compute are
 // compiled, so tests/host_emulation can run every block on the CPU.
 #include <atomic>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -1288,95 +1289,118 @@ inline size_t x_bytes_per_image(int n, int L, size_t elem) {
 }
 
 #ifndef ADRT_HOST_EMULATION
-// One work-group per item, rounded up to whole rows of the 8 XCDs.  The LDS
-// limit is raised once per device for kernels above the default 64 KB.
-template <typename K>
-static unsigned launch_grid(K kern, size_t lds, int items) {
+// Raises the dynamic-LDS limit of kernel `Kern` to `lds` bytes, once per
+// device and kernel (each kernel address is its own instantiation, so the
+// once-flags are per kernel).  A failure is cleared from the runtime's
+// error state and returned, so the caller reports it instead of launching.
+template <auto Kern>
+static hipError_t lds_limit(size_t lds) {
+    if (lds == 0) return hipSuccess;
     constexpr int kMaxDevices = 64;
-    static bool attr_set[kMaxDevices];
+    static std::once_flag once[kMaxDevices];
+    static hipError_t result[kMaxDevices];
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
-    if (lds > 0 && !attr_set[dev]) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        attr_set[dev] = true;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return hipErrorInvalidDevice;
     }
-    return unsigned((items + kXcds - 1) / kXcds * kXcds);
+    std::call_once(once[dev], [&] {
+        result[dev] = hipFuncSetAttribute(reinterpret_cast<const void *>(Kern),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (result[dev] != hipSuccess) (void)hipGetLastError();
+    });
+    return result[dev];
 }
 
+// One work-group per item, rounded up to whole rows of the 8 XCDs.
+static unsigned launch_grid(int items) { return unsigned((items + kXcds - 1) / kXcds * kXcds); }
+
 template <typename T, int KB>
-static void launch_pass1(const T *img, T *X, int n, int first, int count, hipStream_t s) {
+static hipError_t launch_pass1(const T *img, T *X, int n, int first, int count, hipStream_t s) {
     using P = Pass1<T, KB>;
     const int items = P::num_items(n, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
-    auto kern = fwd_pass1<T, KB>;
-    const unsigned grid = launch_grid(kern, lds, items);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, img, X, n, first, items);
+    const hipError_t e = lds_limit<fwd_pass1<T, KB>>(lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((fwd_pass1<T, KB>), dim3(launch_grid(items)), dim3(kThreads), lds, s, img,
+                       X, n, first, items);
+    return hipSuccess;
 }
 
 template <typename T, int KD>
-static void launch_pass2(const T *X, T *out, int n, int k1, int first, int count,
-                         hipStream_t s) {
+static hipError_t launch_pass2(const T *X, T *out, int n, int k1, int first, int count,
+                               hipStream_t s) {
     using P = Pass2<T, KD>;
     const int items = P::num_items(n, k1, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
-    auto kern = fwd_pass2<T, KD>;
-    const unsigned grid = launch_grid(kern, lds, items);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, X, out, n, k1, first, items);
+    const hipError_t e = lds_limit<fwd_pass2<T, KD>>(lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((fwd_pass2<T, KD>), dim3(launch_grid(items)), dim3(kThreads), lds, s, X,
+                       out, n, k1, first, items);
+    return hipSuccess;
 }
 
 template <typename T>
-static void run_group(const T *img, T *X, T *out, int n, int L, int k1, int first, int count,
-                      hipStream_t s) {
+static hipError_t run_group(const T *img, T *X, T *out, int n, int L, int k1, int first,
+                            int count, hipStream_t s) {
+    hipError_t e;
     switch (k1 - 3) {
-        case 0: launch_pass1<T, 0>(img, X, n, first, count, s); break;
-        case 1: launch_pass1<T, 1>(img, X, n, first, count, s); break;
-        case 2: launch_pass1<T, 2>(img, X, n, first, count, s); break;
-        default: launch_pass1<T, 3>(img, X, n, first, count, s); break;
+        case 0: e = launch_pass1<T, 0>(img, X, n, first, count, s); break;
+        case 1: e = launch_pass1<T, 1>(img, X, n, first, count, s); break;
+        case 2: e = launch_pass1<T, 2>(img, X, n, first, count, s); break;
+        default: e = launch_pass1<T, 3>(img, X, n, first, count, s); break;
     }
+    if (e != hipSuccess) return e;
     switch (L - k1 - 3) {
-        case 0: launch_pass2<T, 0>(X, out, n, k1, first, count, s); break;
-        case 1: launch_pass2<T, 1>(X, out, n, k1, first, count, s); break;
-        case 2: launch_pass2<T, 2>(X, out, n, k1, first, count, s); break;
-        default: launch_pass2<T, 3>(X, out, n, k1, first, count, s); break;
+        case 0: return launch_pass2<T, 0>(X, out, n, k1, first, count, s);
+        case 1: return launch_pass2<T, 1>(X, out, n, k1, first, count, s);
+        case 2: return launch_pass2<T, 2>(X, out, n, k1, first, count, s);
+        default: return launch_pass2<T, 3>(X, out, n, k1, first, count, s);
     }
 }
 
 template <typename T, int K2>
-static void launch_bdrt_a(const T *y, T *Z, int n, int first, int count, hipStream_t s) {
+static hipError_t launch_bdrt_a(const T *y, T *Z, int n, int first, int count, hipStream_t s) {
     using P = BdrtA<T, K2>;
     const int items = P::num_items(n, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
-    auto kern = bdrt_passA<T, K2>;
-    const unsigned grid = launch_grid(kern, lds, items);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, y, Z, n, first, items);
+    const hipError_t e = lds_limit<bdrt_passA<T, K2>>(lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((bdrt_passA<T, K2>), dim3(launch_grid(items)), dim3(kThreads), lds, s, y,
+                       Z, n, first, items);
+    return hipSuccess;
 }
 
 template <typename T, int K4>
-static void launch_bdrt_b(const T *Z, T *out, int n, int kA, int first, int count, hipStream_t s) {
+static hipError_t launch_bdrt_b(const T *Z, T *out, int n, int kA, int first, int count,
+                                hipStream_t s) {
     using P = BdrtB<T, K4>;
     const int items = P::num_items(n, kA, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
-    auto kern = bdrt_passB<T, K4>;
-    const unsigned grid = launch_grid(kern, lds, items);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, Z, out, n, kA, first, items);
+    const hipError_t e = lds_limit<bdrt_passB<T, K4>>(lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((bdrt_passB<T, K4>), dim3(launch_grid(items)), dim3(kThreads), lds, s, Z,
+                       out, n, kA, first, items);
+    return hipSuccess;
 }
 
 // bdrt split: pass A takes kA stages, pass B kB = L - kA.
 template <typename T>
-static void run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int kA, int first, int count,
-                           hipStream_t s) {
+static hipError_t run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int kA, int first,
+                                 int count, hipStream_t s) {
+    hipError_t e;
     switch (kA - 3) {
-        case 0: launch_bdrt_a<T, 0>(y, Z, n, first, count, s); break;
-        case 1: launch_bdrt_a<T, 1>(y, Z, n, first, count, s); break;
-        case 2: launch_bdrt_a<T, 2>(y, Z, n, first, count, s); break;
-        default: launch_bdrt_a<T, 3>(y, Z, n, first, count, s); break;
+        case 0: e = launch_bdrt_a<T, 0>(y, Z, n, first, count, s); break;
+        case 1: e = launch_bdrt_a<T, 1>(y, Z, n, first, count, s); break;
+        case 2: e = launch_bdrt_a<T, 2>(y, Z, n, first, count, s); break;
+        default: e = launch_bdrt_a<T, 3>(y, Z, n, first, count, s); break;
     }
+    if (e != hipSuccess) return e;
     switch (L - kA - 3) {
-        case 0: launch_bdrt_b<T, 0>(Z, out, n, kA, first, count, s); break;
-        case 1: launch_bdrt_b<T, 1>(Z, out, n, kA, first, count, s); break;
-        case 2: launch_bdrt_b<T, 2>(Z, out, n, kA, first, count, s); break;
-        default: launch_bdrt_b<T, 3>(Z, out, n, kA, first, count, s); break;
+        case 0: return launch_bdrt_b<T, 0>(Z, out, n, kA, first, count, s);
+        case 1: return launch_bdrt_b<T, 1>(Z, out, n, kA, first, count, s);
+        case 2: return launch_bdrt_b<T, 2>(Z, out, n, kA, first, count, s);
+        default: return launch_bdrt_b<T, 3>(Z, out, n, kA, first, count, s);
     }
 }
 #endif  // ADRT_HOST_EMULATION
@@ -1472,7 +1496,12 @@ static int run_fused_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream
     const size_t G = fused::group_images(B, per, group_budget(ADRT_HIP_OP_ADRT, n));
     for (size_t first = 0; first < B; first += G) {
         const size_t count = (B - first) < G ? (B - first) : G;
-        fused::run_group<T>(in, ws, out, int(n), L, k1, int(first), int(count), s);
+        const hipError_t e =
+            fused::run_group<T>(in, ws, out, int(n), L, k1, int(first), int(count), s);
+        if (e != hipSuccess) {
+            set_last_error(e);
+            return ADRT_HIP_ERR_DEVICE;
+        }
     }
     return launch_status();
 }
@@ -1499,7 +1528,12 @@ static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, hipS
     const size_t G = fused::group_images(B, per, group_budget(ADRT_HIP_OP_BDRT, n));
     for (size_t first = 0; first < B; first += G) {
         const size_t count = (B - first) < G ? (B - first) : G;
-        fused::run_bdrt_group<T>(in, ws, out, int(n), L, kA, int(first), int(count), s);
+        const hipError_t e =
+            fused::run_bdrt_group<T>(in, ws, out, int(n), L, kA, int(first), int(count), s);
+        if (e != hipSuccess) {
+            set_last_error(e);
+            return ADRT_HIP_ERR_DEVICE;
+        }
     }
     return launch_status();
 }

@@ -766,12 +766,17 @@ def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=False, compa
     return x_out, iters_out, conv_out, apps
 
 
-def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
+def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5):
     """Wall time to relative residual ``rtol`` on ``count`` held-out phantoms
     (phantom seed ``seed``, noise seed ``seed``) for plain CG, CG with the
     multiscale preconditioner from zero, and the learned method (U-Net initial
     guess + preconditioned CG).  Each solve is timed from the right-hand side
-    to the converged batch, device synchronised on both sides."""
+    to the converged batch, device synchronised on both sides; the three
+    methods are run ``repeats`` times in interleaved rounds (plain, learned,
+    preconditioner only, network alone), and each reports the minimum,
+    median and maximum of its times.  The speedups are ratios of medians
+    (and of minima).  Iteration counts and residuals are those of the last
+    round (every round computes the same iterates)."""
     import time
 
     from . import batched
@@ -790,13 +795,20 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
         torch.cuda.synchronize()
         return out, time.perf_counter() - t0
 
-    def stats(name, res, secs):
+    def spread(ts):
+        ts = sorted(ts)
+        return {"seconds_min": round(ts[0], 4), "seconds_median": round(ts[len(ts) // 2], 4),
+                "seconds_max": round(ts[-1], 4)}
+
+    def stats(res, ts):
         xs, it, conv, apps = res
         resid = (bdot(b - normal_op(xs), b - normal_op(xs)).sqrt() / bn)
-        return {"seconds": round(secs, 4), "iterations_mean": round(float(it.float().mean()), 2),
-                "iterations_max": int(it.max()), "image_operator_applications": apps,
-                "converged": int(conv.sum()), "max_rel_residual": float(resid.max()),
-                "rel_error_vs_phantom": float((bdot(xs - x, xs - x) / bdot(x, x)).sqrt().mean())}
+        out = {"seconds": round(sorted(ts)[len(ts) // 2], 4), **spread(ts),
+               "iterations_mean": round(float(it.float().mean()), 2),
+               "iterations_max": int(it.max()), "image_operator_applications": apps,
+               "converged": int(conv.sum()), "max_rel_residual": float(resid.max()),
+               "rel_error_vs_phantom": float((bdot(xs - x, xs - x) / bdot(x, x)).sqrt().mean())}
+        return out
 
     # one-time costs (kernel loading, first graph captures, the network's
     # buffers) are paid before any timed solve
@@ -804,24 +816,50 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500):
     solver.initial_guess(b, train=False)
     pcg(normal_op, b, rtol=rtol, maxiter=3)
     pcg(normal_op, b, prec=solver.pyramid, rtol=rtol, maxiter=3)
-    plain, t_plain = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter))
-    learned, t_learned = run(lambda: pcg(normal_op, b, solver.initial_guess(b, train=False),
-                                         prec=solver.pyramid, rtol=rtol, maxiter=maxiter))
-    prec_only, t_prec = run(lambda: pcg(normal_op, b, prec=solver.pyramid, rtol=rtol,
-                                        maxiter=maxiter))
-    x0, t_net = run(lambda: solver.initial_guess(b, train=False))
+    times = {"plain": [], "learned": [], "prec": [], "net": []}
+    for _ in range(max(int(repeats), 1)):
+        plain, t = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter))
+        times["plain"].append(t)
+        learned, t = run(lambda: pcg(normal_op, b, solver.initial_guess(b, train=False),
+                                     prec=solver.pyramid, rtol=rtol, maxiter=maxiter))
+        times["learned"].append(t)
+        prec_only, t = run(lambda: pcg(normal_op, b, prec=solver.pyramid, rtol=rtol,
+                                       maxiter=maxiter))
+        times["prec"].append(t)
+        x0, t = run(lambda: solver.initial_guess(b, train=False))
+        times["net"].append(t)
     xs = plain[0]
+    # FP64 judge (untimed): plain CG in float64 on the first `judge` systems
+    # to a tight tolerance; the learned float32 solutions' relative distance
+    # to it
+    from . import cg as _cg
+
+    judge = min(64, count)
+    b64 = b[:judge].double()
+    x64, _, _ = _cg.cg_solve(_cg.ADRTNormalOperator(solver.n), b64, rtol=1e-10, maxiter=2000)
+    d = learned[0][:judge].double() - x64
+    fp64_err = float((bdot(d, d) / bdot(x64, x64)).sqrt().max())
+
+    def med(k):
+        return sorted(times[k])[len(times[k]) // 2]
+
     out = {
-        "count": count, "n": solver.n, "rtol": rtol, "tuned_adrt_bdrt": tuning,
-        "plain_cg": stats("plain", plain, t_plain),
-        "preconditioner_only": stats("prec", prec_only, t_prec),
-        "learned": stats("learned", learned, t_learned),
-        "network_seconds": round(t_net, 4),
+        "count": count, "n": solver.n, "rtol": rtol, "repeats": len(times["plain"]),
+        "tuned_adrt_bdrt": tuning,
+        "plain_cg": stats(plain, times["plain"]),
+        "preconditioner_only": stats(prec_only, times["prec"]),
+        "learned": stats(learned, times["learned"]),
+        "network_seconds": round(med("net"), 4),
         "initial_guess_rel_error": float((bdot(x0 - x, x0 - x) / bdot(x, x)).sqrt().mean()),
         "initial_guess_rel_error_vs_solution": float(
             (bdot(x0 - xs, x0 - xs) / bdot(xs, xs)).sqrt().mean()),
-        "speedup_learned_vs_plain": round(t_plain / t_learned, 3),
-        "speedup_preconditioner_only_vs_plain": round(t_plain / t_prec, 3),
+        "speedup_learned_vs_plain": round(med("plain") / med("learned"), 3),
+        "speedup_learned_vs_plain_min_times": round(min(times["plain"]) / min(times["learned"]),
+                                                    3),
+        "speedup_preconditioner_only_vs_plain": round(med("plain") / med("prec"), 3),
+        "fp64_error": fp64_err,
+        "fp64_error_basis": f"max over the first {judge} images of ||x - x64|| / ||x64||, x64 "
+                            "= float64 plain CG to relative residual 1e-10",
         "learned_vs_plain_solution_rel_diff": float(
             (bdot(learned[0] - plain[0], learned[0] - plain[0]) / bdot(plain[0], plain[0]))
             .sqrt().max()),

@@ -150,16 +150,25 @@ def coord_adrt(n, /):
     height 1 - (h + 1/2)/n on the quadrant's axis, and its signed distance to
     the centre is ((height + (2n-1)/(2n) * slope) / (1 + slope) - 1/2) *
     (cos + sin) of the base angle.  Quadrants 1 and 3 mirror the offsets;
-    the angles of quadrants 0..3 are base - pi/2, -base, base, pi/2 - base.
+    the angles of quadrants 0..3 are base - pi/2, -base, base, -(base - pi/2).
+
+    The slopes and heights are built as the reference builds them, with
+    NumPy's ``linspace`` (slope k = k * fl(1/(n-1)), last entry exactly 1;
+    height h = 1 + h * step + step/2 with step = ((1-n)/n - 1)/(2n-1)), and
+    every later operation is the reference's in its order, so the values are
+    the reference's bit for bit (k/(n-1) rounds differently for some k).
     """
     n = _domain_size(n)
-    slope = np.arange(n, dtype=np.float64) / (n - 1)
+    height, step = np.linspace(1, (1 - n) / n, num=2 * n - 1, endpoint=False, retstep=True,
+                               dtype=np.float64)
+    height += step / 2
+    slope = np.linspace(0, 1, num=n, endpoint=True, dtype=np.float64)
     base = np.arctan(slope)
-    height = 1.0 - (np.arange(2 * n - 1, dtype=np.float64) + 0.5) / n
-    line_pos = (height[:, None] + ((2 * n - 1) / (2 * n)) * slope[None, :]) / (1.0 + slope[None, :])
-    dist = (line_pos - 0.5) * (np.cos(base) + np.sin(base))[None, :]
+    line_pos = np.add.outer(height, ((2 * n - 1) / (2 * n)) * slope) / (1 + slope)
+    dist = (line_pos - 0.5) * (np.cos(base) + np.sin(base))
     offset = np.stack([dist, -dist, dist, -dist])
-    angle = np.stack([base - np.pi / 2, -base, base, np.pi / 2 - base])[:, None, :]
+    below = base - np.pi / 2
+    angle = np.stack([below, -base, base, -below])[:, None, :]
     return ADRTCoord(offset, angle)
 
 

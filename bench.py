@@ -7,13 +7,33 @@ the stream the kernels run on (torch's current stream), which gives the
 per-call device time used for the roofline.
 
 Roofline basis (compulsory traffic, each input read once and each output
-written once): B_fwd(N) = s * (N^2 + 4 N (2N - 1)) bytes per image;
-achieved = batch * B_fwd / (device time per call); peak = 8000 GB/s.
+written once): B_fwd(N) = s * (N^2 + 4 N (2N - 1)) bytes per image,
+B_bdrt(N) = B_iadrt(N) = 2 s 4N(2N - 1); achieved = batch * B / (device
+time per call); peak = 8000 GB/s.
+
+Besides the headline line's own fields, the default run reports (key
+"legs"), each with images/s, algorithmic GB/s, the fraction of peak, a
+byte check of image 0 against the oracle in the same dtype and the error of
+image 0 against an FP64 oracle run on the same input ("fp64_error" =
+max |result - FP64| / max |FP64|):
+  * bdrt and iadrt of the forward's output (the rest of the 2048^2 round
+    trip), then forward, bdrt and iadrt of the same phantoms in the other
+    dtype;
+  * the uneven-size batch (1000 float32 images, sides 32..1024 with
+    p ~ 1/n, forward + bdrt, autotuned, one hipGraph replay);
+  * least squares through autograd (32 x 512^2 float32, 200 CGLS steps),
+    with the float64 run of the same iteration as its FP64 judge;
+and (key "cg_time_to_1e-4") the learned-vs-plain CG time to relative
+residual 1e-4 on 1000 held-out 256^2 systems, 5 interleaved rounds, min /
+median / max per method.  The iadrt's fp64_error in float32 is large by the
+nature of the exact inverse (it amplifies the input's rounding error; the
+reference's own float32 inverse behaves the same way), while its bytes
+equal the oracle's.
 
 The CPU baseline times the reference's own OpenMP CPU path (compiled from
 the reference sources into oracle/_ref/libadrt_ref.so) on a bounded sample
-of the same phantoms, on this host's cores; if that library is absent the
-oracle restatement is timed instead.
+(at least 4) of the same phantoms, on this host's cores; if that library is
+absent the oracle restatement is timed instead.  The CPU model is reported.
 
 Measured HBM traffic ("roofline.traffic") comes from the committed rocprofv3
 summary profiles/fwd_adrt_16x2048_f32.md, made by two counter runs of
@@ -49,10 +69,14 @@ def parse():
     p.add_argument("--dtype", default="float32", choices=["float32", "float64"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
-    p.add_argument("--extras", action="store_true", help="also time bdrt / iadrt / fp64")
+    p.add_argument("--no-legs", action="store_true",
+                   help="time only the headline forward (skip bdrt / iadrt / the other "
+                        "dtype / uneven batch / least squares)")
     p.add_argument("--no-learned", action="store_true",
                    help="skip the learned-vs-plain CG time-to-1e-4 leg")
     p.add_argument("--cg-count", type=int, default=1000)
+    p.add_argument("--cg-repeats", type=int, default=5,
+                   help="interleaved rounds of the plain / learned / preconditioner-only solves")
     p.add_argument("--profile", action="store_true",
                    help="short run for rocprofv3 (1 warm-up + 3 steps, no CPU leg), followed "
                         "by a 256 MiB device copy that calibrates the memory counters")
@@ -125,21 +149,28 @@ def lsq_leg(iters=200):
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     per_iter = bytes_fwd(512, 4) * 32 * 2  # A and A^T, compulsory bytes each
+    gbs = per_iter / (secs / iters) / 1e9
+    # the same iteration in float64 (untimed): the error of the float32 result
+    # against an FP64 run
+    x64, _ = autograd.lsq_cg(b.double(), iters=iters)
     return {
         "workload": f"CGLS through autograd, 32 x 512x512 float32, {iters} steps, x0 = 0",
         "seconds": round(secs, 4),
         "ms_per_step": round(secs / iters * 1e3, 3),
         "loss_first": float(hist[0].mean()), "loss_last": float(hist[-1].mean()),
         "rel_error_vs_phantom": float(((x - x_true).norm() / x_true.norm())),
-        "GBps_algorithmic": round(per_iter / (secs / iters) / 1e9, 1),
+        "GBps_algorithmic": round(gbs, 1),
+        "frac": round(gbs / PEAK_GBS, 4),
+        "fp64_error": float((x.double() - x64).abs().max() / x64.abs().max()),
     }
 
 
-def learned_cg_leg(count):
+def learned_cg_leg(count, repeats):
     """Second half of the metric: wall time to relative residual 1e-4 of CG
     on the normal equations for ``count`` held-out 256x256 float32 phantoms
     (noise 0.5% of max|adrt(x)|), learned (U-Net initial guess + learned
-    multiscale preconditioner) vs plain CG, on the same GPU in the same run.
+    multiscale preconditioner) vs plain CG, on the same GPU in the same run,
+    ``repeats`` interleaved rounds of (plain, learned, preconditioner only).
     Uses the committed weights trained on the GPU by
     ``python -m adrt_amd.train_learned``; trains for 60 s if they are absent."""
     from adrt_amd import learned
@@ -152,7 +183,7 @@ def learned_cg_leg(count):
         solver.fit_preconditioner()
         solver.train(2000, batch=64)
         source = "trained in this run"
-    out = learned.evaluate(solver, count=count)
+    out = learned.evaluate(solver, count=count, repeats=repeats)
     out["weights"] = source
     out["workload"] = (f"{count} held-out 256x256 float32 ellipse phantoms (seed 2000), "
                        "d = adrt(x) + 0.5% noise, CG on mean_q truncate(bdrt(adrt(.)))")
@@ -187,6 +218,20 @@ def uneven_extra(steps, warmup):
     gbs = algo / (graph_ms * 1e-3) / 1e9
     choice = {f"{t['op']}{t['n']}": [t["choice"]["variant"], t["choice"]["group_bytes"] >> 20]
               for t in plan.tuning}
+    # checks on the first image of every side: the replayed outputs against
+    # the oracle (bytes; bdrt up to the sign of zero) and against an FP64
+    # oracle run on the same image
+    from oracle import oracle as O
+
+    exact, err = True, 0.0
+    for n in plan.classes:
+        i = plan.slot.index((n, 0))
+        x = plan.images[n][:1].cpu().numpy()
+        y, z = plan.forward(i).cpu().numpy()[None], plan.backprojection(i).cpu().numpy()[None]
+        y_ref = O.adrt(x)
+        exact = exact and same_bytes(y, y_ref) and same_bytes_up_to_zero_sign(z, O.bdrt(y_ref))
+        y64 = O.adrt(x.astype(np.float64))
+        err = max(err, fp64_error(y, y64), fp64_error(z, O.bdrt(y.astype(np.float64))))
     return {
         "workload": "forward ADRT + bdrt of 1000 float32 images, n in {32..1024} with "
                     "p ~ 1/n (seed 0), one hipGraph",
@@ -196,10 +241,24 @@ def uneven_extra(steps, warmup):
         "imgs_per_s": round(1000 / graph_ms * 1e3, 1),
         "algorithmic_bytes": algo,
         "GBps_algorithmic": round(gbs, 1),
-        "frac_of_peak": round(gbs / PEAK_GBS, 4),
+        "frac": round(gbs / PEAK_GBS, 4),
+        "bit_exact_first_image_per_side": exact,
+        "fp64_error": err,
         "autotune_s": round(tune_s, 2),
         "tuned_variant_groupMiB": choice,
     }
+
+
+def cpu_model():
+    """The host CPU's model name (first "model name" line of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(host_imgs, budget_s):
@@ -218,7 +277,8 @@ def cpu_baseline(host_imgs, budget_s):
     fn(one)
     est = time.perf_counter() - t0
     runs = 5
-    reps = max(1, min(len(host_imgs), int(budget_s / max(est, 1e-3) / runs)))
+    reps = max(min(4, len(host_imgs)),
+               min(len(host_imgs), int(budget_s / max(est, 1e-3) / runs)))
     sample = np.ascontiguousarray(host_imgs[:reps])
     times = []
     for _ in range(runs):
@@ -232,8 +292,10 @@ def cpu_baseline(host_imgs, budget_s):
         "unit": "imgs/s",
         "cores": cores,
         "kind": kind,
+        "cpu_model": cpu_model(),
         "sample": f"adrt of {reps} of the same {n}x{n} {host_imgs.dtype} phantoms, "
                   f"median of {runs} runs after a warm-up, OpenMP threads={cores}",
+        "seconds_per_run_median": round(med, 4),
     }
     if kind == "reference":
         # the same call on one thread (the reference's OpenMP runtime is
@@ -357,6 +419,93 @@ def profiled_traffic():
     return None
 
 
+def same_bytes(got, want):
+    """Raw byte equality (signs of zero included), NaNs matched by position."""
+    import numpy as np
+
+    got, want = np.asarray(got), np.asarray(want)
+    if got.shape != want.shape or got.dtype != want.dtype:
+        return False
+    nan = np.isnan(got) & np.isnan(want)
+    return bool(np.where(nan, 0, got).tobytes() == np.where(nan, 0, want).tobytes())
+
+
+def same_bytes_up_to_zero_sign(got, want):
+    import numpy as np
+
+    got = np.where(np.asarray(got) == 0, 0, got).astype(np.asarray(got).dtype)
+    want = np.where(np.asarray(want) == 0, 0, want).astype(np.asarray(want).dtype)
+    return same_bytes(got, want)
+
+
+def fp64_error(got, ref64):
+    """max |got - ref64| / max |ref64| over the compared entries: the error of
+    a result against an FP64 run of the oracle on the same input."""
+    import numpy as np
+
+    ref64 = np.asarray(ref64, dtype=np.float64)
+    scale = float(np.abs(ref64).max())
+    err = float(np.abs(np.asarray(got, dtype=np.float64) - ref64).max())
+    return err / scale if scale > 0 else err
+
+
+def transform_leg(lib, name, x, steps, stream):
+    """Times adrt_hip_<name> (adrt, bdrt or iadrt) on the device batch x
+    through the C ABI (HIP events on the kernels' stream, one warm-up call),
+    then checks image 0: bytes against the oracle in x's dtype (bdrt: up to
+    the sign of zero, see the fused back-projection's boundary rule) and the
+    error against an FP64 oracle run on the same input.  Returns (output,
+    record)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from adrt_amd import _lib
+    from oracle import oracle as O
+
+    op = {"adrt": _lib.OP_ADRT, "bdrt": _lib.OP_BDRT, "iadrt": _lib.OP_IADRT}[name]
+    code = _lib.F32 if x.dtype == torch.float32 else _lib.F64
+    batch, n, s = int(x.shape[0]), int(x.shape[-1]), x.element_size()
+    shape = (batch, 4, 2 * n - 1, n) if name == "adrt" else tuple(x.shape)
+    out = torch.empty(shape, device=x.device, dtype=x.dtype)
+    wb = lib.adrt_hip_workspace_bytes(op, code, batch, n)
+    ws = torch.empty(max(wb, 1), device=x.device, dtype=torch.uint8)
+    fn_c = getattr(lib, f"adrt_hip_{name}")
+    call = (ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+            ctypes.c_void_p(ws.data_ptr()), wb, code, batch, n,
+            ctypes.c_void_p(stream.cuda_stream))
+
+    def run():
+        st = fn_c(*call)
+        if st != 0:
+            _lib.check(st, f"adrt_hip_{name}")
+
+    _, ms = time_calls(run, steps, 1, stream)
+    del ws
+    algo = batch * (bytes_fwd(n, s) if name == "adrt" else bytes_bdrt(n, s))
+    gbs = algo / (ms * 1e-3) / 1e9
+    xin = x[:1].cpu().numpy()
+    got = out[:1].cpu().numpy()
+    ref = getattr(O, name)
+    same = ref(xin)
+    rec = {
+        "workload": f"{name} of {batch} x {n}x{n} {x.dtype}".replace("torch.", ""),
+        "imgs_per_s": round(batch / ms * 1e3, 1),
+        "ms_per_call": round(ms, 4),
+        "GBps_algorithmic": round(gbs, 1),
+        "frac": round(gbs / PEAK_GBS, 4),
+        "algorithmic_bytes_per_call": algo,
+        "bit_exact_first_image": (same_bytes_up_to_zero_sign(got, same) if name == "bdrt"
+                                  else same_bytes(got, same)),
+        "fp64_error": fp64_error(got, same if x.dtype == torch.float64
+                                 else ref(xin.astype(np.float64))),
+    }
+    if name == "bdrt":
+        rec["bit_exact_first_image_signed_zeros"] = same_bytes(got, same)
+    return out, rec
+
+
 def main():
     args = parse()
     if args.summarize:
@@ -366,7 +515,7 @@ def main():
             print(summarize(args.summarize, args.calls, args.match))
         return
     if args.profile:
-        args.steps, args.warmup, args.no_cpu, args.extras = 3, 1, True, False
+        args.steps, args.warmup, args.no_cpu, args.no_legs = 3, 1, True, True
         args.no_learned = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: "
@@ -476,14 +625,9 @@ def main():
         return
 
     wall, dev_ms = time_calls(fwd, args.steps, args.warmup, stream)
-    first_out = out[:1].cpu().numpy()
     value = batch * args.steps / wall
     algo = batch * bytes_fwd(n, s)
     achieved = algo / (dev_ms * 1e-3) / 1e9
-
-    # parity spot check of the timed output against the reference hash-free
-    # path: first image vs the compiled reference (or oracle) on the host.
-    host = imgs[:2].cpu().numpy()
     result = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -521,39 +665,45 @@ def main():
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
-    if args.extras:
-        extras = {}
-        y = out.clone()
-        bout = torch.empty_like(y)
-        for name, op in (("bdrt", _lib.OP_BDRT), ("iadrt", _lib.OP_IADRT)):
-            wb = lib.adrt_hip_workspace_bytes(op, code, batch, n)
-            wst = torch.empty(max(wb, 1), device=device, dtype=torch.uint8)
-            fn_c = getattr(lib, f"adrt_hip_{name}")
-            args_c = (ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(bout.data_ptr()),
-                      ctypes.c_void_p(wst.data_ptr()), wb, code, batch, n, sp)
-            _, ms = time_calls(lambda: fn_c(*args_c), max(3, args.steps // 4), 1, stream)
-            gbs = batch * bytes_bdrt(n, s) / (ms * 1e-3) / 1e9
-            extras[name] = {"ms_per_call": round(ms, 3), "imgs_per_s": round(batch / ms * 1e3, 2),
-                            "GBps_algorithmic": round(gbs, 1)}
-        extras["uneven"] = uneven_extra(max(3, args.steps // 4), 2)
-        extras["lsq_autograd"] = lsq_leg()
-        result["extras"] = extras
-    if not args.no_learned:
-        del out, ws
+        print(json.dumps(result), flush=True)
+        return
+    from oracle import oracle as O
+
+    # the headline's own checks on image 0: bytes against the oracle in the
+    # same dtype, and the error against an FP64 run of the oracle
+    x0 = imgs[:1].cpu().numpy()
+    y0 = out[:1].cpu().numpy()
+    result["parity_first_image_bit_exact"] = same_bytes(y0, O.adrt(x0))
+    result["fp64_error"] = fp64_error(y0, O.adrt(x0.astype(np.float64)))
+    host = imgs.cpu().numpy()
+    legs = {}
+    if not args.no_legs:
+        steps = max(3, args.steps // 4)
+        # the round trip's other legs on the forward's output, then the whole
+        # round trip in float64 on the same phantoms
+        y = out
+        del ws
+        for name in ("bdrt", "iadrt"):
+            _, legs[f"{name}_{args.dtype}"] = transform_leg(lib, name, y, steps, stream)
+        del y, out
         torch.cuda.empty_cache()
-        result["cg_time_to_1e-4"] = learned_cg_leg(args.cg_count)
+        other = "float64" if args.dtype == "float32" else "float32"
+        x64 = phantoms(batch, n, seed=0, dtype=other, device=device)
+        y64, legs[f"adrt_{other}"] = transform_leg(lib, "adrt", x64, steps, stream)
+        del x64
+        for name in ("bdrt", "iadrt"):
+            _, legs[f"{name}_{other}"] = transform_leg(lib, name, y64, steps, stream)
+        del y64
+        torch.cuda.empty_cache()
+        legs["uneven_c3"] = uneven_extra(steps, 2)
+        torch.cuda.empty_cache()
+        legs["lsq_autograd_c4"] = lsq_leg()
+        torch.cuda.empty_cache()
+    result["legs"] = legs
+    if not args.no_learned:
+        result["cg_time_to_1e-4"] = learned_cg_leg(args.cg_count, args.cg_repeats)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(host, args.cpu_seconds)
-        # bit-exact spot check of the first image against the CPU leg's library
-        from oracle import ref as R
-        from oracle import oracle as O
-
-        ref_fn = R.adrt if R.available() else O.adrt
-        got = first_out
-        want = ref_fn(np.ascontiguousarray(host[:1]))
-        got = np.where(got == 0, 0, got)
-        want = np.where(want == 0, 0, want)
-        result["parity_first_image_bit_exact"] = bool(got.tobytes() == want.tobytes())
     print(json.dumps(result), flush=True)
 
 

@@ -183,7 +183,7 @@ static int run(int B, int n, const char *fin, const char *fout) {
     const size_t per = x_bytes_per_image(n, L, sizeof(T));
     const size_t G = group_images(size_t(B), per, g_budget);
     std::vector<T> X(G * per / sizeof(T), T(NAN));
-    const int k1 = fwd_pass1_stages(L, g_variant);
+    const int k1 = fwd_pass1_stages(L, g_variant, int(sizeof(T)));
     for (int first = 0; first < B; first += int(G)) {
         const int count = (B - first) < int(G) ? (B - first) : int(G);
         const int code = (k1 - 3) * 4 + (L - k1 - 3);

@@ -27,8 +27,54 @@ def test_stage_chain_is_bdrt_adjoint_on_cpu():
     assert abs(np.sum(O.bdrt(g) * h) - np.sum(g * chain)) < 1e-9 * np.sum(np.abs(g * chain))
 
 
+@pytest.mark.parametrize("shape", [(2, 32, 64), (64, 32), (2, 2, 32, 32), (5,), (2, 24, 24),
+                                   (0, 16, 16)])
+def test_adrt_rejects_bad_shapes_before_any_device_work(shape):
+    """Non-square, non-power-of-two, 1-D, 4-D and empty images are refused on
+    the host (a 4-D input used to be read as one image, a non-square one as
+    a larger buffer)."""
+    import torch
+    from adrt_amd import autograd as AG
+
+    with pytest.raises(ValueError):
+        AG.adrt(torch.zeros(shape))
+
+
+@pytest.mark.parametrize("shape", [(4, 30, 16), (3, 31, 16), (2, 4, 31, 15), (2, 2, 4, 31, 16),
+                                   (4, 23, 12), (31, 16)])
+def test_bdrt_and_lsq_reject_bad_shapes(shape):
+    import torch
+    from adrt_amd import autograd as AG
+
+    with pytest.raises(ValueError):
+        AG.bdrt(torch.zeros(shape))
+    with pytest.raises(ValueError):
+        AG.lsq_cg(torch.zeros(shape), iters=1)
+
+
+def test_rejects_host_tensors_and_other_dtypes():
+    import torch
+    from adrt_amd import autograd as AG
+
+    with pytest.raises(TypeError, match="host tensor"):
+        AG.adrt(torch.zeros((2, 16, 16)))
+    with pytest.raises(TypeError, match="host tensor"):
+        AG.bdrt(torch.zeros((4, 31, 16), dtype=torch.float64))
+    with pytest.raises(TypeError, match="torch tensor"):
+        AG.adrt(np.zeros((16, 16), dtype=np.float32))
+
+
 @gpu
 class TestAutograd:
+    def test_device_dtype_rejected(self, cuda):
+        import torch
+        from adrt_amd import autograd as AG
+
+        with pytest.raises(TypeError, match="unsupported array dtype"):
+            AG.adrt(torch.zeros((16, 16), dtype=torch.float16, device=cuda))
+        with pytest.raises(ValueError, match="square"):
+            AG.adrt(torch.zeros((2, 16, 32), device=cuda))
+
     def test_adrt_backward_is_reference_adjoint(self, cuda):
         import torch
         from adrt_amd import autograd as AG

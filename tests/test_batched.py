@@ -95,17 +95,25 @@ class TestConfig:
                                8, 256, torch.cuda.current_stream().cuda_stream)
         assert st == _lib.ERR_WORKSPACE
 
-    @pytest.mark.parametrize("n", [1, 2, 16, 32, 64, 128, 512, 2048])
-    def test_every_variant_matches_oracle(self, cuda, reset_config, n):
+    @pytest.mark.parametrize("n,dt", [(n, "float32") for n in (1, 2, 16, 32, 64, 128, 512, 2048)]
+                             + [(n, "float64") for n in (128, 512, 2048)])
+    def test_every_variant_matches_oracle(self, cuda, reset_config, n, dt):
+        """Every configuration the tuner can select (both fused splits, the
+        streaming path, the LDS kernel, group budgets) gives the oracle's
+        bytes for the forward and the back-projection; in float64 the
+        default forward split differs from float32's and pass 1 at 2048
+        with ceil(L/2) stages needs 160 KB of LDS."""
         import torch
         from adrt_amd import _kernels
 
+        print(f"every variant n={n} {dt}", flush=True)
         b = 3 if n < 2048 else 1
-        x = np.random.default_rng(n).standard_normal((b, n, n)).astype(np.float32)
+        x = np.random.default_rng(n).standard_normal((b, n, n)).astype(dt)
         want = O.adrt(x)
         back_want = O.bdrt(want)
         xt = torch.from_numpy(x).to(cuda)
-        per = max(int(_lib.load().adrt_hip_workspace_bytes(_lib.OP_ADRT, _lib.F32, 1, n)), 1)
+        code = _lib.F32 if dt == "float32" else _lib.F64
+        per = max(int(_lib.load().adrt_hip_workspace_bytes(_lib.OP_ADRT, code, 1, n)), 1)
         for variant, budget in batched.candidates(_lib.OP_ADRT, n, b, per):
             batched.set_config(_lib.OP_ADRT, n, variant, budget)
             batched.set_config(_lib.OP_BDRT, n, variant, budget)

@@ -36,6 +36,44 @@ def test_coord_adrt_shapes_and_angles(n):
                                            [0, np.pi / 4], [np.pi / 2, np.pi / 4]])
 
 
+def test_coord_adrt_exact_at_n2():
+    """Every entry at n = 2, where the definition fixes each value exactly:
+    heights 3/4, 1/4, -1/4 (midpoints of three steps of 1/2 down from 1),
+    slopes 0 and 1, so offsets h - 1/2 at slope 0 and ((h + 3/4)/2 - 1/2) *
+    (cos + sin)(pi/4) at slope 1; angles 0 and atan(1) with the quadrant
+    shifts (quadrant 1's first angle is -0.0)."""
+    offset, angle = U.coord_adrt(2)
+    c = np.cos(np.arctan(1.0)) + np.sin(np.arctan(1.0))
+    q0 = np.array([[0.25, 0.25 * c], [-0.25, 0.0], [-0.75, -0.25 * c]])
+    want = np.stack([q0, -q0, q0, -q0])
+    assert offset.tobytes() == want.tobytes()
+    t = np.arctan(1.0)
+    want_angle = np.array([[-np.pi / 2, t - np.pi / 2], [-0.0, -t], [0.0, t],
+                           [np.pi / 2, -(t - np.pi / 2)]])[:, None, :]
+    assert angle.tobytes() == want_angle.tobytes()
+
+
+@pytest.mark.parametrize("n", [2 ** k for k in range(1, 13)])
+def test_coord_adrt_exact_first_and_last_angle(n):
+    """At slope 0 and slope 1 every operation of the definition is exact in
+    binary for a power-of-two n except the final product with cos + sin:
+    offset(row k, slope 0) = 1/2 - (k + 1/2)/n exactly, and
+    offset(row k, slope 1) = (1/2 - (k + 1)/(2n)) * (cos + sin)(pi/4),
+    one rounding; the angles are exactly -pi/2, 0, 0, pi/2 at slope 0 and
+    +-atan(1), +-(atan(1) - pi/2) at slope 1."""
+    offset, angle = U.coord_adrt(n)
+    k = np.arange(2 * n - 1, dtype=np.float64)
+    first = 0.5 - (k + 0.5) / n
+    c = np.cos(np.arctan(1.0)) + np.sin(np.arctan(1.0))
+    last = (0.5 - (k + 1) / (2 * n)) * c
+    for q, sgn in enumerate((1, -1, 1, -1)):
+        assert np.array_equal(offset[q, :, 0], sgn * first)
+        assert np.array_equal(offset[q, :, -1], sgn * last)
+    t = np.arctan(1.0)
+    assert np.array_equal(angle[:, 0, 0], [-np.pi / 2, 0.0, 0.0, np.pi / 2])
+    assert np.array_equal(angle[:, 0, -1], [t - np.pi / 2, -t, t, -(t - np.pi / 2)])
+
+
 def test_coord_adrt_angles_stitch():
     angle = U.coord_adrt(16).angle
     stitched = np.concatenate([angle[0, 0], angle[1, 0, :-1][::-1], angle[2, 0, 1:],

@@ -98,18 +98,23 @@ def test_fused_emulation_capped_grid(emulator, tmp_path, slots):
     assert np.array_equal(got.view(np.uint8), O.adrt(x).view(np.uint8))
 
 
+@pytest.mark.parametrize("dt", ["float32", "float64"])
 @pytest.mark.parametrize("n", [128, 512, 2048])
-def test_fused_alt_split_emulation(emulator, tmp_path, n):
-    """The other stage split (k1 = floor(L/2) forward, kA = ceil(L/2)
-    back-projection) that the autotuner may pick for odd log2 n."""
+def test_fused_alt_split_emulation(emulator, tmp_path, n, dt):
+    """The other stage split (ALT_SPLIT) that the autotuner may pick for odd
+    log2 n: forward pass 1 takes ceil(L/2) stages where the default takes
+    floor(L/2) (float64, or L <= 9) and floor(L/2) where the default takes
+    ceil(L/2) (float32 with L > 9); back-projection pass A takes floor(L/2)
+    for L <= 9 and ceil(L/2) otherwise.  In float64 at 2048 this is the
+    160 KB-LDS pass 1."""
     rng = np.random.default_rng(n + 5)
-    x = (rng.standard_normal((1, n, n)) * 10).astype(np.float32)
+    x = (rng.standard_normal((1, n, n)) * 10).astype(dt)
     x[0, 0, :3] = [-0.0, np.inf, np.nan]
     got = run_emu(emulator, x, tmp_path, variant=1)
     want = O.adrt(x)
     same = np.isnan(got) & np.isnan(want)
     assert np.array_equal(np.where(same, 0, got).view(np.uint8), np.where(same, 0, want).view(np.uint8))
-    y = (rng.standard_normal((1, 4, 2 * n - 1, n)) * 10).astype(np.float32)
+    y = (rng.standard_normal((1, 4, 2 * n - 1, n)) * 10).astype(dt)
     got = run_emu(emulator, y, tmp_path, op="bdrt", variant=1)
     assert np.array_equal(canonical(got).view(np.uint8), canonical(O.bdrt(y)).view(np.uint8))
 
