@@ -27,6 +27,8 @@ ERR_NULL = 5
 ERR_WORKSPACE = 6
 ERR_DEVICE = 7
 ERR_ARG = 8
+ERR_SCRATCH = 9
+NOALLOC_SCRATCH_BYTES = 68 << 20
 
 VARIANT_DEFAULT = 0
 VARIANT_ALT_SPLIT = 1
@@ -61,6 +63,7 @@ SIGNATURES = {
     "adrt_hip_valid_highpass_shape": (_INT, [_SHAPE3]),
     "adrt_hip_workspace_bytes": (_SZ, [_INT, _INT, _SZ, _SZ]),
     "adrt_hip_adrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_adrt_noalloc": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_adrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
     "adrt_hip_bdrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_bdrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
@@ -150,7 +153,7 @@ def check(status, what):
     if status == ERR_DEVICE:
         detail = lib.adrt_hip_last_error().decode()
         raise RuntimeError(f"{what}: {msg} ({detail})")
-    if status in (ERR_SHAPE, ERR_STEP, ERR_TOO_BIG, ERR_ARG):
+    if status in (ERR_SHAPE, ERR_STEP, ERR_TOO_BIG, ERR_ARG, ERR_SCRATCH):
         raise ValueError(f"{what}: {msg}")
     if status == ERR_DTYPE:
         raise TypeError(f"{what}: {msg}")

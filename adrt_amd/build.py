@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libadrt_hip.so")
-SOURCES = ["capi.hip", "fwd.hip", "fused.hip", "small.hip", "bdrt.hip", "iadrt.hip", "fmg.hip", "layout.hip", "nn.hip", "cg.hip", "interp.hip"]
+SOURCES = ["capi.hip", "fwd.hip", "fused.hip", "small.hip", "bdrt.hip", "iadrt.hip", "fmg.hip",
+           "layout.hip", "nn.hip", "cg.hip", "interp.hip", "noalloc.hip"]
 
 
 def hipcc():

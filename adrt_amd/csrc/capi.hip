@@ -55,6 +55,7 @@ const char *adrt_hip_status_string(int status) {
         case ADRT_HIP_ERR_WORKSPACE: return "workspace too small";
         case ADRT_HIP_ERR_DEVICE: return "HIP device error";
         case ADRT_HIP_ERR_ARG: return "invalid argument";
+        case ADRT_HIP_ERR_SCRATCH: return "needs more than the no-allocation scratch buffer";
         default: return "unknown status";
     }
 }

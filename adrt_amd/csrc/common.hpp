@@ -86,6 +86,11 @@ bool use_small(int op, size_t B, size_t n);
 int launch_small(const void *src, void *out, int op, int dtype, size_t B, size_t n,
                  hipStream_t s);
 int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, hipStream_t s);
+// The fused forward with launch groups of at most max(1, group_bytes /
+// per-image intermediate) images, and that per-image intermediate size.
+int run_fused_budget(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
+                     size_t group_bytes, hipStream_t s);
+size_t fused_workspace_per_image(int dtype, size_t n);
 size_t fused_bdrt_workspace(int dtype, size_t B, size_t n);
 int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
                    hipStream_t s);

@@ -1488,12 +1488,17 @@ size_t fused_workspace(int dtype, size_t B, size_t n) {
     return fused::group_images(B, per, group_budget(ADRT_HIP_OP_ADRT, n)) * per;
 }
 
+size_t fused_workspace_per_image(int dtype, size_t n) {
+    return fused::x_bytes_per_image(int(n), num_iters(n), dtype == ADRT_HIP_F64 ? 8 : 4);
+}
+
 template <typename T>
-static int run_fused_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
+static int run_fused_t(const T *in, T *out, T *ws, size_t B, size_t n, size_t budget,
+                       hipStream_t s) {
     const int L = num_iters(n);
     const int k1 = fused::fwd_pass1_stages(L, fused_variant(ADRT_HIP_OP_ADRT, n), int(sizeof(T)));
     const size_t per = fused::x_bytes_per_image(int(n), L, sizeof(T));
-    const size_t G = fused::group_images(B, per, group_budget(ADRT_HIP_OP_ADRT, n));
+    const size_t G = fused::group_images(B, per, budget);
     for (size_t first = 0; first < B; first += G) {
         const size_t count = (B - first) < G ? (B - first) : G;
         const hipError_t e =
@@ -1506,12 +1511,17 @@ static int run_fused_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream
     return launch_status();
 }
 
-int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, hipStream_t s) {
+int run_fused_budget(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
+                     size_t group_bytes, hipStream_t s) {
     if (dtype == ADRT_HIP_F32)
         return run_fused_t(static_cast<const float *>(in), static_cast<float *>(out),
-                           static_cast<float *>(ws), B, n, s);
+                           static_cast<float *>(ws), B, n, group_bytes, s);
     return run_fused_t(static_cast<const double *>(in), static_cast<double *>(out),
-                       static_cast<double *>(ws), B, n, s);
+                       static_cast<double *>(ws), B, n, group_bytes, s);
+}
+
+int run_fused(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, hipStream_t s) {
+    return run_fused_budget(in, out, ws, dtype, B, n, group_budget(ADRT_HIP_OP_ADRT, n), s);
 }
 
 size_t fused_bdrt_workspace(int dtype, size_t B, size_t n) {

@@ -42,6 +42,7 @@ extern "C" {
 #define ADRT_HIP_ERR_WORKSPACE 6    /* workspace smaller than required           */
 #define ADRT_HIP_ERR_DEVICE 7       /* HIP runtime / launch failure              */
 #define ADRT_HIP_ERR_ARG 8          /* other invalid argument                    */
+#define ADRT_HIP_ERR_SCRATCH 9      /* needs more than the no-allocation scratch */
 
 /* Operation ids for adrt_hip_workspace_bytes(). */
 #define ADRT_HIP_OP_ADRT 0
@@ -100,6 +101,24 @@ size_t adrt_hip_workspace_bytes(int op, int dtype, size_t batch, size_t n);
  * adrt_basic, adrt_cdefs_adrt.hpp:100-212).  Bit-exact to the reference. */
 int adrt_hip_adrt(const void *in, void *out, void *workspace, size_t workspace_bytes,
                   int dtype, size_t batch, size_t n, void *stream);
+
+/* Forward ADRT that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, validation and bit-identical results as adrt_hip_adrt, without
+ * the workspace: the fused kernels' intermediate lives in a library-owned
+ * static device array of ADRT_HIP_NOALLOC_SCRATCH_BYTES (68 MiB) on each
+ * device the library runs on, held until the process ends; launch groups are
+ * sized to it.  Sides whose intermediate for one image exceeds it (float64 at
+ * n = 2048; the streaming path when its output-sized buffer exceeds it) are
+ * refused with ADRT_HIP_ERR_SCRATCH before any device work.  No hipMalloc,
+ * hipFree or allocating library call is made.  All calls on one device share
+ * its array (nothing in it needs clearing between calls): calls that overlap
+ * on one device, from other streams or threads, race and must be serialised
+ * by the caller. */
+#define ADRT_HIP_NOALLOC_SCRATCH_BYTES (68u << 20)
+int adrt_hip_adrt_noalloc(const void *in, void *out, int dtype, size_t batch, size_t n,
+                          void *stream);
 
 /* One forward stage on the ADRT-output layout, 0 <= step < num_iters(n).
  * Replaces adrt._adrt_cdefs.adrt_step (adrt_cdefs_py.cpp:343-412 ->
