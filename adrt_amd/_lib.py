@@ -113,6 +113,9 @@ SIGNATURES = {
     "adrt_hip_pyr_stencil_up": (_INT, [_VP, _VP, _VP, _VP, _INT, _INT, _INT, _VP]),
     "adrt_hip_pyr_correlate": (_INT, [_VP, _VP, _VP, _INT, _INT, _INT, _INT, _VP]),
     "adrt_hip_phantom_raster": (_INT, [_VP, _VP, _INT, _VP, _INT, _INT, _VP]),
+    "adrt_hip_dct_table_len": (_SZ, [_SZ]),
+    "adrt_hip_dct_table": (_INT, [_SZ, _VP, _SZ]),
+    "adrt_hip_dct_filter": (_INT, [_VP, _VP, _VP, _VP, _VP, _SZ, _SZ, _INT, _VP]),
 }
 
 _lib = None

@@ -1,0 +1,311 @@
+// Radial DCT-domain preconditioner of the learned CG solver (no reference
+// counterpart): y = IDCT2(H * DCT2(x)) for float32 images (batch, n, n),
+// with the orthonormal DCT-II along both axes, H an (n, n) table of positive
+// weights and IDCT2 its transpose (the orthonormal DCT-III).  H radial and
+// positive makes the filter symmetric positive definite.
+//
+// A 1-D orthonormal DCT-II of length n (Makhoul 1980): reorder
+// v[k] = x[2k], v[n-1-k] = x[2k+1] (k < n/2); V = DFT_n(v);
+// X[m] = 2 Re(e^{-i pi m / 2n} V[m]) * s(m), s(0) = sqrt(1/4n),
+// s(m > 0) = sqrt(1/2n).  Its inverse: Y[m] = X[m] / s(m),
+// V[m] = (1/2) e^{i pi m / 2n} (Y[m] - i Y[n-m]) (Y[n] = 0), v = IDFT_n(V),
+// x[2k] = v[k], x[2k+1] = v[n-1-k].  The DFTs are in-LDS radix-2
+// decimation-in-frequency FFTs (natural order in, bit-reversed out), so no
+// permutation pass is needed: the post-processing reads V[bitrev(m)].
+//
+// A work-group holds R = 4096 / n whole rows (4096 complex values, 32 KB of
+// LDS) and the twiddles (n <= 1024).  The 2-D transform is three row passes over the batch:
+//   pass 0 (kForward):  rows of x -> DCT-II -> written transposed
+//   pass 1 (kFiltered): rows (now columns of x) -> DCT-II -> * H -> DCT-III
+//                       -> written transposed (back to x's orientation)
+//   pass 2 (kInverse):  rows -> DCT-III -> written in place
+// Transposed stores go through LDS with a padded pitch so that each column
+// of R rows leaves as one contiguous run.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace adrt_hip {
+namespace dctp {
+
+constexpr int kThreads = 256;
+constexpr int kElems = 4096;                  // complex values per work-group
+constexpr int kPer = kElems / kThreads;       // elements per thread
+constexpr int kMaxN = 1024;                   // longest row (the twiddles live in LDS)
+
+enum Mode : int { kForward = 0, kFiltered = 1, kInverse = 2 };
+
+struct Args {
+    const float *in;
+    float *out;
+    const float2 *tw;  // [n/2] e^{-2 pi i t / n}, then [n] e^{-i pi m / 2n}
+    const float *H;    // (n, n), kFiltered only
+    long long rows;    // batch * n
+    int n, L;
+    float s0, s1;      // s(0), s(m > 0)
+};
+
+__device__ __forceinline__ int bitrev(int m, int L) {
+    return int(__builtin_bitreverse32(unsigned(m)) >> (32 - L));
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// LDS position of complex element e of the work-group's rows: one padding
+// element after every 16, so the strided accesses of the FFT steps (stride
+// 16 elements in the last radix-16 step) fall on distinct banks.
+__device__ __forceinline__ int pad(int e) { return e + (e >> 4); }
+constexpr int kPadded = kElems + kElems / 16;
+
+__device__ __forceinline__ float2 twid(const float2 *tw, int t, bool inverse) {
+    float2 w = tw[t];
+    if (inverse) w.y = -w.y;
+    return w;
+}
+
+// In-place DIF FFT of the R rows of `buf` (row r holds elements r * n ...,
+// at LDS positions pad(.)); forward (e^{-2 pi i ...}) or inverse (conjugate
+// twiddles, unscaled); `tw` (in LDS) holds W^t = e^{-2 pi i t / n}, t < n/2.
+// Radix-2 stage s pairs elements half = n >> (s + 1) apart, with twiddle
+// W^(pos << s) on the difference (pos = position within the half).  Step<K>
+// does stages s .. s+K-1 in registers on the 2^K elements
+// {base + t q : t < 2^K} (q = n >> (s + K)), with exactly the operations of
+// those K radix-2 stages: stage s+u pairs t with t + h, h = 2^(K-1-u), at
+// position (t mod h) q + p.  Steps of 4 stages (radix 16) go first; one
+// shorter step finishes.
+template <int K>
+__device__ __forceinline__ void fft_step(float2 *buf, const float2 *tw, int L, int s,
+                                         bool inverse) {
+    constexpr int E = 1 << K;
+    const int lq = L - s - K;  // log2 q
+    const int q = 1 << lq;
+    const int gpr = 1 << (L - K);  // groups per row
+#pragma unroll
+    for (int j = threadIdx.x; j < kElems / E; j += kThreads) {
+        const int r = j >> (L - K), jj = j & (gpr - 1);
+        const int g = jj >> lq, p = jj & (q - 1);
+        const int base = (r << L) + (g << (lq + K)) + p;
+        float2 a[E];
+#pragma unroll
+        for (int t = 0; t < E; ++t) a[t] = buf[pad(base + t * q)];
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int h = E >> (u + 1);
+#pragma unroll
+            for (int t = 0; t < E; ++t) {
+                if (t & h) continue;
+                const float2 w = twid(tw, ((t & (h - 1)) * q + p) << (s + u), inverse);
+                const float2 x = a[t], y = a[t + h];
+                a[t] = make_float2(x.x + y.x, x.y + y.y);
+                a[t + h] = cmul(make_float2(x.x - y.x, x.y - y.y), w);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < E; ++t) buf[pad(base + t * q)] = a[t];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void fft_dif(float2 *buf, const float2 *tw, int L, bool inverse) {
+    int s = 0;
+    for (; s + 4 <= L; s += 4) fft_step<4>(buf, tw, L, s, inverse);
+    switch (L - s) {
+        case 3: fft_step<3>(buf, tw, L, s, inverse); break;
+        case 2: fft_step<2>(buf, tw, L, s, inverse); break;
+        case 1: fft_step<1>(buf, tw, L, s, inverse); break;
+        default: break;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
+    __shared__ __align__(16) float2 buf[kPadded];
+    extern __shared__ __align__(16) float2 tws[];  // n/2 + n twiddles (dynamic LDS)
+    float *stg = reinterpret_cast<float *>(buf);  // real staging view (2 * kPadded floats)
+    const int n = a.n, L = a.L, R = kElems >> L;
+    const long long g0 = (long long)blockIdx.x * R;  // first row of this work-group
+    const int tid = threadIdx.x;
+    for (int t = tid; t < (n >> 1) + n; t += kThreads) tws[t] = a.tw[t];
+    const float2 *tw = tws;
+    const float2 *pt = tws + (n >> 1);
+    float val[kPer];
+
+    if (MODE != kInverse) {
+        // load rows, Makhoul reorder into the complex buffer
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const int r = e >> L, k = e & (n - 1);
+            const long long gr = g0 + r;
+            const float x = gr < a.rows ? a.in[gr * n + k] : 0.f;
+            const int vp = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
+            buf[pad((r << L) + vp)] = make_float2(x, 0.f);
+        }
+        __syncthreads();
+        fft_dif(buf, tw, L, false);
+        // X[m] = 2 Re(pt[m] V[m]) s(m), element e = (row r, frequency m)
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const int r = e >> L, m = e & (n - 1);
+            const float2 V = buf[pad((r << L) + bitrev(m, L))];
+            const float2 p = pt[m];
+            val[q] = 2.f * (p.x * V.x - p.y * V.y) * (m == 0 ? a.s0 : a.s1);
+        }
+        if (MODE == kFiltered) {
+            // * H[j][m] with j = the row's index in its image (the frequency
+            // of the other axis, after the transposed first pass)
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const int e = tid + q * kThreads;
+                const int r = e >> L, m = e & (n - 1);
+                const int j = int((g0 + r) & (n - 1));
+                val[q] *= a.H[(size_t)j * n + m];
+            }
+        }
+        __syncthreads();  // every read of buf is done
+    } else {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const long long gr = g0 + (e >> L);
+            val[q] = gr < a.rows ? a.in[gr * n + (e & (n - 1))] : 0.f;
+        }
+    }
+
+    if (MODE == kInverse) __syncthreads();  // the twiddles are in LDS
+    if (MODE != kForward) {
+        // inverse: Y = X / s in natural order, then V[m] = (1/2) conj(pt[m]) (Y[m] - i Y[n-m])
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const int m = e & (n - 1);
+            stg[e] = val[q] / (m == 0 ? a.s0 : a.s1);
+        }
+        __syncthreads();
+        float2 V[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const int r = e >> L, m = e & (n - 1);
+            const float ym = stg[e];
+            const float ynm = m == 0 ? 0.f : stg[(r << L) + n - m];
+            const float2 c = pt[m];  // e^{-i pi m / 2n}; its conjugate is applied
+            V[q] = make_float2(0.5f * (c.x * ym - c.y * ynm), 0.5f * (-c.y * ym - c.x * ynm));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) buf[pad(tid + q * kThreads)] = V[q];
+        __syncthreads();
+        fft_dif(buf, tw, L, true);
+        const float inv_n = 1.f / float(n);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const int r = e >> L, k = e & (n - 1);
+            const int j = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
+            val[q] = buf[pad((r << L) + bitrev(j, L))].x * inv_n;
+        }
+    }
+
+    if (MODE == kInverse) {
+        // rows back in place, coalesced
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + q * kThreads;
+            const long long gr = g0 + (e >> L);
+            if (gr < a.rows) a.out[gr * n + (e & (n - 1))] = val[q];
+        }
+        return;
+    }
+    // transposed store: element (r, m) -> out[b][m][i], i = row index in its
+    // image; staged as stg[m * (R + 1) + r] so each m leaves as a run of rows
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + q * kThreads;
+        const int r = e >> L, m = e & (n - 1);
+        stg[m * (R + 1) + r] = val[q];
+    }
+    __syncthreads();
+    const int lr = 12 - L;  // log2 R
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + q * kThreads;
+        const int m = e >> lr, r = e & (R - 1);
+        const long long gr = g0 + r;
+        if (gr < a.rows) {
+            const long long b = gr >> L, i = gr & (n - 1);
+            a.out[(b << (2 * L)) + ((long long)m << L) + i] = stg[m * (R + 1) + r];
+        }
+    }
+}
+
+template <int MODE>
+static void launch(const Args &a, hipStream_t s) {
+    const long long R = kElems >> a.L;
+    const unsigned grid = unsigned((a.rows + R - 1) / R);
+    const size_t lds = size_t((a.n >> 1) + a.n) * sizeof(float2);
+    hipLaunchKernelGGL(dct_rows_kernel<MODE>, dim3(grid), dim3(kThreads), lds, s, a);
+}
+
+}  // namespace dctp
+}  // namespace adrt_hip
+
+using namespace adrt_hip;
+
+extern "C" size_t adrt_hip_dct_table_len(size_t n) { return n / 2 + n; }
+
+extern "C" int adrt_hip_dct_table(size_t n, float *host, size_t len) {
+    if (n < 2 || n > size_t(dctp::kMaxN) || !is_pow2(n)) return ADRT_HIP_ERR_SHAPE;
+    if (!host) return ADRT_HIP_ERR_NULL;
+    if (len < 2 * adrt_hip_dct_table_len(n)) return ADRT_HIP_ERR_ARG;
+    const double pi = 3.14159265358979323846;
+    for (size_t t = 0; t < n / 2; ++t) {
+        const double ang = 2.0 * pi * double(t) / double(n);
+        host[2 * t] = float(std::cos(ang));
+        host[2 * t + 1] = float(-std::sin(ang));
+    }
+    float *pt = host + n;
+    for (size_t m = 0; m < n; ++m) {
+        const double ang = pi * double(m) / (2.0 * double(n));
+        pt[2 * m] = float(std::cos(ang));
+        pt[2 * m + 1] = float(-std::sin(ang));
+    }
+    return ADRT_HIP_OK;
+}
+
+extern "C" int adrt_hip_dct_filter(const float *x, float *y, const float *table, const float *H,
+                                   float *tmp, size_t batch, size_t n, int mode, void *stream) {
+    if (n < 2 || n > size_t(dctp::kMaxN) || !is_pow2(n) || batch == 0) return ADRT_HIP_ERR_SHAPE;
+    if (mode != 0 && mode != 1) return ADRT_HIP_ERR_ARG;
+    if (!x || !y || !table || !tmp || (mode == 1 && !H)) return ADRT_HIP_ERR_NULL;
+    if (x == y || x == tmp || y == tmp) return ADRT_HIP_ERR_ARG;
+    size_t elems;
+    if (!mul_ok(batch, n * n, &elems) || elems > (size_t(1) << 62)) return ADRT_HIP_ERR_TOO_BIG;
+    dctp::Args a;
+    a.tw = reinterpret_cast<const float2 *>(table);
+    a.H = H;
+    a.rows = (long long)(batch * n);
+    a.n = int(n);
+    a.L = num_iters(n);
+    a.s0 = float(std::sqrt(1.0 / (4.0 * double(n))));
+    a.s1 = float(std::sqrt(1.0 / (2.0 * double(n))));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    a.in = x;
+    a.out = tmp;
+    dctp::launch<dctp::kForward>(a, s);
+    a.in = tmp;
+    a.out = y;
+    if (mode == 0) {
+        dctp::launch<dctp::kForward>(a, s);
+        return launch_status();
+    }
+    dctp::launch<dctp::kFiltered>(a, s);
+    a.in = y;
+    a.out = y;
+    dctp::launch<dctp::kInverse>(a, s);
+    return launch_status();
+}

@@ -583,8 +583,8 @@ def main():
         _, b = solver.data(2000, 0, 1000, noise_seed=2000)
         for op in (_lib.OP_ADRT, _lib.OP_BDRT):
             batched.tune(op, 256, 1000)
-        learned.pcg(learned.normal_op, b, prec=solver.pyramid, rtol=1e-4, maxiter=2)
-        learned.pcg(learned.normal_op, b, prec=solver.pyramid, rtol=1e-4, maxiter=5, compact=0)
+        learned.pcg(learned.normal_op, b, prec=solver.prec, rtol=1e-4, maxiter=2)
+        learned.pcg(learned.normal_op, b, prec=solver.prec, rtol=1e-4, maxiter=5, compact=0)
         torch.cuda.synchronize()
         print(json.dumps({"op": "pcg", "images": 1000, "n": 256, "iterations": 5}), flush=True)
         return

@@ -413,6 +413,22 @@ int adrt_hip_pyr_stencil_up(const float *x, const float *coarse, float *y, const
 int adrt_hip_pyr_correlate(const float *e, const float *x, float *partial, int parts, int batch,
                            int h, int w, void *stream);
 
+/* Radial DCT-domain preconditioner of the learned solver: for float32
+ * images (batch, n, n), n a power of two in [2, 1024], with C the
+ * orthonormal 2-D DCT-II (coefficient [u][v]: u along rows, v along
+ * columns, as scipy.fft.dctn(..., norm="ortho")):
+ *   mode 0: y = C x
+ *   mode 1: y = C^T (H .* C x), H an (n, n) symmetric float32 table
+ * (symmetric positive weights make the filter symmetric positive definite).
+ * table: adrt_hip_dct_table_len(n) float2 twiddles from adrt_hip_dct_table
+ * (a host function), copied to the device; tmp: batch * n * n floats of
+ * workspace; x, y, tmp distinct.  Three row passes (two for mode 0) of
+ * in-LDS FFTs (Makhoul's DCT). */
+size_t adrt_hip_dct_table_len(size_t n);
+int adrt_hip_dct_table(size_t n, float *host_table, size_t floats);
+int adrt_hip_dct_filter(const float *x, float *y, const float *table, const float *H, float *tmp,
+                        size_t batch, size_t n, int mode, void *stream);
+
 /* Ellipse phantoms: out[b][i][j] = sum of val over the ellipses of image b
  * (params[b][k] = cx, cy, cos, sin, ax, ay, val as float64; counts[b] <=
  * max_ellipses) whose closed interior holds the pixel centre

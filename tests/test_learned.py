@@ -551,6 +551,88 @@ class TestPreconditioner:
 
 
 @gpu
+class TestDCTFilter:
+    @pytest.mark.parametrize("n,b", [(2, 3), (4, 1), (8, 5), (32, 3), (64, 2), (128, 3),
+                                     (256, 2), (512, 1)])
+    def test_dct2_matches_scipy(self, cuda, n, b):
+        """The device 2-D DCT-II (Makhoul's algorithm over in-LDS FFTs)
+        against scipy.fft.dctn(norm='ortho') in float64, including batches
+        that leave a work-group partly empty (n < 64)."""
+        import torch
+        from scipy.fft import dctn
+
+        x = np.random.default_rng(n + b).standard_normal((b, n, n)).astype(np.float32)
+        f = LN.DCTFilter(n, device=cuda)
+        got = f.dct2(torch.from_numpy(x).to(cuda)).cpu().numpy()
+        want = dctn(x.astype(np.float64), axes=(1, 2), norm="ortho")
+        assert np.max(np.abs(got - want)) <= 2e-6 * np.sqrt(n) * np.max(np.abs(want))
+
+    @pytest.mark.parametrize("n,b", [(2, 2), (16, 3), (64, 4), (256, 3)])
+    def test_filter_matches_scipy(self, cuda, n, b):
+        """P x = C^T (H .* C x) for a radial profile, against SciPy in float64."""
+        import torch
+        from scipy.fft import dctn, idctn
+
+        rng = np.random.default_rng(n)
+        x = rng.standard_normal((b, n, n)).astype(np.float32)
+        f = LN.DCTFilter(n, device=cuda)
+        f.set_profile(rng.uniform(0.1, 2.0, f.nbins))
+        got = f(torch.from_numpy(x).to(cuda)).cpu().numpy()
+        H = f.h[f.bins]
+        want = idctn(H * dctn(x.astype(np.float64), axes=(1, 2), norm="ortho"), axes=(1, 2),
+                     norm="ortho")
+        assert np.max(np.abs(got - want)) <= 4e-6 * np.sqrt(n) * np.max(np.abs(want))
+
+    def test_symmetric_positive(self, cuda):
+        import torch
+
+        rng = np.random.default_rng(9)
+        f = LN.DCTFilter(128, device=cuda)
+        f.set_profile(rng.uniform(0.1, 2.0, f.nbins))
+        x, y = (torch.from_numpy(rng.standard_normal((1, 128, 128)).astype(np.float32)).to(cuda)
+                for _ in range(2))
+        a, c = float(LN.bdot(f(x), y)), float(LN.bdot(x, f(y)))
+        assert abs(a - c) <= 1e-5 * max(abs(a), 1.0)
+        assert float(LN.bdot(f(x), x)) > 0
+
+    def test_fit_is_least_squares_optimum(self, cuda):
+        """The fitted profile minimises sum ||P(M v) - v||^2: random
+        perturbations of it, and the pyramid, do no better."""
+        import torch
+
+        n = 32
+        solver = LN.LearnedSolver(n=n, device=cuda)
+        vecs = solver._fit_vectors(3, 4)
+        h = solver.dct.fit(vecs)
+        w = LN.normal_op(vecs)
+
+        def obj(prof):
+            solver.dct.set_profile(prof)
+            return float(((solver.dct(w) - vecs).double() ** 2).sum())
+
+        best = obj(h)
+        rng = np.random.default_rng(0)
+        for _ in range(4):
+            assert obj(h * (1 + 0.1 * rng.standard_normal(h.shape))) >= best * (1 - 1e-5)
+        solver.pyramid.fit(vecs)
+        assert float(((solver.pyramid(w) - vecs).double() ** 2).sum()) >= best * (1 - 1e-3)
+
+    def test_fewer_iterations_than_pyramid(self, cuda):
+        """On held-out 64 x 64 systems the fitted DCT filter needs fewer CG
+        iterations than the fitted pyramid, and both fewer than plain CG,
+        to the same solution."""
+        solver = LN.LearnedSolver(n=64, device=cuda)
+        solver.fit_preconditioner(count=8)
+        _, _, b = TestSolver()._problem(cuda, count=6)
+        xp, itp, cp, _ = LN.pcg(LN.normal_op, b, rtol=1e-5)
+        xy, ity, cy, _ = LN.pcg(LN.normal_op, b, prec=solver.pyramid, rtol=1e-5)
+        xd, itd, cd, _ = LN.pcg(LN.normal_op, b, prec=solver.dct, rtol=1e-5)
+        assert bool(cp.all()) and bool(cy.all()) and bool(cd.all())
+        assert float(itd.float().mean()) < float(ity.float().mean()) < float(itp.float().mean())
+        assert float((xd - xp).norm() / xp.norm()) < 1e-3
+
+
+@gpu
 class TestSolver:
     def _problem(self, cuda, n=64, count=3):
         solver = LN.LearnedSolver(n=n, device=cuda)
@@ -596,7 +678,7 @@ class TestSolver:
         solver = LN.LearnedSolver.load(path, device=cuda)
         _, b = solver.data(31337, 0, 4, noise_seed=31337)
         x0 = solver.initial_guess(b, train=False).clone()
-        x, it, conv, _ = LN.pcg(LN.normal_op, b, x0, prec=solver.pyramid, rtol=1e-4)
+        x, it, conv, _ = LN.pcg(LN.normal_op, b, x0, prec=solver.prec, rtol=1e-4)
         xp, itp, convp, _ = LN.pcg(LN.normal_op, b, rtol=1e-4)
         assert bool(conv.all()) and bool(convp.all())
         e = b - LN.normal_op(x)
@@ -614,7 +696,7 @@ class TestSolver:
         solver.train(300, batch=16, pairs=4800, lr=2e-3)
         x, b = solver.data(seed=999, start=0, count=16, noise_seed=999)
         x0 = solver.initial_guess(b).clone()
-        _, it_net, c1, _ = LN.pcg(LN.normal_op, b, x0, prec=solver.pyramid, rtol=1e-4)
-        _, it_zero, c2, _ = LN.pcg(LN.normal_op, b, prec=solver.pyramid, rtol=1e-4)
+        _, it_net, c1, _ = LN.pcg(LN.normal_op, b, x0, prec=solver.prec, rtol=1e-4)
+        _, it_zero, c2, _ = LN.pcg(LN.normal_op, b, prec=solver.prec, rtol=1e-4)
         assert bool(c1.all()) and bool(c2.all())
         assert float(it_net.float().mean()) < float(it_zero.float().mean())
