@@ -778,6 +778,149 @@ __global__ void __launch_bounds__(kWgThreads)
 }
 
 // ---------------------------------------------------------------------------
+// The network's first and last convolutions at full resolution, on the
+// vector ALU: their real channel counts (1 in, 16 out; 16 in, 1 out) would
+// fill a 16-wide matrix tile with zeros.  A work-group computes a tile of
+// kDTH x kDTW output pixels (4 consecutive pixels per thread) from an LDS
+// copy of the input tile with its 1-pixel halo (zero outside the image);
+// each output is one chain of fp32 fused multiply-adds over the taps (and
+// channels), in tap order, then + bias.
+// ---------------------------------------------------------------------------
+constexpr int kDTH = 16, kDTW = 64;
+constexpr int kDPX = 4;  // pixels per thread (kDTH * kDTW / kWgThreads)
+
+// y[p][co] = relu(bias[co] + sum_t w[co][t][0] * bf16(scale * z[p + tap t]))
+// for co < 16 (the first layer: one real input channel; w rows of cinw).
+__global__ void __launch_bounds__(kWgThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    conv_in_kernel(const float *z, float scale, const bf16 *w, int cinw, const float *bias,
+                   bf16 *y, int y_stride, int H, int W) {
+    __shared__ float tile[kDTH + 2][kDTW + 3];
+    __shared__ float wl[9][16];
+    const int tiles_x = (W + kDTW - 1) / kDTW, tiles_y = (H + kDTH - 1) / kDTH;
+    int tt = int(blockIdx.x);
+    const int tx = tt % tiles_x;
+    tt /= tiles_x;
+    const int ty = tt % tiles_y;
+    const long long b = tt / tiles_y;
+    const int x0 = tx * kDTW, y0 = ty * kDTH;
+    const float *zi = z + b * H * W;
+    for (int i = threadIdx.x; i < (kDTH + 2) * (kDTW + 2); i += kWgThreads) {
+        const int r = i / (kDTW + 2), c = i - r * (kDTW + 2);
+        const int gy = y0 + r - 1, gx = x0 + c - 1;
+        const bool ok = unsigned(gy) < unsigned(H) && unsigned(gx) < unsigned(W);
+        tile[r][c] = ok ? float(bf16(scale * zi[(long long)gy * W + gx])) : 0.f;
+    }
+    if (threadIdx.x < 144) {
+        const int t = threadIdx.x >> 4, co = threadIdx.x & 15;
+        wl[t][co] = float(w[((size_t)co * 9 + t) * cinw]);
+    }
+    __syncthreads();
+    const int r = threadIdx.x / (kDTW / kDPX), c0 = (threadIdx.x % (kDTW / kDPX)) * kDPX;
+    float acc[kDPX][16];
+#pragma unroll
+    for (int p = 0; p < kDPX; ++p)
+#pragma unroll
+        for (int co = 0; co < 16; ++co) acc[p][co] = 0.f;
+#pragma unroll 1
+    for (int dy = 0; dy < 3; ++dy) {
+        float xv[kDPX + 2];
+#pragma unroll
+        for (int u = 0; u < kDPX + 2; ++u) xv[u] = tile[r + dy][c0 + u];
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+#pragma unroll
+            for (int co = 0; co < 16; ++co) {
+                const float wv = wl[dy * 3 + dx][co];
+#pragma unroll
+                for (int p = 0; p < kDPX; ++p) acc[p][co] = __builtin_fmaf(wv, xv[p + dx], acc[p][co]);
+            }
+        }
+    }
+    const int gy = y0 + r;
+    if (gy >= H) return;
+#pragma unroll
+    for (int p = 0; p < kDPX; ++p) {
+        const int gx = x0 + c0 + p;
+        if (gx >= W) continue;
+        bf16x8 v0, v1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float a0 = acc[p][j] + bias[j], a1 = acc[p][8 + j] + bias[8 + j];
+            v0[j] = bf16(a0 > 0.f ? a0 : 0.f);
+            v1[j] = bf16(a1 > 0.f ? a1 : 0.f);
+        }
+        bf16 *dst = y + ((b * H + gy) * (long long)W + gx) * y_stride;
+        *reinterpret_cast<bf16x8 *>(dst) = v0;
+        *reinterpret_cast<bf16x8 *>(dst + 8) = v1;
+    }
+}
+
+// y32[p] = resid[p] + (sum_{t, ci} w[0][t][ci] * x[p + tap t][ci] + bias[0])
+// (the last layer: 16 input channels, one real output channel).
+__global__ void __launch_bounds__(kWgThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    conv_out_kernel(const bf16 *x, int x_stride, const bf16 *w, const float *bias, float *y32,
+                    const float *resid, int H, int W) {
+    __shared__ __align__(16) bf16 tile[kDTH + 2][kDTW + 2][16];
+    __shared__ float wl[9][16];
+    const int tiles_x = (W + kDTW - 1) / kDTW, tiles_y = (H + kDTH - 1) / kDTH;
+    int tt = int(blockIdx.x);
+    const int tx = tt % tiles_x;
+    tt /= tiles_x;
+    const int ty = tt % tiles_y;
+    const long long b = tt / tiles_y;
+    const int x0 = tx * kDTW, y0 = ty * kDTH;
+    const long long img = b * H * W;
+    for (int i = threadIdx.x; i < (kDTH + 2) * (kDTW + 2) * 2; i += kWgThreads) {
+        const int half = i & 1, pix = i >> 1;
+        const int r = pix / (kDTW + 2), c = pix - r * (kDTW + 2);
+        const int gy = y0 + r - 1, gx = x0 + c - 1;
+        const bool ok = unsigned(gy) < unsigned(H) && unsigned(gx) < unsigned(W);
+        const long long q = ok ? img + (long long)gy * W + gx : img;
+        *reinterpret_cast<bf16x8 *>(&tile[r][c][8 * half]) =
+            keep_or_zero(load8(x + q * x_stride + 8 * half), ok);
+    }
+    if (threadIdx.x < 144) {
+        const int t = threadIdx.x >> 4, ci = threadIdx.x & 15;
+        wl[t][ci] = float(w[(size_t)t * 16 + ci]);  // output channel 0: w[0][t][ci]
+    }
+    __syncthreads();
+    const int r = threadIdx.x / (kDTW / kDPX), c0 = (threadIdx.x % (kDTW / kDPX)) * kDPX;
+    // taps in order; within a tap the 16 channels in order (channels are
+    // read 4 at a time, so few values are live at once)
+    float acc[kDPX];
+#pragma unroll
+    for (int p = 0; p < kDPX; ++p) acc[p] = 0.f;
+#pragma unroll 1
+    for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll 1
+        for (int dx = 0; dx < 3; ++dx) {
+#pragma unroll
+            for (int c4 = 0; c4 < 16; c4 += 4) {
+                float wv[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) wv[c] = wl[dy * 3 + dx][c4 + c];
+#pragma unroll
+                for (int p = 0; p < kDPX; ++p) {
+                    const bf16 *px = &tile[r + dy][c0 + p + dx][c4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        acc[p] = __builtin_fmaf(wv[c], float(px[c]), acc[p]);
+                }
+            }
+        }
+    }
+    const int gy = y0 + r;
+    if (gy >= H) return;
+#pragma unroll
+    for (int p = 0; p < kDPX; ++p) {
+        const int gx = x0 + c0 + p;
+        if (gx >= W) continue;
+        const long long q = img + (long long)gy * W + gx;
+        y32[q] = resid[q] + (acc[p] + bias[0]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Phantoms: out[b][i][j] = sum of val over the ellipses of image b whose
 // closed interior contains the pixel centre (x, y) = (-1 + (2j+1)/n,
 // 1 - (2i+1)/n); params[b][e] = (cx, cy, cos, sin, ax, ay, val), float64.
@@ -1081,6 +1224,32 @@ int adrt_hip_nn_pack_input(const float *x, void *y, int batch, int h, int w, flo
     const long long P = (long long)batch * h * w;
     hipLaunchKernelGGL(pack_input_kernel, dim3(blocks_for(P)), dim3(kWgThreads), 0,
                        static_cast<hipStream_t>(stream), x, static_cast<bf16 *>(y), P, scale);
+    return launch_status();
+}
+
+int adrt_hip_nn_conv_in(const float *z, float scale, const void *w, int cinw, const float *bias,
+                        void *y, int y_stride, int batch, int h, int w_, void *stream) {
+    if (batch < 1 || h < 1 || w_ < 1 || cinw < 1 || y_stride < 16 || y_stride % 8 != 0)
+        return ADRT_HIP_ERR_ARG;
+    if (!z || !w || !bias || !y || !aligned16(y)) return ADRT_HIP_ERR_NULL;
+    const long long tiles =
+        (long long)batch * ((h + kDTH - 1) / kDTH) * ((w_ + kDTW - 1) / kDTW);
+    hipLaunchKernelGGL(conv_in_kernel, dim3(unsigned(tiles)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), z, scale, static_cast<const bf16 *>(w),
+                       cinw, bias, static_cast<bf16 *>(y), y_stride, h, w_);
+    return launch_status();
+}
+
+int adrt_hip_nn_conv_out(const void *x, int x_stride, const void *w, const float *bias, float *y32,
+                         const float *resid, int batch, int h, int w_, void *stream) {
+    if (batch < 1 || h < 1 || w_ < 1 || x_stride < 16 || x_stride % 8 != 0)
+        return ADRT_HIP_ERR_ARG;
+    if (!x || !w || !bias || !y32 || !resid || !aligned16(x)) return ADRT_HIP_ERR_NULL;
+    const long long tiles =
+        (long long)batch * ((h + kDTH - 1) / kDTH) * ((w_ + kDTW - 1) / kDTW);
+    hipLaunchKernelGGL(conv_out_kernel, dim3(unsigned(tiles)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const bf16 *>(x), x_stride,
+                       static_cast<const bf16 *>(w), bias, y32, resid, h, w_);
     return launch_status();
 }
 

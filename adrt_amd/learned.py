@@ -464,8 +464,9 @@ class UNet:
         if not train:
             return self._infer(z, f, B, n)
         cat0, cat1, cat2 = f["cat0"], f["cat1"], f["cat2"]
+        # the packed input is kept for e0a's weight gradient
         _check(lib.adrt_hip_nn_pack_input(_p(z), _p(f["x8"]), B, n, n, 1.0, s), "pack_input")
-        self._conv(f["x8"], 8, "e0a", f["t0"], 16, B, n, 0)
+        self._conv_in(z, f["t0"], B, n)
         self._conv(f["t0"], 16, "e0b", cat0, 48, B, n, 0)
         _check(lib.adrt_hip_nn_pool(_p(cat0), 48, _p(f["p0"]), 16, B, n, n, 16, s), "pool")
         self._conv(f["p0"], 16, "e1a", f["t1"], 32, B, n1, 0)
@@ -488,8 +489,21 @@ class UNet:
                "upsample")
         self._conv(cat0, 48, "d0a", f["t0d"], 16, B, n, 0)
         self._conv(f["t0d"], 16, "d0b", f["d0"], 16, B, n, 0)
-        self._conv(f["d0"], 16, "out", None, 0, B, n, 1, y32=f["y"], resid=z)
+        self._conv_out(f["d0"], z, f["y"], B, n)
         return f["y"]
+
+    def _conv_in(self, z, y, B, n):
+        """e0a on the vector ALU from the float32 input (one real channel)."""
+        _, _, cin, _ = self.offsets["e0a"]
+        _check(_lib.load().adrt_hip_nn_conv_in(_p(z), 1.0, _p(self.weight("e0a")), cin,
+                                               _p(self.bias("e0a")), _p(y), 16, B, n, n,
+                                               _stream()), "conv e0a")
+
+    def _conv_out(self, d0, z, y, B, n):
+        """The output layer on the vector ALU: y = z + conv(d0)[channel 0] + bias."""
+        _check(_lib.load().adrt_hip_nn_conv_out(_p(d0), 16, _p(self.weight("out")),
+                                                _p(self.bias("out")), _p(y), _p(z), B, n, n,
+                                                _stream()), "conv out")
 
     def _upcat(self, f, skip, cs, up, name, y, B, h):
         """Decoder convolution over [skip | nearest-upsampled up], read in
@@ -524,8 +538,7 @@ class UNet:
         for key, hw, c in (("s0", n, 16), ("s1", n1, 32), ("s2", n2, 64)):
             if key not in f:
                 f[key] = torch.empty((B, hw, hw, c), dtype=torch.bfloat16, device=self.device)
-        _check(lib.adrt_hip_nn_pack_input(_p(z), _p(f["x8"]), B, n, n, 1.0, s), "pack_input")
-        self._conv(f["x8"], 8, "e0a", f["t0"], 16, B, n, 0)
+        self._conv_in(z, f["t0"], B, n)
         self._conv(f["t0"], 16, "e0b", f["s0"], 16, B, n, 0)
         _check(lib.adrt_hip_nn_pool(_p(f["s0"]), 16, _p(f["p0"]), 16, B, n, n, 16, s), "pool")
         self._conv(f["p0"], 16, "e1a", f["t1"], 32, B, n1, 0)
@@ -542,7 +555,7 @@ class UNet:
         self._conv(f["t1d"], 32, "d1b", f["d1"], 32, B, n1, 0)
         self._upcat(f, f["s0"], 16, f["d1"], "d0a", f["t0d"], B, n)
         self._conv(f["t0d"], 16, "d0b", f["d0"], 16, B, n, 0)
-        self._conv(f["d0"], 16, "out", None, 0, B, n, 1, y32=f["y"], resid=z)
+        self._conv_out(f["d0"], z, f["y"], B, n)
         return f["y"]
 
     def backward(self, g_scale_target, B, n):
@@ -650,6 +663,8 @@ class LearnedSolver:
         self.pyramid = Pyramid(levels=int(math.log2(n)), device=self.device)
         self.dct = DCTFilter(n, device=self.device)
         self.net = UNet(seed=seed, device=self.device)
+        # the filter that forms the network's input z = P b
+        self.input_prec = "pyramid"
 
     @property
     def prec(self):
@@ -683,12 +698,13 @@ class LearnedSolver:
         return self.dct.fit(self._fit_vectors(seed, count))
 
     def initial_guess(self, b, train=True):
-        z = self.pyramid(b)
+        z = (self.dct if self.input_prec == "dct" else self.pyramid)(b)
         return self.net.forward(z, train)
 
     def save(self, path, **extra):
         np.savez(path, unet=self.net.w.cpu().numpy(), theta=self.pyramid.theta,
-                 dct_h=self.dct.h, n=self.n, sigma_frac=self.sigma_frac, **extra)
+                 dct_h=self.dct.h, input_prec=self.input_prec, n=self.n,
+                 sigma_frac=self.sigma_frac, **extra)
 
     @classmethod
     def load(cls, path, device=None):
@@ -700,6 +716,7 @@ class LearnedSolver:
             solver.pyramid.set_theta(f["theta"])
             solver.net.load_state(f["unet"])
             h = f["dct_h"] if "dct_h" in f.files else None
+            solver.input_prec = str(f["input_prec"]) if "input_prec" in f.files else "pyramid"
         if h is None:
             solver.fit_dct()
         else:
@@ -754,7 +771,7 @@ class LearnedSolver:
         return hist
 
 
-def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=False, compact=0.75):
+def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=False, compact=0.97):
     """Batched preconditioned CG (plain CG when prec is None) for (B, n, n)
     systems with float64 scalars per image.  Stops each image when
     ||b - M x|| <= rtol * ||b|| (the recursively updated residual).  An image
@@ -863,7 +880,7 @@ def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=False, compa
     return x_out, iters_out, conv_out, apps
 
 
-def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5):
+def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5, compact=0.97):
     """Wall time to relative residual ``rtol`` on ``count`` held-out phantoms
     (phantom seed ``seed``, noise seed ``seed``) for plain CG, CG with the
     multiscale preconditioner from zero, and the learned method (U-Net initial
@@ -915,13 +932,14 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5):
     pcg(normal_op, b, prec=solver.prec, rtol=rtol, maxiter=3)
     times = {"plain": [], "learned": [], "prec": [], "net": []}
     for _ in range(max(int(repeats), 1)):
-        plain, t = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter))
+        plain, t = run(lambda: pcg(normal_op, b, rtol=rtol, maxiter=maxiter, compact=compact))
         times["plain"].append(t)
         learned, t = run(lambda: pcg(normal_op, b, solver.initial_guess(b, train=False),
-                                     prec=solver.prec, rtol=rtol, maxiter=maxiter))
+                                     prec=solver.prec, rtol=rtol, maxiter=maxiter,
+                                     compact=compact))
         times["learned"].append(t)
         prec_only, t = run(lambda: pcg(normal_op, b, prec=solver.prec, rtol=rtol,
-                                       maxiter=maxiter))
+                                       maxiter=maxiter, compact=compact))
         times["prec"].append(t)
         x0, t = run(lambda: solver.initial_guess(b, train=False))
         times["net"].append(t)
@@ -942,6 +960,7 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5):
 
     out = {
         "count": count, "n": solver.n, "rtol": rtol, "repeats": len(times["plain"]),
+        "compact": compact, "network_input": solver.input_prec,
         "tuned_adrt_bdrt": tuning,
         "plain_cg": stats(plain, times["plain"]),
         "preconditioner_only": stats(prec_only, times["prec"]),

@@ -33,6 +33,8 @@ def main(argv=None):
                    help="stop after this many seconds of training (0: no limit)")
     p.add_argument("--log-every", type=int, default=100)
     p.add_argument("--target", default="phantom", choices=["phantom", "solution", "residual"])
+    p.add_argument("--input", default="pyramid", choices=["pyramid", "dct"],
+                   help="preconditioner forming the network's input z = P b")
     p.add_argument("--out", required=True)
     args = p.parse_args(argv)
 
@@ -42,6 +44,7 @@ def main(argv=None):
 
     torch.cuda.init()
     solver = learned.LearnedSolver(n=args.n, sigma_frac=args.sigma)
+    solver.input_prec = args.input
     t0 = time.perf_counter()
     solver.fit_preconditioner()
     torch.cuda.synchronize()

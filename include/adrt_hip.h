@@ -392,6 +392,19 @@ int adrt_hip_nn_upsample_backward(const void *dy, int dy_stride, void *dx, int d
 int adrt_hip_nn_pack_input(const float *x, void *y, int batch, int h, int w, float scale,
                            void *stream);
 
+/* The network's first and last 3x3 convolutions (zero padding) on the vector
+ * ALU, fp32 fused multiply-adds in tap (then channel) order:
+ *   conv_in:  y[p][co] = relu(bias[co] + sum_t w[co][t][0] * bf16(scale * z[p + t])),
+ *             co < 16, z fp32 (batch, h, w_), w bf16 [16][9][cinw], y bf16
+ *             NHWC with pixel stride y_stride (16-byte aligned);
+ *   conv_out: y32[p] = resid[p] + (sum_{t, ci < 16} w[0][t][ci] * x[p + t][ci]
+ *             + bias[0]), x bf16 NHWC (16 channels, pixel stride x_stride),
+ *             w bf16 [cout][9][16] (row 0 used). */
+int adrt_hip_nn_conv_in(const float *z, float scale, const void *w, int cinw, const float *bias,
+                        void *y, int y_stride, int batch, int h, int w_, void *stream);
+int adrt_hip_nn_conv_out(const void *x, int x_stride, const void *w, const float *bias, float *y32,
+                         const float *resid, int batch, int h, int w_, void *stream);
+
 /* 16-channel bf16 gradient: channel 0 = scale * (pred - target), others 0. */
 int adrt_hip_nn_loss_grad(const float *pred, const float *target, void *dz, int batch, int h,
                           int w, float scale, void *stream);

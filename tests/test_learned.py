@@ -551,6 +551,72 @@ class TestPreconditioner:
 
 
 @gpu
+class TestDirectConvolutions:
+    """The first and last layers' vector-ALU kernels against float64
+    convolutions of the same bf16-rounded values (zero padding)."""
+
+    @staticmethod
+    def _conv_ref(x, w):
+        # x (B, H, W, C) float64, w (Cout, 9, C): y[b, i, j, co]
+        B, H, W, C = x.shape
+        xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (0, 0)))
+        y = np.zeros((B, H, W, w.shape[0]))
+        for t in range(9):
+            dy, dx = divmod(t, 3)
+            y += np.einsum("bhwc,oc->bhwo", xp[:, dy:dy + H, dx:dx + W], w[:, t, :])
+        return y
+
+    @pytest.mark.parametrize("shape", [(2, 256, 256), (3, 32, 48), (1, 20, 70)])
+    def test_conv_in(self, cuda, shape):
+        import torch
+        from adrt_amd import _lib
+
+        rng = np.random.default_rng(shape[1])
+        B, H, W = shape
+        z = rng.standard_normal(shape).astype(np.float32)
+        w = torch.from_numpy(rng.standard_normal(16 * 9 * 8).astype(np.float32) * 0.3).to(torch.bfloat16)
+        bias = rng.standard_normal(16).astype(np.float32) * 0.1
+        y = torch.full((B, H, W, 16), float("nan"), dtype=torch.bfloat16, device=cuda)
+        # device copies held until the kernel has run
+        zd, wd, bd = torch.from_numpy(z).to(cuda), w.to(cuda), torch.from_numpy(bias).to(cuda)
+        st = _lib.load().adrt_hip_nn_conv_in(zd.data_ptr(), 1.0, wd.data_ptr(), 8, bd.data_ptr(),
+                                             y.data_ptr(), 16, B, H, W,
+                                             torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        torch.cuda.synchronize()
+        zb = torch.from_numpy(z).to(torch.bfloat16).double().numpy()[..., None]
+        wf = w.double().numpy().reshape(16, 9, 8)[:, :, :1]
+        want = np.maximum(self._conv_ref(zb, wf) + bias, 0)
+        got = y.float().cpu().numpy()
+        # bf16 output rounding (8 bits) dominates the fp32 accumulation error
+        assert np.all(np.abs(got - want) <= 2 ** -7 * np.abs(want) + 1e-6)
+
+    @pytest.mark.parametrize("shape", [(2, 256, 256), (3, 32, 48), (1, 20, 70)])
+    def test_conv_out(self, cuda, shape):
+        import torch
+        from adrt_amd import _lib
+
+        rng = np.random.default_rng(shape[2])
+        B, H, W = shape
+        x = torch.from_numpy(rng.standard_normal((B, H, W, 16)).astype(np.float32)).to(torch.bfloat16)
+        w = torch.from_numpy(rng.standard_normal(16 * 9 * 16).astype(np.float32) * 0.3).to(torch.bfloat16)
+        bias = rng.standard_normal(16).astype(np.float32)
+        resid = rng.standard_normal((B, H, W)).astype(np.float32)
+        y = torch.full((B, H, W), float("nan"), device=cuda)
+        xd, wd = x.to(cuda), w.to(cuda)
+        bd, rd = torch.from_numpy(bias).to(cuda), torch.from_numpy(resid).to(cuda)
+        st = _lib.load().adrt_hip_nn_conv_out(xd.data_ptr(), 16, wd.data_ptr(), bd.data_ptr(),
+                                              y.data_ptr(), rd.data_ptr(), B, H, W,
+                                              torch.cuda.current_stream().cuda_stream)
+        assert st == 0
+        torch.cuda.synchronize()
+        wf = w.double().numpy().reshape(16, 9, 16)[:1]
+        want = resid + (self._conv_ref(x.double().numpy(), wf)[..., 0] + bias[0])
+        got = y.cpu().numpy()
+        assert np.max(np.abs(got - want)) <= 1e-5 * (np.abs(want).max() + 1)
+
+
+@gpu
 class TestDCTFilter:
     @pytest.mark.parametrize("n,b", [(2, 3), (4, 1), (8, 5), (32, 3), (64, 2), (128, 3),
                                      (256, 2), (512, 1)])
