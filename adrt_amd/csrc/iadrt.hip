@@ -169,6 +169,95 @@ __host__ __device__ inline bool pair_step(const PairLane &pp, int y, int R, T p0
     return true;
 }
 
+// Two paired stages in one pass (stage k with Co >= 32 and stage k + 1).
+// The wave's 64 stage-k chains (lanes 0-31: even group 2pl, columns
+// c0 .. c0 + 31; lanes 32-63: odd group 2pl + 1, the same columns; computed
+// exactly as in the pair kernel) are the sources of 64 stage-(k+1) chains of
+// columns col' = c0/2 + i (i < 16) in the four output groups 4pl + type:
+//   lane 2i       type A (group 4pl, even):      a = S_e[2col'][r],     b = S_e[2col'+1][r+1]
+//   lane 2i + 1   type B (group 4pl + 1, odd):   a = S_e[2col'+1][u],   b = S_e[2col'][u]
+//   lane 32 + 2i  type C (group 4pl + 2, even):  a = S_o[2col'][r],     b = S_o[2col'+1][r+1]
+//   lane 32 + 2i + 1  type D (group 4pl + 3):    a = S_o[2col'+1][u],   b = S_o[2col'][u]
+// (u = r + 1 + col'; S_e / S_o the stage-k chains of the even / odd group).
+// A stage-k odd chain of column c produces row y - 1 - c while reading
+// source row y, so at step y each operand is a value the lane itself or its
+// partner lane (lane ^ 1) produced at this step or one or two steps before:
+//   A: a = own now,  b = partner's previous,  row r = y
+//   B: a = own now,  b = partner's now,       r = y - 1 - col'
+//   C: a = own now,  b = partner's 2 steps ago, r = y - 1 - 2col'
+//   D: a = own previous, b = partner's now,   r = y - 2 - 3col'
+// Each lane keeps its last three stage-k values and sends its partner the
+// one the partner needs (one lane swap per step).  Rows of a stage-k odd
+// chain above its first source row, and of every stage-(k+1) chain above
+// its first row reached in the sweep, are +0 (their sources are +0 rows);
+// the histories start at +0 for them.  Output rows lag by type and column;
+// each type stores, at step y, its row y - minlag (the row its least-lagged
+// lane produces now, which every lane of the type has produced by then)
+// through a ring per type in LDS of 16 (B), 32 (C) or 64 (D) rows (lag
+// spreads 15, 30, 45); A stores its row y directly.  The rows above the
+// first flushed row are filled with +0 after the sweep.
+constexpr int kQuadRing = 16 * (16 + 32 + 64);  // ring elements of B, C and D
+
+struct QuadLane {
+    PairLane p;  // the lane's stage-k chain
+    int type;    // 0 A, 1 B, 2 C, 3 D
+    int colq;    // stage-(k+1) column in its group
+    int lag;     // stage-(k+1) output row = y - lag
+    int minlag;  // smallest lag of the type in the wave (flush row = y - minlag)
+    int oc;      // stage-(k+1) output column
+    int rbase, rmask, ri;  // ring element of row r: rbase + (r & rmask) * 16 + ri
+};
+
+__host__ __device__ inline QuadLane make_quad_lane(int n, int Li, int wave_in_plane, int lane,
+                                                   int &c0) {
+    QuadLane q;
+    q.p = make_pair_lane(n, Li, wave_in_plane, lane, c0);
+    const int Ci = n / Li, Co = Ci / 2, Coq = Co / 2;
+    const int e = lane & (kHalf - 1);
+    const int i = e >> 1;
+    const int pl = wave_in_plane / (Co / kHalf);
+    q.type = ((lane >> 5) << 1) | (e & 1);
+    q.colq = c0 / 2 + i;
+    q.oc = (4 * pl + q.type) * Coq + q.colq;
+    const int cm = c0 / 2;
+    const int k = q.type == 0 ? 0 : (q.type == 1 ? 1 : (q.type == 2 ? 2 : 3));
+    const int mult = k == 3 ? 3 : k;        // lag = add + mult * col'
+    const int add = k == 0 ? 0 : (k == 3 ? 2 : 1);
+    q.lag = add + mult * q.colq;
+    q.minlag = add + mult * cm;
+    q.ri = i;
+    q.rmask = k == 1 ? 15 : (k == 2 ? 31 : 63);
+    q.rbase = k == 1 ? 0 : (k == 2 ? 16 * 16 : 16 * (16 + 32));
+    return q;
+}
+
+// The value this lane sends its partner at a step (h0 = this step's stage-k
+// value, h1 the previous, h2 the one before).
+template <typename T>
+__host__ __device__ __forceinline__ T quad_send(const QuadLane &q, T h0, T h1, T h2) {
+    return q.type == 1 ? h1 : (q.type == 3 ? h2 : h0);
+}
+
+// Stage k + 1 at step y: a = the lane's own operand (D: h1, else h0), b =
+// the partner's; the reference's order of additions.
+template <typename T>
+__host__ __device__ __forceinline__ bool quad_step(const QuadLane &q, int y, int R, T a, T b,
+                                                   T &acc, T &val, int &row) {
+    row = y - q.lag;
+    if (row < 0) return false;
+    T v = T(0);
+    v += a;
+    if (q.type == 0 || q.type == 2) {  // even chains: b enters while row + 1 < R
+        if (row + 1 < R) v -= b;
+    } else {  // odd chains: row + 1 + col' < R at every row the sweep reaches
+        v -= b;
+    }
+    if (row + 1 < R) v += acc;
+    acc = v;
+    val = v;
+    return true;
+}
+
 // Fused tail.  Once a stage's input groups are at most 64 columns wide
 // (Ci <= 64), one wavefront's 64 columns hold whole groups of that stage and
 // of every later one, so the last S stages run in one pass: the first of
@@ -398,6 +487,83 @@ __global__ void __launch_bounds__(kLanes)
         for (int r = R - 1 - c0; r < R; ++r) dst[size_t(r) * n] = T(0);
 }
 
+template <typename T>
+__device__ __forceinline__ T lane_swap1(T v);
+template <>
+__device__ __forceinline__ float lane_swap1<float>(float v) {
+    return __builtin_bit_cast(float, __shfl_xor(__builtin_bit_cast(int, v), 1));
+}
+template <>
+__device__ __forceinline__ double lane_swap1<double>(double v) {
+    return __shfl_xor(v, 1);
+}
+
+// Two paired stages (stage k has Co >= 32): grid = planes * (n / 64)
+// single-wave work-groups; `in` holds stage k's input, `out` receives stage
+// k + 1's output.
+template <typename T>
+__global__ void __launch_bounds__(kLanes)
+    quad_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
+                int waves_per_plane) {
+    using P2 = typename Pair<T>::type;
+    __shared__ T ring[kQuadRing];
+    const int lane = threadIdx.x;
+    const size_t plane = blockIdx.x / waves_per_plane;
+    int c0;
+    const QuadLane q = make_quad_lane(n, Li, int(blockIdx.x % waves_per_plane), lane, c0);
+    const T *src = in + plane * size_t(R) * n + q.p.pc;
+    T *dst = out + plane * size_t(R) * n + q.oc;
+    for (int e = lane; e < kQuadRing; e += kLanes) ring[e] = T(0);
+    __syncthreads();
+
+    auto load = [&](int t) -> P2 {
+        const int y = R - 1 - t;
+        return *reinterpret_cast<const P2 *>(src + size_t(y < 0 ? 0 : y) * n);
+    };
+    T acc = T(0), prev1 = T(0), acc2 = T(0);
+    T h0 = T(0), h1 = T(0), h2 = T(0);
+    auto run_step = [&](int t, const P2 &p) {
+        const int y = R - 1 - t;
+        T val;
+        int row;
+        // stage k (values of rows a stage-k odd chain does not reach stay +0)
+        const bool made = pair_step<T>(q.p, y, R, p.x, p.y, prev1, acc, val, row);
+        prev1 = p.y;
+        h2 = h1;
+        h1 = h0;
+        h0 = made ? val : T(0);
+        // stage k + 1
+        const T b = lane_swap1<T>(quad_send<T>(q, h0, h1, h2));
+        const T a = q.type == 3 ? h1 : h0;
+        T v2;
+        int row2;
+        const bool made2 = quad_step<T>(q, y, R, a, b, acc2, v2, row2);
+        if (q.type != 0 && made2) ring[q.rbase + (row2 & q.rmask) * 16 + q.ri] = v2;
+        // one store per step: A its row y, the others their type's row y - minlag
+        const int f = y - q.minlag;
+        const T ov = q.type == 0 ? v2 : ring[q.rbase + (f & q.rmask) * 16 + q.ri];
+        if (f >= 0) dst[size_t(f) * n] = ov;
+    };
+
+    P2 cur[kPairSteps];
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < kPairSteps; ++k) cur[k] = load(k);
+    for (; t + kPairSteps <= R; t += kPairSteps) {  // whole blocks
+        P2 nxt[kPairSteps];
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) nxt[k] = load(t + kPairSteps + k);
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) run_step(t + k, cur[k]);
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) cur[k] = nxt[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kPairSteps; ++k)  // tail: steps t .. R-1
+        if (t + k < R) run_step(t + k, cur[k]);
+    for (int r = R - q.minlag; r < R; ++r) dst[size_t(r) * n] = T(0);  // the +0 rows
+}
+
 // The last S stages (the first with Li0 groups, Ci = n / Li0 = 2^S), one
 // single-wave work-group per 64 columns of a plane.
 template <typename T, int S>
@@ -509,6 +675,19 @@ inline int tail_stages(int L, int variant = ADRT_HIP_VARIANT_DEFAULT) {
     return L < cap ? L : cap;
 }
 
+// Passes of the k0 stages before the tail: two paired stages (Co >= 32)
+// per pass while both remain (with `quad`), else one.
+inline int iadrt_passes(int n, int k0, bool quad) {
+    int passes = 0, Li = 1;
+    for (int k = 0; k < k0; ++passes) {
+        const bool paired = n / Li / 2 >= iadrt_native::kHalf;
+        const int step = paired && quad && k + 1 < k0 ? 2 : 1;
+        k += step;
+        Li <<= step;
+    }
+    return passes;
+}
+
 #ifndef ADRT_HOST_EMULATION
 template <typename T>
 static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
@@ -520,31 +699,45 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
             return launch_status();
         return launch_status();
     }
-    // the last S stages run fused (S = 1: no tail kernel)
-    const int S = tail_stages<T>(L, fused_variant(ADRT_HIP_OP_IADRT, n));
-    const int k0 = L - (S >= 2 ? S : 0);  // stages run one per pass
-    const int passes = k0 + (S >= 2 ? 1 : 0);
+    // the last S stages run fused (S = 1: no tail kernel); before them,
+    // pairs of paired stages run as one pass each (not in STREAMING)
+    const int variant = fused_variant(ADRT_HIP_OP_IADRT, n);
+    const int S = tail_stages<T>(L, variant);
+    const bool quad = variant != ADRT_HIP_VARIANT_STREAMING;
+    const int k0 = L - (S >= 2 ? S : 0);  // stages before the tail
+    const int passes = iadrt_passes(int(n), k0, quad) + (S >= 2 ? 1 : 0);
     // ping-pong so that the last pass writes `out`
     T *bufs[2] = {(passes % 2 == 1) ? out : ws, (passes % 2 == 1) ? ws : out};
     const int waves_per_plane = int((n + iadrt_native::kLanes - 1) / iadrt_native::kLanes);
     const unsigned grid = unsigned(B * 4 * size_t(waves_per_plane));
     const T *src = in;
     int Li = 1;
-    for (int k = 0; k < k0; ++k) {
-        T *dst = bufs[k & 1];
-        if (int(n) / Li / 2 >= iadrt_native::kHalf)  // Co >= 32: even and odd chains paired
-            hipLaunchKernelGGL(iadrt_native::pair_kernel<T>, dim3(grid),
+    int pass = 0;
+    for (int k = 0; k < k0; ++pass) {
+        T *dst = bufs[pass & 1];
+        const bool paired = int(n) / Li / 2 >= iadrt_native::kHalf;  // Co >= 32
+        if (paired && quad && k + 1 < k0) {  // stages k and k + 1 in one pass
+            hipLaunchKernelGGL(iadrt_native::quad_kernel<T>, dim3(grid),
                                dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n), Li,
                                waves_per_plane);
-        else
-            hipLaunchKernelGGL(iadrt_native::stage_kernel<T>, dim3(grid),
-                               dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n), Li,
-                               waves_per_plane);
+            k += 2;
+            Li *= 4;
+        } else {
+            if (paired)  // even and odd chains paired
+                hipLaunchKernelGGL(iadrt_native::pair_kernel<T>, dim3(grid),
+                                   dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n), Li,
+                                   waves_per_plane);
+            else
+                hipLaunchKernelGGL(iadrt_native::stage_kernel<T>, dim3(grid),
+                                   dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
+                                   Li, waves_per_plane);
+            k += 1;
+            Li *= 2;
+        }
         src = dst;
-        Li *= 2;
     }
     if (S >= 2) {
-        T *dst = bufs[k0 & 1];
+        T *dst = bufs[pass & 1];
         const dim3 g(grid), blk(iadrt_native::kLanes);
         const int r = int(R), nn = int(n);
         switch (S) {

@@ -6,9 +6,10 @@
 // 64-row ring exactly sized as the kernel's LDS ring (poisoned with NaN), and
 // row s is flushed at step s with the kernel's unconditional stores.  Pair loads use the kernel's clamped rows.
 // Buffers are exactly sized, so an out-of-bounds access aborts under
-// AddressSanitizer.  Usage:  iadrt_emu f32|f64 B n in.bin out.bin [tail_stages]
+// AddressSanitizer.  Usage:  iadrt_emu f32|f64 B n in.bin out.bin [tail_stages] [quad]
 // (tail_stages: how many final stages the fused tail runs; default: the
-// library's choice; 0: none).
+// library's choice; 0: none; quad 0: every paired stage in its own pass,
+// default 1: two paired stages per pass where both precede the tail).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -125,6 +126,84 @@ static void emulate_pair_stage(const std::vector<T> &in, std::vector<T> &out, in
     }
 }
 
+// Two paired stages in one pass (the quad kernel): lanes in lockstep, each
+// keeping its last three stage-k values; the partner exchange is lane ^ 1;
+// the B / C / D rings are exactly the kernel's, every slot tagged with the
+// row it holds (+0 slots with their own tag), and every flush checks that it
+// reads its row or a +0 row above the lane's first real row.
+template <typename T>
+static void emulate_quad_stage(const std::vector<T> &in, std::vector<T> &out, int planes, int R,
+                               int n, int Li) {
+    const int waves = n / kLanes;
+    const int kZero = -(1 << 30);
+    for (int plane = 0; plane < planes; ++plane) {
+        const T *src = in.data() + size_t(plane) * R * n;
+        T *dst = out.data() + size_t(plane) * R * n;
+        for (int wv = 0; wv < waves; ++wv) {
+            QuadLane q[kLanes];
+            int c0 = 0;
+            for (int l = 0; l < kLanes; ++l) {
+                q[l] = make_quad_lane(n, Li, wv, l, c0);
+                if (q[l].p.pc < 0 || q[l].p.pc + 1 >= n || q[l].oc < 0 || q[l].oc >= n)
+                    std::abort();
+                if (q[l].type != 0 &&
+                    (q[l].rbase + q[l].rmask * 16 + q[l].ri >= kQuadRing || q[l].ri > 15))
+                    std::abort();
+            }
+            std::vector<T> ring(kQuadRing, T(0));
+            std::vector<int> tag(kQuadRing, kZero);
+            T acc[kLanes], prev1[kLanes], acc2[kLanes], h0[kLanes], h1[kLanes], h2[kLanes];
+            for (int l = 0; l < kLanes; ++l)
+                acc[l] = prev1[l] = acc2[l] = h0[l] = h1[l] = h2[l] = T(0);
+            for (int t = 0; t < R; ++t) {
+                const int y = R - 1 - t;
+                for (int l = 0; l < kLanes; ++l) {
+                    const size_t at = size_t(y) * n + q[l].p.pc;
+                    const T p0 = src[at], p1 = src[at + 1];
+                    T val;
+                    int row;
+                    const bool made = pair_step<T>(q[l].p, y, R, p0, p1, prev1[l], acc[l], val, row);
+                    prev1[l] = p1;
+                    h2[l] = h1[l];
+                    h1[l] = h0[l];
+                    h0[l] = made ? val : T(0);
+                }
+                T send[kLanes];
+                for (int l = 0; l < kLanes; ++l) send[l] = quad_send<T>(q[l], h0[l], h1[l], h2[l]);
+                for (int l = 0; l < kLanes; ++l) {
+                    const T b = send[l ^ 1];
+                    const T a = q[l].type == 3 ? h1[l] : h0[l];
+                    T v2;
+                    int row2;
+                    const bool made2 = quad_step<T>(q[l], y, R, a, b, acc2[l], v2, row2);
+                    const int f = y - q[l].minlag;
+                    if (q[l].type == 0) {
+                        if (!made2 || row2 != y) std::abort();
+                        dst[size_t(y) * n + q[l].oc] = v2;
+                        continue;
+                    }
+                    if (made2) {
+                        const size_t slot = size_t(q[l].rbase + (row2 & q[l].rmask) * 16 + q[l].ri);
+                        ring.at(slot) = v2;
+                        tag.at(slot) = row2;
+                    }
+                    if (f < 0) continue;
+                    const size_t slot = size_t(q[l].rbase + (f & q[l].rmask) * 16 + q[l].ri);
+                    const int first = R - 1 - q[l].lag;  // the lane's first row in the sweep
+                    const bool skipped = tag[slot] == kZero && f > first;
+                    if (tag[slot] != f && !skipped) {
+                        std::fprintf(stderr, "stale quad flush: row %d lane %d\n", f, l);
+                        std::abort();
+                    }
+                    dst[size_t(f) * n + q[l].oc] = ring[slot];
+                }
+            }
+            for (int l = 0; l < kLanes; ++l)
+                for (int r = R - q[l].minlag; r < R; ++r) dst[size_t(r) * n + q[l].oc] = T(0);
+        }
+    }
+}
+
 // The fused tail kernel: the last S stages, the first with Li0 groups.  Lanes
 // run in lockstep, in the kernel's order within a step: every later stage's
 // ring reads, then stage 0, then the later stages' results.  The ring buffer
@@ -230,7 +309,7 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
 }
 
 template <typename T>
-static int run(int B, int n, const char *fin, const char *fout, int tail) {
+static int run(int B, int n, const char *fin, const char *fout, int tail, bool quad) {
     const int R = 2 * n - 1;
     const size_t elems = size_t(B) * 4 * R * n;
     std::vector<T> a(elems), b(elems, T(NAN));
@@ -242,14 +321,22 @@ static int run(int B, int n, const char *fin, const char *fout, int tail) {
     const int S = tail < 0 ? tail_stages<T>(L) : (tail < L ? tail : L);
     const int k0 = L - (S >= 2 ? S : 0);
     int Li = 1;
-    for (int k = 0; k < k0; ++k) {
-        if (n / Li / 2 >= kHalf)
-            emulate_pair_stage<T>(a, b, B * 4, R, n, Li);
-        else
-            emulate_stage<T>(a, b, B * 4, R, n, Li);
+    for (int k = 0; k < k0;) {
+        const bool paired = n / Li / 2 >= kHalf;
+        if (paired && quad && k + 1 < k0) {
+            emulate_quad_stage<T>(a, b, B * 4, R, n, Li);
+            k += 2;
+            Li *= 4;
+        } else {
+            if (paired)
+                emulate_pair_stage<T>(a, b, B * 4, R, n, Li);
+            else
+                emulate_stage<T>(a, b, B * 4, R, n, Li);
+            k += 1;
+            Li *= 2;
+        }
         std::swap(a, b);
         for (auto &x : b) x = T(NAN);
-        Li *= 2;
     }
     if (S >= 2) {
         emulate_tail<T>(a, b, B * 4, R, n, Li, S);
@@ -262,12 +349,14 @@ static int run(int B, int n, const char *fin, const char *fout, int tail) {
 }
 
 int main(int argc, char **argv) {
-    if (argc != 6 && argc != 7) {
-        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [tail_stages]\n", argv[0]);
+    if (argc < 6 || argc > 8) {
+        std::fprintf(stderr, "usage: %s f32|f64 B n in.bin out.bin [tail_stages] [quad 0|1]\n",
+                     argv[0]);
         return 1;
     }
     const int B = std::atoi(argv[2]), n = std::atoi(argv[3]);
-    const int tail = argc == 7 ? std::atoi(argv[6]) : -1;
-    if (std::strcmp(argv[1], "f32") == 0) return run<float>(B, n, argv[4], argv[5], tail);
-    return run<double>(B, n, argv[4], argv[5], tail);
+    const int tail = argc >= 7 ? std::atoi(argv[6]) : -1;
+    const bool quad = argc == 8 ? std::atoi(argv[7]) != 0 : true;
+    if (std::strcmp(argv[1], "f32") == 0) return run<float>(B, n, argv[4], argv[5], tail, quad);
+    return run<double>(B, n, argv[4], argv[5], tail, quad);
 }

@@ -199,6 +199,32 @@ def test_iadrt_fused_tail_emulation(iadrt_emulator, tmp_path, dt, n, tail):
     assert np.array_equal(got.view(np.uint8), O.iadrt(x).view(np.uint8))
 
 
+@pytest.mark.parametrize("dt,n,tail,quad", [("float32", 128, 5, 1), ("float64", 128, 5, 1),
+                                           ("float32", 512, 5, 1), ("float64", 512, 2, 1),
+                                           ("float32", 512, 5, 0), ("float32", 2048, 5, 1),
+                                           ("float32", 64, 0, 1), ("float64", 32, 0, 1)])
+def test_iadrt_two_stage_pass_emulation(iadrt_emulator, tmp_path, dt, n, tail, quad):
+    """Two paired stages per pass (the quad kernel: the four stage-(k+1)
+    chain types fed by lane pairs, the B / C / D rings with every flush
+    checked against the row its slot holds) and, with quad 0, one paired stage
+    per pass: byte-equal to the oracle (signs of zero included), at sizes
+    where the passes run 1, 2 and 3 of them before the tail (512: 2 plus one
+    single, 2048: 3) and with no tail (Co of the second stage 16)."""
+    rng = np.random.default_rng(n + 3 * tail + quad)
+    b = 1
+    x = (rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)
+    x.flat[:3] = [-0.0, 0.0, -0.0]
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    x.tofile(fin)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    res = subprocess.run([iadrt_emulator, "f32" if dt == "float32" else "f64", str(b), str(n),
+                          str(fin), str(fout), str(tail), str(quad)], capture_output=True,
+                         text=True, env=env)
+    assert res.returncode == 0, res.stderr[-4000:]
+    got = np.fromfile(fout, dtype=dt).reshape(x.shape)
+    assert np.array_equal(got.view(np.uint8), O.iadrt(x).view(np.uint8))
+
+
 SMALL_SRC = os.path.join(HERE, "host_emulation", "small_emu.cpp")
 
 
