@@ -148,52 +148,103 @@ def _apply_adjoint(x, r):
     return g
 
 
-def lsq_cg(b, x0=None, *, iters=200, callback=None, check_every=16):
-    """Least squares min_x 0.5 * ||adrt(x) - b||^2 by CGLS for a batch of
-    ADRT data b (B, 4, 2n-1, n) (or unbatched).  Runs ``iters`` iterations;
-    every ``check_every`` iterations the host checks whether every image's
-    gradient has vanished and stops early if so (an image whose gradient
-    vanished meanwhile takes zero steps: alpha = beta = 0).  Returns
-    ``(x, loss_history)``; the history holds per-image losses (float64
-    tensors) before the first and after every iteration run.
+def lsq_cg(b, x0=None, *, iters=200, callback=None, check_every=16, graph=True):
+    """Least squares min_x 0.5 * ||adrt(x) - b||^2 by conjugate gradients on
+    the normal equations A^T A x = A^T b (CGNR: the CGLS iterates in exact
+    arithmetic) for a batch of ADRT data b (B, 4, 2n-1, n) (or unbatched).
 
-    Each iteration applies A once and A^T once: the autograd graph of
-    ``q = adrt(p)`` also gives ``A^T r`` for the updated residual (the
-    backward of a linear map does not depend on the point), so no forward is
-    run only to reach its backward."""
+    Every vector of the iteration lives in image space: one step applies A
+    once (q = adrt(p), with the autograd graph) and A^T once (A^T q, the
+    autograd backward: the hand-written back-projection kernel), with
+    alpha = ||s||^2 / p.(A^T A p) (= ||A p||^2), x += alpha p,
+    s -= alpha A^T A p, beta = ||s'||^2 / ||s||^2, p = s' + beta p, in two
+    device passes (adrt_hip_pcg_update / adrt_hip_pcg_direction) that keep
+    every float64 per-image scalar on the device.  The loss follows the
+    recurrence ||r_{k+1}||^2 = ||r_k||^2 - alpha_k ||s_k||^2 (exact in exact
+    arithmetic), so no sinogram-sized residual is kept; the first and the
+    last history entries are computed directly from b - A x.
+
+    Runs ``iters`` steps; every ``check_every`` steps the host checks whether
+    every image's gradient has vanished and stops early if so (an image
+    whose gradient vanished meanwhile takes zero steps: alpha = beta = 0).
+    With ``graph`` (default) the first block of ``check_every`` steps runs
+    eagerly and the following whole blocks replay one captured hipGraph of a
+    block (``callback``, if given, runs after every block instead of every
+    step).  Returns ``(x, loss_history)`` with iters + 1 per-image float64
+    losses (fewer after an early stop)."""
     torch = _torch()
     _check_adrt_data(b)
     dev.require_gpu()
     single = b.dim() == 3
     bb = (b.unsqueeze(0) if single else b).contiguous()
     n = int(bb.shape[-1])
-    x = torch.zeros(bb.shape[0], n, n, dtype=bb.dtype, device=bb.device) if x0 is None else (
-        (x0.unsqueeze(0) if x0.dim() == 2 else x0).to(bb.dtype).clone())
+    nb = int(bb.shape[0])
+    dt = bb.dtype
+    x = torch.zeros(nb, n, n, dtype=dt, device=bb.device) if x0 is None else (
+        (x0.unsqueeze(0) if x0.dim() == 2 else x0).to(dt).clone().contiguous())
     r = bb - adrt(x).detach()
-    s = _apply_adjoint(x, r)
+    s = _apply_adjoint(x, r).contiguous()
+    rr = _bdot(r, r)
+    del r
     p = s.clone()
-    gamma = _bdot(s, s)
-    history = [0.5 * _bdot(r, r)]
-    one = torch.ones_like(gamma)
-    zero = torch.zeros_like(gamma)
-    for k in range(iters):
-        if k % max(int(check_every), 1) == 0 and not bool((gamma > 0).any()):
-            break
-        pv = p.detach().requires_grad_(True)
+    history = [0.5 * rr.clone()]
+    # CG on A^T A x = A^T b with the device-side scalar logic of the PCG
+    # kernels (tol 0: every image runs until its gradient vanishes exactly):
+    # pcg_update forms p.(A^T A p) = ||A p||^2 in image space
+    st = _kernels.PcgState(nb, bb.device)
+    st.rz.copy_(_bdot(s, s))
+    st.active.copy_((st.rz > 0).int())
+    st.iters.zero_()
+    st.use_p.zero_()
+    ws = _kernels.cg_workspace(x)
+
+    def step():
+        pv = p.detach().requires_grad_(True)  # p's storage
         with torch.enable_grad():
             y = adrt(pv)
-        q = y.detach()
-        qq = _bdot(q, q)
-        alpha = torch.where(qq > 0, gamma / torch.where(qq > 0, qq, one), zero)
-        a = alpha.to(bb.dtype)
-        x = x + a[:, None, None] * p
-        r = r - a[:, None, None, None] * q
-        (s,) = torch.autograd.grad(y, pv, grad_outputs=r)  # A^T r
-        gamma_new = _bdot(s, s)
-        beta = torch.where(gamma > 0, gamma_new / torch.where(gamma > 0, gamma, one), zero)
-        p = s + beta.to(bb.dtype)[:, None, None] * p
-        gamma = gamma_new
-        history.append(0.5 * _bdot(r, r))
-        if callback is not None:
-            callback(x[0] if single else x)
+        (w,) = torch.autograd.grad(y, pv, grad_outputs=y.detach())  # A^T A p
+        # alpha = ||s||^2 / p.w, x += a p, s -= a w, ||s'||^2 (one pass)
+        _kernels.pcg_update(x, s, p, w.contiguous(), st, ws)
+        rr.sub_(st.alpha * st.rz)  # ||r||^2 - alpha ||s||^2
+        # beta = ||s'||^2 / ||s||^2, p = s' + beta p
+        _kernels.pcg_direction(p, s, None, st, ws)
+
+    blk = max(int(check_every), 1)
+    hist_blk = torch.zeros((blk, nb), dtype=torch.float64, device=bb.device)
+
+    def eager_steps(count):
+        for _ in range(count):
+            step()
+            history.append(0.5 * rr.clone())
+            if callback is not None:
+                callback(x[0] if single else x)
+
+    def vanished():
+        return not bool((st.active != 0).any())
+
+    done = 0
+    g = None
+    while done < iters:
+        if vanished():
+            break
+        if graph and g is None and done >= blk and iters - done >= blk:
+            torch.cuda.synchronize(bb.device)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                for j in range(blk):
+                    step()
+                    hist_blk[j].copy_(rr)
+        if g is not None and iters - done >= blk:
+            g.replay()
+            history.extend(0.5 * h for h in hist_blk.clone())
+            if callback is not None:
+                callback(x[0] if single else x)
+            done += blk
+        else:
+            k = min(blk, iters - done)
+            eager_steps(k)
+            done += k
+    # the last entry from the residual itself
+    r = bb - adrt(x).detach()
+    history[-1] = 0.5 * _bdot(r, r)
     return (x[0] if single else x), history

@@ -92,7 +92,7 @@ def parse():
                    help="--summarize a kernel trace per solver iteration over the last K "
                         "iterations (e.g. --op pcg: --match pcg_dir_kernel --iterations 5)")
     p.add_argument("--op", default="adrt",
-                   choices=["adrt", "bdrt", "iadrt", "unet", "pcg", "uneven"],
+                   choices=["adrt", "bdrt", "iadrt", "unet", "pcg", "uneven", "lsq"],
                    help="operation a --profile run calls (bdrt/iadrt on the forward output)")
     return p.parse_args()
 
@@ -135,8 +135,10 @@ def lsq_leg(iters=200):
     (the fused back-projection).  Each step applies A once and A^T once."""
     import torch
 
-    from adrt_amd import autograd, learned
+    from adrt_amd import _lib, autograd, batched, learned
 
+    # the autotuner's kernel configuration for this batch shape
+    tuning = [batched.tune(op, 512, 32)["choice"] for op in (_lib.OP_ADRT, _lib.OP_BDRT)]
     x_true = learned.phantom_batch(512, 4, 0, 32)
     a = learned._kernels.adrt(x_true)
     g = torch.Generator(device=a.device)
@@ -154,7 +156,9 @@ def lsq_leg(iters=200):
     # against an FP64 run
     x64, _ = autograd.lsq_cg(b.double(), iters=iters)
     return {
-        "workload": f"CGLS through autograd, 32 x 512x512 float32, {iters} steps, x0 = 0",
+        "workload": f"least squares through autograd (CG on the normal equations, hipGraph "
+                    f"blocks), 32 x 512x512 float32, {iters} steps, x0 = 0",
+        "tuned_adrt_bdrt": tuning,
         "seconds": round(secs, 4),
         "ms_per_step": round(secs / iters * 1e3, 3),
         "loss_first": float(hist[0].mean()), "loss_last": float(hist[-1].mean()),
@@ -547,6 +551,17 @@ def main():
             solver.net.forward(z, train=False)
         torch.cuda.synchronize()
         print(json.dumps({"op": "unet", "images": 1000, "n": 256}), flush=True)
+        return
+    if args.profile and args.op == "lsq":
+        # least squares through autograd: 32 x 512^2 float32, 48 steps (one
+        # eager block of 16, then two graph replays)
+        from adrt_amd import autograd, learned
+
+        x_true = learned.phantom_batch(512, 4, 0, 32)
+        b = learned._kernels.adrt(x_true)
+        autograd.lsq_cg(b, iters=48)
+        torch.cuda.synchronize()
+        print(json.dumps({"op": "lsq", "images": 32, "n": 512, "steps": 48}), flush=True)
         return
     if args.profile and args.op == "uneven":
         # the uneven-size pipeline of 1000 images (autotuned, captured):

@@ -144,6 +144,25 @@ class TestAutograd:
         assert np.all(losses[-1] < 1e-12 * losses[0])
         assert float((x - img).abs().max()) < 1e-6
 
+    @pytest.mark.parametrize("dt", ["float32", "float64"])
+    def test_lsq_cg_graph_replays_equal_eager(self, cuda, dt):
+        """The captured blocks of iterations (replayed 5 times, with the
+        host's checks and copies between replays) give exactly the eager
+        iterates and loss history."""
+        import torch
+        from adrt_amd import autograd as AG
+        from adrt_amd.phantom import phantoms
+
+        img = phantoms(2, 64, seed=6, dtype=dt, device=cuda)
+        b = AG.adrt(img).detach()
+        b = b + 0.01 * b.abs().max() * torch.randn(b.shape, dtype=b.dtype, device=cuda,
+                                                   generator=torch.Generator(device=cuda).manual_seed(1))
+        xg, hg = AG.lsq_cg(b, iters=100, check_every=16, graph=True)
+        xe, he = AG.lsq_cg(b, iters=100, check_every=16, graph=False)
+        assert len(hg) == len(he) == 101
+        assert torch.equal(xg, xe)
+        assert all(torch.equal(u, v) for u, v in zip(hg, he))
+
     def test_lsq_cg_float32_noisy(self, cuda):
         import torch
         from adrt_amd import autograd as AG
