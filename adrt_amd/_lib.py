@@ -48,6 +48,13 @@ class Batch(ctypes.Structure):
                 ("bdrt", ctypes.c_void_p), ("batch", ctypes.c_size_t), ("n", ctypes.c_size_t)]
 
 
+class Queue(ctypes.Structure):
+    """``adrt_hip_queue``: a planned persistent queue (adrt_hip_queue_plan)."""
+    _fields_ = [("state", ctypes.c_void_p), ("dtype", ctypes.c_int), ("entries", ctypes.c_int),
+                ("segments", ctypes.c_int), ("counters", ctypes.c_int),
+                ("lds_bytes", ctypes.c_int), ("waves", ctypes.c_int)]
+
+
 # name -> (restype, argtypes); mirrors include/adrt_hip.h
 SIGNATURES = {
     "adrt_hip_status_string": (ctypes.c_char_p, [_INT]),
@@ -68,6 +75,12 @@ SIGNATURES = {
     "adrt_hip_bdrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_bdrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
     "adrt_hip_adrt_bdrt_small": (_INT, [ctypes.POINTER(Batch), _INT, _INT, _INT, _VP]),
+    "adrt_hip_queue_workspace_bytes": (_SZ, [ctypes.POINTER(Batch), _INT, _INT]),
+    "adrt_hip_queue_state_bytes": (_SZ, [ctypes.POINTER(Batch), _INT]),
+    "adrt_hip_queue_plan": (_INT, [ctypes.POINTER(Batch), _INT, _INT, _INT, _INT, _VP, _SZ, _VP,
+                                   _SZ, ctypes.POINTER(Queue)]),
+    "adrt_hip_queue_run": (_INT, [ctypes.POINTER(Queue), _INT, _VP]),
+    "adrt_hip_queue_timeouts": (_INT, [ctypes.POINTER(Queue), ctypes.POINTER(_INT)]),
     "adrt_hip_iadrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_fmg_restriction": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_fmg_prolongation": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _SZ, _VP]),

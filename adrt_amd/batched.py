@@ -6,8 +6,13 @@ forward ADRTs and back-projections, and the workspaces.  ``run()`` enqueues
 all sides n <= 64 as one persistent launch of the LDS kernel (every stage of
 both transforms in LDS, ``adrt_hip_adrt_bdrt_small``) and, per larger side,
 one batched forward call and one batched back-projection call; these units
-go onto at most 4 streams, and ``capture()`` records the whole pipeline once
-as a hipGraph that ``replay()`` relaunches with no host work.
+go onto at most 4 streams.  Alternatively (``mode="queue"``, or chosen by
+the tuner under ``mode="auto"``) the whole batch runs as ONE persistent
+launch with a stage-ordered work queue (``QueuePlan``, sides up to 1024):
+forward pass 1 of every image, the small sides' LDS chunks, forward pass 2,
+then back-projection passes A and B, each entry waiting only for its own
+image's previous pass.  ``capture()`` records either pipeline once as a
+hipGraph that ``replay()`` relaunches with no host work.
 
 ``tune()`` is the autotuner: for one (op, n, batch) it times every kernel
 variant the library offers for that size (the one-launch LDS kernel for
@@ -126,6 +131,105 @@ def adrt_bdrt_small(stacks, *, adrt=True, bdrt=True):
     return [(f, k) for _, f, k in entries]
 
 
+QUEUE_MAX = 1024  # largest side of the persistent queue kernel
+QUEUE_BATCHES = 16  # entries per queue
+
+
+class QueuePlan:
+    """Forward ADRT and back-projection of several stacks of images of any
+    power-of-two sides n <= 1024 in ONE persistent launch
+    (``adrt_hip_queue_run``): a stage-ordered work queue of the fused passes
+    of every image of side >= 64 and the LDS chunks of the sides <= 32.
+
+    ``entries``: (images (B, n, n), adrt (B, 4, 2n-1, n), bdrt or None)
+    tensor triples, one dtype and device; the tensors must stay alive and in
+    place while the plan is used.  ``bundle``: fused items per queue entry
+    (1, 2, 4, 8); ``waves``: register budget of the kernel (8 or 4 waves per
+    SIMD; 0 = the dtype's default).  The plan owns its device workspace and
+    state; ``run()`` is one launch, capturable in a graph and replayable."""
+
+    def __init__(self, entries, *, bundle=8, waves=0):
+        torch = _torch()
+        dev.require_gpu()
+        if not entries or len(entries) > QUEUE_BATCHES:
+            raise ValueError(f"expected 1 to {QUEUE_BATCHES} stacks, got {len(entries)}")
+        self.dtype, self.device = entries[0][0].dtype, entries[0][0].device
+        self.code = _code(self.dtype)
+        for img, fwd, back in entries:
+            n = img.shape[-1]
+            if img.dim() != 3 or img.shape[1] != n or n > QUEUE_MAX or n & (n - 1):
+                raise ValueError(f"stack of shape {tuple(img.shape)}: expected (B, n, n), "
+                                 f"n a power of two <= {QUEUE_MAX}")
+            for x in (img, fwd, back):
+                if x is None:
+                    continue
+                if x.dtype != self.dtype or x.device != self.device or not x.is_contiguous():
+                    raise ValueError("stacks must be contiguous and share one dtype and device")
+            out = (img.shape[0], 4, 2 * n - 1, n)
+            if fwd is None or tuple(fwd.shape) != out or (back is not None
+                                                          and tuple(back.shape) != out):
+                raise ValueError(f"outputs of shape {out} expected for stack {tuple(img.shape)}")
+        self.entries = list(entries)
+        self.batches = _batch_array(self.entries)
+        lib = _lib.load()
+        nb = len(self.entries)
+        ws = int(lib.adrt_hip_queue_workspace_bytes(self.batches, nb, self.code))
+        sb = int(lib.adrt_hip_queue_state_bytes(self.batches, nb))
+        if sb == 0:
+            raise ValueError("invalid batches for the queue")
+        self.workspace = torch.empty(max(ws, 1), dtype=torch.uint8, device=self.device)
+        self.state = torch.empty(sb, dtype=torch.uint8, device=self.device)
+        self.plan = _lib.Queue()
+        with torch.cuda.device(self.device):
+            _lib.check(lib.adrt_hip_queue_plan(self.batches, nb, self.code, int(bundle),
+                                               int(waves), self.workspace.data_ptr(), ws,
+                                               self.state.data_ptr(), sb,
+                                               ctypes.byref(self.plan)), "queue_plan")
+        self.bundle, self.waves = int(bundle), int(self.plan.waves)
+
+    def run(self, stream=None, max_groups=0):
+        torch = _torch()
+        stream = stream or torch.cuda.current_stream(self.device)
+        _lib.check(_lib.load().adrt_hip_queue_run(ctypes.byref(self.plan), int(max_groups),
+                                                  ctypes.c_void_p(stream.cuda_stream)),
+                   "queue_run")
+
+    def timeouts(self):
+        """Dependency waits abandoned since the last call (0 unless the
+        device stalled for seconds; nonzero means invalid results)."""
+        c = ctypes.c_int()
+        with _torch().cuda.device(self.device):
+            _lib.check(_lib.load().adrt_hip_queue_timeouts(ctypes.byref(self.plan),
+                                                           ctypes.byref(c)), "queue_timeouts")
+        return int(c.value)
+
+
+def adrt_bdrt_queue(stacks, *, bdrt=True, bundle=8, waves=0):
+    """Forward ADRT (and back-projection) of up to 16 stacks (B, n, n) of
+    power-of-two sides n <= 1024 in one persistent launch.  Returns one
+    (adrt, bdrt) pair per stack (bdrt None when not asked), equal bit for bit
+    to ``adrt(x)`` and ``bdrt(adrt(x))``."""
+    torch = _torch()
+    dev.require_gpu()
+    entries = []
+    for x in stacks:
+        if x.dim() != 3:
+            raise ValueError(f"stack of shape {tuple(x.shape)}: expected (B, n, n)")
+        x = x.contiguous()
+        b, n = x.shape[0], x.shape[-1]
+        shape = (b, 4, 2 * n - 1, n)
+        f = torch.empty(shape, dtype=x.dtype, device=x.device)
+        k = torch.empty(shape, dtype=x.dtype, device=x.device) if bdrt else None
+        entries.append((x, f, k))
+    plan = QueuePlan(entries, bundle=bundle, waves=waves)
+    with torch.cuda.device(plan.device):
+        plan.run()
+    torch.cuda.current_stream(plan.device).synchronize()
+    if plan.timeouts():
+        raise RuntimeError("the queue kernel abandoned a dependency wait; results invalid")
+    return [(f, k) for _, f, k in entries]
+
+
 def candidates(op, n, batch, per_image_bytes):
     """Distinct (variant, group_bytes) configurations for (op, n, batch).
 
@@ -205,10 +309,30 @@ def tune(op, n, batch, *, dtype=None, device=None, reps=3):
             "choice": best, "timings": timings}
 
 
-class UnevenBatch:
-    """Forward ADRT + back-projection plan for images of sides ``sizes``."""
+def _time_ms(fn, stream, reps):
+    torch = _torch()
+    fn()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        fn()
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / reps
 
-    def __init__(self, sizes, *, dtype=None, device=None, autotune=True, reps=3):
+
+class UnevenBatch:
+    """Forward ADRT + back-projection plan for images of sides ``sizes``.
+
+    ``mode``: "streams" (one forward and one back-projection call per side,
+    on up to 4 streams), "queue" (one persistent launch of the stage-ordered
+    work queue, ``QueuePlan``; sides <= 1024) or "auto" (with ``autotune``,
+    whichever of the streams and the queue configurations measures fastest;
+    without, the streams)."""
+
+    QUEUE_CONFIGS = ((8, 8), (4, 8), (8, 4))  # (bundle, waves) tried for float32
+
+    def __init__(self, sizes, *, dtype=None, device=None, autotune=True, reps=3, mode="auto"):
         torch = _torch()
         dev.require_gpu()
         self.dtype = dtype or torch.float32
@@ -261,6 +385,45 @@ class UnevenBatch:
         self.lanes = self._assign_lanes()
         self.streams = [torch.cuda.Stream(self.device) for _ in self.lanes]
         self.graph = None
+        if mode not in ("auto", "streams", "queue"):
+            raise ValueError(f"unknown mode {mode!r}")
+        self.queue = None
+        self.queue_timings = []
+        self.streams_ms = None
+        queue_ok = max(self.classes) <= QUEUE_MAX and len(self.classes) <= QUEUE_BATCHES
+        if mode == "queue" and not queue_ok:
+            raise ValueError(f"the queue takes at most {QUEUE_BATCHES} sides, each <= {QUEUE_MAX}")
+        if mode == "queue" or (mode == "auto" and autotune and queue_ok):
+            self._tune_queue(reps if autotune else 0, keep_streams=mode == "auto")
+
+    def _queue_entries(self):
+        return [(self.images[n], self.fwd[n], self.back[n]) for n in reversed(self.classes)]
+
+    def _tune_queue(self, reps, keep_streams):
+        """Times the queue configurations (and the streams) and keeps the
+        fastest; with reps == 0 takes the default queue configuration."""
+        torch = _torch()
+        stream = torch.cuda.current_stream(self.device)
+        configs = self.QUEUE_CONFIGS if self.code == _lib.F32 else ((8, 4), (4, 4))
+        if reps == 0:
+            self.queue = QueuePlan(self._queue_entries())
+            return
+        best, best_ms = None, None
+        for bundle, waves in configs:
+            q = QueuePlan(self._queue_entries(), bundle=bundle, waves=waves)
+            ms = _time_ms(lambda: q.run(stream), stream, reps)
+            if q.timeouts():
+                raise RuntimeError("the queue kernel abandoned a dependency wait")
+            self.queue_timings.append({"bundle": bundle, "waves": waves, "ms": ms})
+            if best_ms is None or ms < best_ms:
+                best, best_ms = q, ms
+            else:
+                del q
+        if keep_streams:
+            self.streams_ms = _time_ms(self._enqueue_streams, stream, reps)
+            if self.streams_ms <= best_ms:
+                return
+        self.queue = best
 
     # The runtime maps streams onto 4 hardware queues (HIP's default), so
     # more than 4 streams share queues and serialise whatever lands
@@ -314,6 +477,12 @@ class UnevenBatch:
             self.images[n][k].copy_(img)
 
     def _enqueue(self):
+        if self.queue is not None:
+            self.queue.run(_torch().cuda.current_stream(self.device))
+        else:
+            self._enqueue_streams()
+
+    def _enqueue_streams(self):
         torch = _torch()
         main = torch.cuda.current_stream(self.device)
         if len(self.lanes) > len(self.streams):

@@ -20,7 +20,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libadrt_hip.so")
 SOURCES = ["capi.hip", "fwd.hip", "fused.hip", "small.hip", "bdrt.hip", "iadrt.hip", "fmg.hip",
-           "layout.hip", "nn.hip", "cg.hip", "interp.hip", "noalloc.hip", "dctp.hip"]
+           "layout.hip", "nn.hip", "cg.hip", "interp.hip", "noalloc.hip", "dctp.hip",
+           "queue.hip"]
 
 
 def hipcc():

@@ -198,7 +198,8 @@ def uneven_extra(steps, warmup):
     """1000 float32 images, sides drawn from UNEVEN_SIDES with probability
     proportional to 1/n (numpy default_rng(0)), pixels U[0, 1) (torch
     generator seed 0): forward ADRT + back-projection of all of them,
-    autotuned per side and captured as one hipGraph; K graph replays timed."""
+    autotuned per side and between the one-launch queue and the per-side
+    calls on streams, captured as one hipGraph; K graph replays timed."""
     import numpy as np
     import torch
 
@@ -222,6 +223,10 @@ def uneven_extra(steps, warmup):
     gbs = algo / (graph_ms * 1e-3) / 1e9
     choice = {f"{t['op']}{t['n']}": [t["choice"]["variant"], t["choice"]["group_bytes"] >> 20]
               for t in plan.tuning}
+    pipeline = ("queue (one persistent launch, bundle %d, %d waves)"
+                % (plan.queue.bundle, plan.queue.waves) if plan.queue is not None
+                else "streams (a forward and a back-projection call per side)")
+    timeouts = plan.queue.timeouts() if plan.queue is not None else 0
     # checks on the first image of every side: the replayed outputs against
     # the oracle (bytes; bdrt up to the sign of zero) and against an FP64
     # oracle run on the same image
@@ -246,9 +251,13 @@ def uneven_extra(steps, warmup):
         "algorithmic_bytes": algo,
         "GBps_algorithmic": round(gbs, 1),
         "frac": round(gbs / PEAK_GBS, 4),
-        "bit_exact_first_image_per_side": exact,
+        "bit_exact_first_image_per_side": exact and timeouts == 0,
         "fp64_error": err,
         "autotune_s": round(tune_s, 2),
+        "pipeline": pipeline,
+        "tuned_ms": {"streams": plan.streams_ms and round(plan.streams_ms, 3),
+                     "queue": [[q["bundle"], q["waves"], round(q["ms"], 3)]
+                               for q in plan.queue_timings]},
         "tuned_variant_groupMiB": choice,
     }
 

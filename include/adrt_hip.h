@@ -153,6 +153,56 @@ typedef struct adrt_hip_batch {
 int adrt_hip_adrt_bdrt_small(const adrt_hip_batch *batches, int count, int dtype,
                              int max_groups, void *stream);
 
+/* Forward ADRT and back-projection of several batches of any power-of-two
+ * sides n <= 1024 in ONE persistent launch with a stage-ordered work queue
+ * (no reference counterpart: the reference runs one call per array,
+ * adrt_cdefs_py.cpp:275-341 and 482-548 -> adrt_cdefs_adrt.hpp:111-211,
+ * adrt_cdefs_bdrt.hpp:120-187; this replaces that loop over arrays for an
+ * uneven-size batch).  Per entry the outputs are bit-identical to
+ * adrt_hip_adrt(images) and adrt_hip_bdrt(adrt).  Sides n >= 64 run the
+ * fused passes (forward pass 1, pass 2, back-projection pass A, pass B, each
+ * over every image before the next, an entry waiting only for its own
+ * image's previous pass); sides n <= 32 run whole in LDS.  For n >= 64 the
+ * adrt output is required (the back-projection reads it) and bdrt may be
+ * NULL; for n <= 32 either may be NULL, not both.  1 <= count <= 16, batch
+ * <= 2^20 per entry, entries of batch 0 skipped; outputs must not overlap
+ * inputs or each other.
+ *
+ *   adrt_hip_queue_workspace_bytes / _state_bytes: sizes of the caller's
+ *     device workspace (intermediates of every image of side >= 64) and
+ *     device state (queue header, segment table, completion counters);
+ *     0 when the batches are invalid.
+ *   adrt_hip_queue_plan: validates, builds the queue for these pointers and
+ *     copies it into `state` (synchronous hipMemcpy; no allocation); fills
+ *     *plan.  bundle = fused items per queue entry: 1, 2, 4 or 8 (0 = 8);
+ *     waves = register budget in waves per SIMD: 8 (float32 only) or 4
+ *     (0 = 8 for float32, 4 for float64).
+ *   adrt_hip_queue_run: the one launch, on `stream`; no allocation, no
+ *     memset (the kernel resets its counters on exit), capturable in a graph
+ *     and replayable.  Runs on one state must not overlap.  max_groups > 0
+ *     caps the grid; 0 = every work-group that fits at once.
+ *   adrt_hip_queue_timeouts: number of dependency waits abandoned after
+ *     their bound since the last call (then reset; synchronous).  Nonzero
+ *     means the results of those runs are invalid; it stays 0 unless the
+ *     device stalls for seconds. */
+typedef struct adrt_hip_queue {
+    void *state;
+    int dtype;
+    int entries;
+    int segments;
+    int counters;
+    int lds_bytes;
+    int waves;
+} adrt_hip_queue;
+
+size_t adrt_hip_queue_workspace_bytes(const adrt_hip_batch *batches, int count, int dtype);
+size_t adrt_hip_queue_state_bytes(const adrt_hip_batch *batches, int count);
+int adrt_hip_queue_plan(const adrt_hip_batch *batches, int count, int dtype, int bundle,
+                        int waves, void *workspace, size_t workspace_bytes, void *state,
+                        size_t state_bytes, adrt_hip_queue *plan);
+int adrt_hip_queue_run(const adrt_hip_queue *plan, int max_groups, void *stream);
+int adrt_hip_queue_timeouts(const adrt_hip_queue *plan, int *count);
+
 /* One back-projection stage.  Replaces adrt._adrt_cdefs.bdrt_step
  * (adrt_cdefs_py.cpp:550-619 -> bdrt_step, adrt_cdefs_bdrt.hpp:189-244). */
 int adrt_hip_bdrt_step(const void *in, void *out, int dtype, size_t batch, size_t n,

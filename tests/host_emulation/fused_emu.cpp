@@ -1,4 +1,4 @@
-// Host emulation of the fused forward kernels (adrt_amd/csrc/fused.hip).
+// Host emulation of the fused forward kernels (adrt_amd/csrc/fused_items.hpp).
 //
 // Compiles the exact per-lane load/store functions of Pass1/Pass2 for the
 // CPU and runs every launch block by block: for each wavefront task all 64
@@ -18,7 +18,7 @@
 #include <vector>
 
 #define ADRT_HOST_EMULATION 1
-#include "../../adrt_amd/csrc/fused.hip"
+#include "../../adrt_amd/csrc/fused_items.hpp"
 
 using namespace adrt_hip;
 using namespace adrt_hip::fused;

@@ -1,4 +1,4 @@
-// Host emulation of the one-launch LDS kernel (adrt_amd/csrc/small.hip).
+// Host emulation of the one-launch LDS kernel (adrt_amd/csrc/small_chunk.hpp).
 //
 // Compiles the exact phase functions, the class/chunk decoding and the slot
 // permutation for the CPU and runs the persistent schedule work-group by
@@ -18,7 +18,7 @@
 #include <vector>
 
 #define ADRT_HOST_EMULATION 1
-#include "../../adrt_amd/csrc/small.hip"
+#include "../../adrt_amd/csrc/small_chunk.hpp"
 
 using namespace adrt_hip;
 using namespace adrt_hip::small;

@@ -1,7 +1,7 @@
 """Host emulation of the fused forward kernels (CPU, no GPU).
 
 tests/host_emulation/fused_emu.cpp compiles the exact phase functions of
-adrt_amd/csrc/fused.hip for the host with AddressSanitizer and runs every
+adrt_amd/csrc/fused_items.hpp for the host with AddressSanitizer and runs every
 (block, thread) serially with exactly sized LDS and workspace buffers.  The
 result must be byte-identical to the oracle (signed zeros included), and any
 out-of-bounds access aborts the emulator.
