@@ -30,6 +30,9 @@ call and raise if the configuration changed under them.
 """
 
 import ctypes
+import json
+import os
+import tempfile
 
 from . import _device as dev
 from . import _lib
@@ -272,15 +275,77 @@ def _workspace(op, code, batch, n, device):
     return _torch().empty(max(int(nbytes), 1), dtype=_torch().uint8, device=device)
 
 
-def tune(op, n, batch, *, dtype=None, device=None, reps=3):
+class TuneCache:
+    """The autotuner's cache file: a JSON object of tuning results keyed by
+    device name, library ABI version, dtype and workload, so that a later
+    process selects the same configuration without timing again.
+
+    Path: ``path``, else $ADRT_AMD_TUNE_CACHE, else
+    ~/.cache/adrt_amd/tune.json.  An unreadable or corrupt file counts as
+    empty; writes go through a temporary file and an atomic rename."""
+
+    def __init__(self, path=None):
+        self.path = path or os.environ.get("ADRT_AMD_TUNE_CACHE") or os.path.join(
+            os.path.expanduser("~"), ".cache", "adrt_amd", "tune.json")
+
+    @staticmethod
+    def key(device, dtype, *parts):
+        torch = _torch()
+        name = torch.cuda.get_device_name(device)
+        abi = int(_lib.load().adrt_hip_abi_version())
+        return "|".join([name, f"abi{abi}", str(dtype).replace("torch.", "")]
+                        + [str(p) for p in parts])
+
+    def _read(self):
+        try:
+            with open(self.path) as f:
+                data = json.load(f)
+            return data if isinstance(data, dict) else {}
+        except (OSError, ValueError):
+            return {}
+
+    def get(self, key):
+        return self._read().get(key)
+
+    def put(self, key, value):
+        data = self._read()
+        data[key] = value
+        folder = os.path.dirname(os.path.abspath(self.path))
+        os.makedirs(folder, exist_ok=True)
+        fd, tmp = tempfile.mkstemp(dir=folder, prefix=".tune-", suffix=".json")
+        try:
+            with os.fdopen(fd, "w") as f:
+                json.dump(data, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.path)
+        except OSError:
+            if os.path.exists(tmp):
+                os.unlink(tmp)
+
+
+def tune(op, n, batch, *, dtype=None, device=None, reps=3, cache=None):
     """Times every candidate configuration of (op, n, batch) and selects the
     fastest in the library.  Returns a dict with the choice and all timings
-    (milliseconds per call, mean of ``reps`` after one warm-up)."""
+    (milliseconds per call, mean of ``reps`` after one warm-up).
+
+    ``cache``: a TuneCache (or None for none).  A cached choice for the
+    same device, library, dtype and (op, n, batch) is selected without
+    timing (the result then has ``"cached": True`` and no timings)."""
     torch = _torch()
     dev.require_gpu()
     dtype = dtype or torch.float32
     device = device or torch.device("cuda", torch.cuda.current_device())
     code = _code(dtype)
+    name = "adrt" if op == _lib.OP_ADRT else "bdrt"
+    key = TuneCache.key(device, dtype, name, n, batch) if cache is not None else None
+    if key is not None:
+        hit = cache.get(key)
+        if isinstance(hit, dict) and "variant" in hit and "group_bytes" in hit:
+            try:
+                set_config(op, n, int(hit["variant"]), int(hit["group_bytes"]))
+                return {"op": name, "n": n, "batch": batch, "choice": hit, "timings": [],
+                        "cached": True}
+            except (RuntimeError, ValueError):
+                pass  # a configuration this library rejects: tune again
     set_config(op, n, _lib.VARIANT_DEFAULT, 0)
     per = max(int(_lib.load().adrt_hip_workspace_bytes(op, code, 1, n)), 1)
     if op == _lib.OP_ADRT:
@@ -305,8 +370,10 @@ def tune(op, n, batch, *, dtype=None, device=None, reps=3):
         del ws
     best = min(timings, key=lambda t: t["ms"])
     set_config(op, n, best["variant"], best["group_bytes"])
-    return {"op": "adrt" if op == _lib.OP_ADRT else "bdrt", "n": n, "batch": batch,
-            "choice": best, "timings": timings}
+    if key is not None:
+        cache.put(key, best)
+    return {"op": name, "n": n, "batch": batch, "choice": best, "timings": timings,
+            "cached": False}
 
 
 def _time_ms(fn, stream, reps):
@@ -328,11 +395,13 @@ class UnevenBatch:
     on up to 4 streams), "queue" (one persistent launch of the stage-ordered
     work queue, ``QueuePlan``; sides <= 1024) or "auto" (with ``autotune``,
     whichever of the streams and the queue configurations measures fastest;
-    without, the streams)."""
+    without, the streams).  ``cache``: a TuneCache that keeps the per-side
+    choices and the pipeline choice across processes (None: no cache)."""
 
     QUEUE_CONFIGS = ((8, 8), (4, 8), (8, 4))  # (bundle, waves) tried for float32
 
-    def __init__(self, sizes, *, dtype=None, device=None, autotune=True, reps=3, mode="auto"):
+    def __init__(self, sizes, *, dtype=None, device=None, autotune=True, reps=3, mode="auto",
+                 cache=None):
         torch = _torch()
         dev.require_gpu()
         self.dtype = dtype or torch.float32
@@ -357,7 +426,7 @@ class UnevenBatch:
             for n in self.large:
                 for op in (_lib.OP_ADRT, _lib.OP_BDRT):
                     self.tuning.append(tune(op, n, count[n], dtype=self.dtype,
-                                            device=self.device, reps=reps))
+                                            device=self.device, reps=reps, cache=cache))
         self.images, self.fwd, self.back, self.ws_f, self.ws_b = {}, {}, {}, {}, {}
         for n in self.classes:
             b = count[n]
@@ -389,17 +458,18 @@ class UnevenBatch:
             raise ValueError(f"unknown mode {mode!r}")
         self.queue = None
         self.queue_timings = []
+        self.queue_cached = False
         self.streams_ms = None
         queue_ok = max(self.classes) <= QUEUE_MAX and len(self.classes) <= QUEUE_BATCHES
         if mode == "queue" and not queue_ok:
             raise ValueError(f"the queue takes at most {QUEUE_BATCHES} sides, each <= {QUEUE_MAX}")
         if mode == "queue" or (mode == "auto" and autotune and queue_ok):
-            self._tune_queue(reps if autotune else 0, keep_streams=mode == "auto")
+            self._tune_queue(reps if autotune else 0, keep_streams=mode == "auto", cache=cache)
 
     def _queue_entries(self):
         return [(self.images[n], self.fwd[n], self.back[n]) for n in reversed(self.classes)]
 
-    def _tune_queue(self, reps, keep_streams):
+    def _tune_queue(self, reps, keep_streams, cache=None):
         """Times the queue configurations (and the streams) and keeps the
         fastest; with reps == 0 takes the default queue configuration."""
         torch = _torch()
@@ -408,6 +478,18 @@ class UnevenBatch:
         if reps == 0:
             self.queue = QueuePlan(self._queue_entries())
             return
+        key = None
+        if cache is not None:
+            mix = ",".join(f"{n}x{self.count[n]}" for n in self.classes)
+            key = TuneCache.key(self.device, self.dtype, "uneven", mix,
+                                "auto" if keep_streams else "queue")
+            hit = cache.get(key)
+            if isinstance(hit, dict) and hit.get("pipeline") in ("streams", "queue"):
+                if hit["pipeline"] == "queue":
+                    self.queue = QueuePlan(self._queue_entries(), bundle=int(hit["bundle"]),
+                                           waves=int(hit["waves"]))
+                self.queue_cached = True
+                return
         best, best_ms = None, None
         for bundle, waves in configs:
             q = QueuePlan(self._queue_entries(), bundle=bundle, waves=waves)
@@ -421,9 +503,14 @@ class UnevenBatch:
                 del q
         if keep_streams:
             self.streams_ms = _time_ms(self._enqueue_streams, stream, reps)
-            if self.streams_ms <= best_ms:
-                return
-        self.queue = best
+        if keep_streams and self.streams_ms <= best_ms:
+            choice = {"pipeline": "streams", "ms": self.streams_ms}
+        else:
+            self.queue = best
+            choice = {"pipeline": "queue", "bundle": best.bundle, "waves": best.waves,
+                      "ms": best_ms}
+        if key is not None:
+            cache.put(key, choice)
 
     # The runtime maps streams onto 4 hardware queues (HIP's default), so
     # more than 4 streams share queues and serialise whatever lands

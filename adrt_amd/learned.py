@@ -784,7 +784,11 @@ def pcg(op, b, x0=None, prec=None, *, rtol=1e-4, maxiter=500, graph=False, compa
     queued: it reads the number of running images of the previous iteration
     (copied to pinned memory behind an event) while the GPU works on the
     current one, so the loop ends one masked (no-op) iteration late at most.
-    With ``graph`` one iteration is captured as a hipGraph and replayed.
+    With ``graph`` one iteration is captured as a hipGraph and replayed (off
+    by default: on MI355X, 1000 images at 256x256 to 1e-4, it measured
+    slower, 158 against 140 ms for plain CG and 67 against 62 ms learned,
+    since an iteration is milliseconds of transform work and every
+    compaction captures anew).
     When the images still running fall to ``compact`` times the current batch
     (and ``compact`` > 0), they are gathered into a smaller dense batch, so
     converged images cost nothing.  Returns (x, iterations per image,

@@ -209,7 +209,7 @@ def uneven_extra(steps, warmup):
     p = 1.0 / sides
     sizes = np.random.default_rng(0).choice(sides, 1000, p=p / p.sum()).tolist()
     t0 = time.perf_counter()
-    plan = batched.UnevenBatch(sizes)
+    plan = batched.UnevenBatch(sizes, cache=batched.TuneCache())
     tune_s = time.perf_counter() - t0
     g = torch.Generator(device=plan.device)
     g.manual_seed(0)
@@ -255,6 +255,8 @@ def uneven_extra(steps, warmup):
         "fp64_error": err,
         "autotune_s": round(tune_s, 2),
         "pipeline": pipeline,
+        "tune_cache": {"path": batched.TuneCache().path,
+                       "hit": plan.queue_cached and all(x["cached"] for x in plan.tuning)},
         "tuned_ms": {"streams": plan.streams_ms and round(plan.streams_ms, 3),
                      "queue": [[q["bundle"], q["waves"], round(q["ms"], 3)]
                                for q in plan.queue_timings]},

@@ -381,3 +381,54 @@ class TestSmallLaunch:
                                      torch.zeros((1, 4, 4), device=cuda, dtype=torch.float64)])
         with pytest.raises(ValueError, match="1 to 8"):
             batched.adrt_bdrt_small([torch.zeros((1, 8, 8), device=cuda)] * 9)
+
+
+def test_tune_cache_file(tmp_path):
+    """The cache round-trips entries, survives a corrupt file and writes
+    atomically (no temporary files left behind)."""
+    path = tmp_path / "sub" / "tune.json"
+    c = batched.TuneCache(str(path))
+    assert c.get("k") is None  # no file yet
+    c.put("k", {"variant": 1, "group_bytes": 64 * MIB, "ms": 0.5})
+    c.put("j", {"pipeline": "streams"})
+    assert batched.TuneCache(str(path)).get("k") == {"variant": 1, "group_bytes": 64 * MIB,
+                                                      "ms": 0.5}
+    assert batched.TuneCache(str(path)).get("j") == {"pipeline": "streams"}
+    assert sorted(p.name for p in path.parent.iterdir()) == ["tune.json"]
+    path.write_text("{not json")
+    assert c.get("k") is None
+    c.put("k", {"variant": 0, "group_bytes": 0})
+    assert c.get("k") == {"variant": 0, "group_bytes": 0}
+
+
+def test_tune_cache_path_from_environment(tmp_path, monkeypatch):
+    monkeypatch.setenv("ADRT_AMD_TUNE_CACHE", str(tmp_path / "t.json"))
+    assert batched.TuneCache().path == str(tmp_path / "t.json")
+    monkeypatch.delenv("ADRT_AMD_TUNE_CACHE")
+    assert batched.TuneCache().path.endswith("tune.json")
+
+
+@gpu
+class TestTuneCache:
+    def test_tune_reuses_cached_choice(self, cuda, reset_config, tmp_path):
+        c = batched.TuneCache(str(tmp_path / "tune.json"))
+        first = batched.tune(_lib.OP_BDRT, 256, 6, reps=1, cache=c)
+        assert not first["cached"] and first["timings"]
+        batched.set_config(_lib.OP_BDRT, 256, _lib.VARIANT_STREAMING, 0)
+        again = batched.tune(_lib.OP_BDRT, 256, 6, reps=1, cache=c)
+        assert again["cached"] and again["timings"] == []
+        want = (first["choice"]["variant"],
+                first["choice"]["group_bytes"] or 136 * MIB)
+        assert batched.get_config(_lib.OP_BDRT, 256) == want
+        other = batched.tune(_lib.OP_BDRT, 256, 7, reps=1, cache=c)  # another batch size
+        assert not other["cached"]
+
+    def test_uneven_plan_reuses_pipeline_choice(self, cuda, reset_config, tmp_path):
+        c = batched.TuneCache(str(tmp_path / "tune.json"))
+        sizes = [128, 128, 64, 32, 32, 8]
+        first = batched.UnevenBatch(sizes, reps=1, cache=c)
+        assert not first.queue_cached and first.queue_timings
+        second = batched.UnevenBatch(sizes, reps=1, cache=c)
+        assert second.queue_cached and not second.queue_timings
+        assert (second.queue is None) == (first.queue is None)
+        assert all(t["cached"] for t in second.tuning)
