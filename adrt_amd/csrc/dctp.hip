@@ -41,8 +41,8 @@ struct Args {
     const float2 *tw;  // [n/2] e^{-2 pi i t / n}, then [n] e^{-i pi m / 2n}
     const float *H;    // (n, n), kFiltered only
     long long rows;    // batch * n
-    int n, L;
     float s0, s1;      // s(0), s(m > 0)
+    float r0, r1;      // 1 / s(0), 1 / s(m > 0)
 };
 
 __device__ __forceinline__ int bitrev(int m, int L) {
@@ -75,13 +75,15 @@ __device__ __forceinline__ float2 twid(const float2 *tw, int t, bool inverse) {
 // those K radix-2 stages: stage s+u pairs t with t + h, h = 2^(K-1-u), at
 // position (t mod h) q + p.  Steps of 4 stages (radix 16) go first; one
 // shorter step finishes.
-template <int K>
-__device__ __forceinline__ void fft_step(float2 *buf, const float2 *tw, int L, int s,
-                                         bool inverse) {
+//
+// log2 n (L), the first stage (s) and the direction are compile-time, so
+// the index arithmetic folds into constants and shifts.
+template <int K, int L, int s, bool inverse>
+__device__ __forceinline__ void fft_step(float2 *buf, const float2 *tw) {
     constexpr int E = 1 << K;
-    const int lq = L - s - K;  // log2 q
-    const int q = 1 << lq;
-    const int gpr = 1 << (L - K);  // groups per row
+    constexpr int lq = L - s - K;  // log2 q
+    constexpr int q = 1 << lq;
+    constexpr int gpr = 1 << (L - K);  // groups per row
 #pragma unroll
     for (int j = threadIdx.x; j < kElems / E; j += kThreads) {
         const int r = j >> (L - K), jj = j & (gpr - 1);
@@ -108,23 +110,22 @@ __device__ __forceinline__ void fft_step(float2 *buf, const float2 *tw, int L, i
     __syncthreads();
 }
 
-__device__ __forceinline__ void fft_dif(float2 *buf, const float2 *tw, int L, bool inverse) {
-    int s = 0;
-    for (; s + 4 <= L; s += 4) fft_step<4>(buf, tw, L, s, inverse);
-    switch (L - s) {
-        case 3: fft_step<3>(buf, tw, L, s, inverse); break;
-        case 2: fft_step<2>(buf, tw, L, s, inverse); break;
-        case 1: fft_step<1>(buf, tw, L, s, inverse); break;
-        default: break;
+template <int L, bool inverse, int s = 0>
+__device__ __forceinline__ void fft_dif(float2 *buf, const float2 *tw) {
+    if constexpr (s + 4 <= L) {
+        fft_step<4, L, s, inverse>(buf, tw);
+        fft_dif<L, inverse, s + 4>(buf, tw);
+    } else if constexpr (s < L) {
+        fft_step<L - s, L, s, inverse>(buf, tw);
     }
 }
 
-template <int MODE>
+template <int MODE, int L>
 __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
     __shared__ __align__(16) float2 buf[kPadded];
     extern __shared__ __align__(16) float2 tws[];  // n/2 + n twiddles (dynamic LDS)
     float *stg = reinterpret_cast<float *>(buf);  // real staging view (2 * kPadded floats)
-    const int n = a.n, L = a.L, R = kElems >> L;
+    constexpr int n = 1 << L, R = kElems >> L;
     const long long g0 = (long long)blockIdx.x * R;  // first row of this work-group
     const int tid = threadIdx.x;
     for (int t = tid; t < (n >> 1) + n; t += kThreads) tws[t] = a.tw[t];
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
             buf[pad((r << L) + vp)] = make_float2(x, 0.f);
         }
         __syncthreads();
-        fft_dif(buf, tw, L, false);
+        fft_dif<L, false>(buf, tw);
         // X[m] = 2 Re(pt[m] V[m]) s(m), element e = (row r, frequency m)
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
@@ -182,7 +183,7 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
             const int m = e & (n - 1);
-            stg[e] = val[q] / (m == 0 ? a.s0 : a.s1);
+            stg[e] = val[q] * (m == 0 ? a.r0 : a.r1);
         }
         __syncthreads();
         float2 V[kPer];
@@ -199,8 +200,8 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) buf[pad(tid + q * kThreads)] = V[q];
         __syncthreads();
-        fft_dif(buf, tw, L, true);
-        const float inv_n = 1.f / float(n);
+        fft_dif<L, true>(buf, tw);
+        constexpr float inv_n = 1.f / float(n);
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
         stg[m * (R + 1) + r] = val[q];
     }
     __syncthreads();
-    const int lr = 12 - L;  // log2 R
+    constexpr int lr = 12 - L;  // log2 R
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const int e = tid + q * kThreads;
@@ -243,12 +244,28 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
     }
 }
 
-template <int MODE>
-static void launch(const Args &a, hipStream_t s) {
-    const long long R = kElems >> a.L;
+template <int MODE, int L>
+static void launch_l(const Args &a, hipStream_t s) {
+    constexpr long long R = kElems >> L;
     const unsigned grid = unsigned((a.rows + R - 1) / R);
-    const size_t lds = size_t((a.n >> 1) + a.n) * sizeof(float2);
-    hipLaunchKernelGGL(dct_rows_kernel<MODE>, dim3(grid), dim3(kThreads), lds, s, a);
+    const size_t lds = size_t(((1 << L) >> 1) + (1 << L)) * sizeof(float2);
+    hipLaunchKernelGGL((dct_rows_kernel<MODE, L>), dim3(grid), dim3(kThreads), lds, s, a);
+}
+
+template <int MODE>
+static void launch(const Args &a, int L, hipStream_t s) {
+    switch (L) {
+        case 1: launch_l<MODE, 1>(a, s); break;
+        case 2: launch_l<MODE, 2>(a, s); break;
+        case 3: launch_l<MODE, 3>(a, s); break;
+        case 4: launch_l<MODE, 4>(a, s); break;
+        case 5: launch_l<MODE, 5>(a, s); break;
+        case 6: launch_l<MODE, 6>(a, s); break;
+        case 7: launch_l<MODE, 7>(a, s); break;
+        case 8: launch_l<MODE, 8>(a, s); break;
+        case 9: launch_l<MODE, 9>(a, s); break;
+        default: launch_l<MODE, 10>(a, s); break;
+    }
 }
 
 }  // namespace dctp
@@ -289,23 +306,24 @@ extern "C" int adrt_hip_dct_filter(const float *x, float *y, const float *table,
     a.tw = reinterpret_cast<const float2 *>(table);
     a.H = H;
     a.rows = (long long)(batch * n);
-    a.n = int(n);
-    a.L = num_iters(n);
+    const int L = num_iters(n);
     a.s0 = float(std::sqrt(1.0 / (4.0 * double(n))));
     a.s1 = float(std::sqrt(1.0 / (2.0 * double(n))));
+    a.r0 = float(std::sqrt(4.0 * double(n)));
+    a.r1 = float(std::sqrt(2.0 * double(n)));
     const hipStream_t s = static_cast<hipStream_t>(stream);
     a.in = x;
     a.out = tmp;
-    dctp::launch<dctp::kForward>(a, s);
+    dctp::launch<dctp::kForward>(a, L, s);
     a.in = tmp;
     a.out = y;
     if (mode == 0) {
-        dctp::launch<dctp::kForward>(a, s);
+        dctp::launch<dctp::kForward>(a, L, s);
         return launch_status();
     }
-    dctp::launch<dctp::kFiltered>(a, s);
+    dctp::launch<dctp::kFiltered>(a, L, s);
     a.in = y;
     a.out = y;
-    dctp::launch<dctp::kInverse>(a, s);
+    dctp::launch<dctp::kInverse>(a, L, s);
     return launch_status();
 }

@@ -342,6 +342,192 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_lds_kernel(ConvArgs a, int
 }
 
 // ---------------------------------------------------------------------------
+// The same convolution for the wide layers: W a multiple of 64, H a multiple
+// of 4, Cin a multiple of 16, 16 output channels per work-group (Cout 16 or
+// 32).  Tiles, channel chunks (32 at a time, then a last one of 16),
+// the K order k = tap * ccv + c, the MFMA operand layout and the epilogue
+// are those of conv3x3_lds_kernel, so the results are bit-identical to it.
+// What changes is where the index work happens.  A chunk of ccv channels is
+// 9 (ccv = 32) or 4.5 (ccv = 16) MFMA K steps whose taps and channels are
+// known at compile time, so every A fragment is an LDS read at a constant
+// offset from one per-lane base (for ccv = 16 a step covers two taps, one
+// per half of the lane groups: a select between two constants).  The
+// chunk's weight fragments are loaded once, together with the tile fill,
+// and the fill's pixel decomposition uses the constant tile width.  The
+// narrow layers issued about 50 VALU instructions per MFMA in the general
+// kernel and were bound by them.  It serves the 16-channel layers: with 32
+// output channels per work-group the chunk's weights need 72 more registers
+// and the kernel drops to 2 waves per SIMD, and two 16-channel work-groups
+// per tile (the mapping below) measured no faster than the general kernel
+// for the 32-channel layers (MI355X, 1000 images at 128^2).
+// ---------------------------------------------------------------------------
+constexpr int kWideRow = kTWmax + 2;  // LDS tile row (pixels)
+
+__device__ __forceinline__ constexpr int tap_offset(int t) {
+    return ((t / 3) * kWideRow + (t % 3)) * kCCP;  // tile element offset of tap t
+}
+
+template <int NT, int CCV>
+__device__ __forceinline__ void wide_chunk(const ConvArgs &a, bf16 (&tile)[kTH + 2][kWideRow][kCCP],
+                                           const bf16 *xb, const bf16 *x2b, const bf16 *wl, int c0,
+                                           int y0, int x0, int wave, int g, int r,
+                                           f32x4 (&acc)[kMT][NT]) {
+    constexpr int NV = CCV / 8;                  // 16-byte vectors per pixel
+    constexpr int LG = CCV == 32 ? 2 : 1;
+    constexpr int NFILL = (kTH + 2) * kWideRow * NV;
+    constexpr int FILL = (NFILL + kWgThreads - 1) / kWgThreads;
+    constexpr int S = CCV == 32 ? 9 : 5;          // K steps of the chunk
+    const int tid = int(threadIdx.x);
+    const int K = 9 * a.Cin;
+
+    // weight fragments of the chunk: step s, lane group g -> k = 32 s + 8 g,
+    // i.e. tap s / channels 8g (ccv = 32) or tap 2s + g/2 / channels 8(g%2)
+    // (ccv = 16; tap 9 of the last step is past the chunk: zero)
+    bf16x8 B[S][NT];
+    const bf16 *wc = wl + c0 + (CCV == 32 ? 8 * g : (g >> 1) * a.Cin + 8 * (g & 1));
+    const int wstep = CCV == 32 ? a.Cin : 2 * a.Cin;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const bool ok = CCV == 32 || s < S - 1 || g < 2;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            B[s][nt] = keep_or_zero(load8(wc + (ok ? s * wstep : 0) + (size_t)nt * 16 * K), ok);
+    }
+
+    __syncthreads();  // the previous chunk's readers are done
+    // tile fill: (kTH + 2) x kWideRow pixels with a zero halo; a thread's
+    // vectors all hold channels c0 + 8v (v = tid % NV) of their pixels
+    const int v = tid & (NV - 1);
+    const int ch = c0 + 8 * v;
+    const bool up = ch >= a.c_split;
+    bf16x8 fv[FILL];
+#pragma unroll
+    for (int j = 0; j < FILL; ++j) {
+        const int i = tid + j * kWgThreads;
+        const int pix = (i < NFILL ? i : 0) >> LG;
+        const int py = pix / kWideRow, px = pix - py * kWideRow;
+        const int gy = y0 + py - 1, gx = x0 + px - 1;
+        const bool ok = unsigned(gy) < unsigned(a.H) && unsigned(gx) < unsigned(a.W);
+        const bf16 *src;
+        if (up)
+            src = x2b + (ok ? (gy >> 1) * (a.W >> 1) + (gx >> 1) : 0) * a.x2_stride + (ch - a.c_split);
+        else
+            src = xb + (ok ? gy * a.W + gx : 0) * a.x_stride + ch;
+        fv[j] = keep_or_zero(load8(src), ok);
+    }
+#pragma unroll
+    for (int j = 0; j < FILL; ++j) {
+        const int i = tid + j * kWgThreads;
+        if (i < NFILL)
+            *reinterpret_cast<bf16x8 *>(&tile[0][0][0] + (i >> LG) * kCCP + 8 * v) = fv[j];
+    }
+    __syncthreads();
+
+    // A fragments: lane (r, g) of tile mt reads pixel 16 mt + r of row
+    // `wave` shifted by its tap, channels of its group
+    const bf16 *tl = &tile[0][0][0] + (wave * kWideRow + r) * kCCP + (CCV == 32 ? 8 * g : 8 * (g & 1));
+    const bool hi = g >= 2;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        int off;
+        if (CCV == 32) {
+            off = tap_offset(s);
+        } else {
+            const int t0 = 2 * s, t1 = 2 * s + 1 < 9 ? 2 * s + 1 : 8;
+            off = hi ? tap_offset(t1) : tap_offset(t0);
+        }
+        bf16x8 A[kMT];
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt) {
+            A[mt] = *reinterpret_cast<const bf16x8 *>(tl + off + 16 * mt * kCCP);
+            if (CCV == 16 && s == S - 1) A[mt] = keep_or_zero(A[mt], !hi);
+        }
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(A[mt], B[s][nt], acc[mt][nt]);
+    }
+}
+
+// Work-group -> (tile, 16-channel group): the cg groups of one tile go to
+// work-groups 8 apart, which run on one XCD (work-groups are dealt to the 8
+// XCDs round-robin) at about the same time, so the tile is read from HBM
+// once and from that XCD's L2 for the other groups.
+template <int MODE>
+__global__ void __launch_bounds__(kWgThreads) conv3x3_wide_kernel(ConvArgs a, int tiles) {
+    constexpr int NT = 1;
+    static_assert(MODE == kReluBias || MODE == kLinear, "bf16 outputs");
+    __shared__ __align__(16) bf16 tile[kTH + 2][kWideRow][kCCP];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int g = lane >> 4, r = lane & 15;
+    const int cg = a.Cout / 16;
+    const int bid = int(blockIdx.x);
+    const int full = tiles / 8 * 8;
+    int tt, grp;
+    if (bid < full * cg) {
+        const int rem = bid % (8 * cg);
+        tt = bid / (8 * cg) * 8 + rem % 8;
+        grp = rem / 8;
+    } else {
+        tt = full + (bid - full * cg) / cg;
+        grp = (bid - full * cg) % cg;
+    }
+    const int co0 = grp * 16;
+    const int tiles_x = a.W / kTWmax, tiles_y = a.H / kTH;
+    const int tx = tt % tiles_x;
+    tt /= tiles_x;
+    const int ty = tt % tiles_y;
+    const long long b = tt / tiles_y;
+    const int x0 = tx * kTWmax, y0 = ty * kTH;
+    const long long img = b * a.H * a.W;  // first pixel of the image
+    const bf16 *xb = a.x + img * a.x_stride;
+    const bf16 *x2b = a.x2 ? a.x2 + b * (a.H >> 1) * (a.W >> 1) * a.x2_stride : a.x;
+    const bf16 *wl = a.w + (size_t)(co0 + r) * (9 * a.Cin);
+
+    f32x4 acc[kMT][NT];
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    int c0 = 0;
+    for (; c0 + 32 <= a.Cin; c0 += 32)
+        wide_chunk<NT, 32>(a, tile, xb, x2b, wl, c0, y0, x0, wave, g, r, acc);
+    if (c0 < a.Cin) wide_chunk<NT, 16>(a, tile, xb, x2b, wl, c0, y0, x0, wave, g, r, acc);
+
+    // bf16 outputs through LDS, one 16-channel slice at a time (as in
+    // conv3x3_lds_kernel)
+    const long long prow = img + (long long)(y0 + wave) * a.W + x0;
+    bf16 *ob = &tile[0][0][0] + wave * (kTWmax * 16);
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int co = co0 + 16 * nt + r;
+        const float bco = MODE == kReluBias ? a.bias[co] : 0.f;
+        __syncthreads();  // the tile's (or the previous slice's) readers are done
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int px = 16 * mt + 4 * g + j;
+                float val = acc[mt][nt][j];
+                if (MODE == kReluBias) {
+                    val += bco;
+                    val = val > 0.f ? val : 0.f;
+                }
+                ob[px * 16 + r] = bf16(val);
+            }
+        }
+        __syncthreads();
+        for (int e = lane; e < 32 * kMT; e += kWaveLanes) {
+            const int px = e >> 1, h = e & 1;
+            const bf16x8 val = *reinterpret_cast<const bf16x8 *>(&ob[px * 16 + 8 * h]);
+            *reinterpret_cast<bf16x8 *>(&a.y[(prow + px) * a.y_stride + co0 + 16 * nt + 8 * h]) = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: partial[part][co][t][ci] = sum over the part's pixels of
 // dZ[p][co] * X[p + tap t][ci]; partial_bias[part][co] = sum dZ[p][co].
 // Wave = one (16 co) x (16 ci) tile for all 9 taps over one pixel part.
@@ -952,6 +1138,18 @@ inline unsigned blocks_for(long long work, int block = kWgThreads) {
 
 template <int NT>
 static void launch_conv_lds(const ConvArgs &a, int mode, hipStream_t s) {
+    if (a.W % kTWmax == 0 && a.Cin % 16 == 0 && a.Cout == 16 &&
+        (mode == kReluBias || mode == kLinear)) {
+        const long long tiles = (a.P / a.W / kTH) * (a.W / kTWmax);
+        const unsigned grid = unsigned(tiles * (a.Cout / 16));
+        if (mode == kReluBias)
+            hipLaunchKernelGGL((conv3x3_wide_kernel<kReluBias>), dim3(grid), dim3(kWgThreads), 0, s,
+                               a, int(tiles));
+        else
+            hipLaunchKernelGGL((conv3x3_wide_kernel<kLinear>), dim3(grid), dim3(kWgThreads), 0, s,
+                               a, int(tiles));
+        return;
+    }
     const int TW = a.W < kTWmax ? a.W : kTWmax;
     // floor(p / (TW + 2)) == (p * recip) >> 16 for every tile pixel index p
     // (checked for TW in {16, 32, 48, 64})
