@@ -13,8 +13,9 @@
 // decimation-in-frequency FFTs (natural order in, bit-reversed out), so no
 // permutation pass is needed: the post-processing reads V[bitrev(m)].
 //
-// A work-group holds R = 4096 / n whole rows (4096 complex values, 32 KB of
-// LDS) and the twiddles (n <= 1024).  The 2-D transform is three row passes over the batch:
+// A work-group holds R = 8192 / n whole rows, two real rows per complex FFT
+// row (4096 complex values, 32 KB of LDS), and the twiddles (n <= 1024).
+// The 2-D transform is three row passes over the batch:
 //   pass 0 (kForward):  rows of x -> DCT-II -> written transposed
 //   pass 1 (kFiltered): rows (now columns of x) -> DCT-II -> * H -> DCT-III
 //                       -> written transposed (back to x's orientation)
@@ -58,6 +59,10 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 // 16 elements in the last radix-16 step) fall on distinct banks.
 __device__ __forceinline__ int pad(int e) { return e + (e >> 4); }
 constexpr int kPadded = kElems + kElems / 16;
+// LDS buffer (complex elements): the padded FFT rows, or as reals the
+// transposed staging of 2 kElems / n rows of n (+1 per column)
+constexpr int kStaged = (2 * kElems + kMaxN + 1) / 2;
+constexpr int kBuf = kPadded > kStaged ? kPadded : kStaged;
 
 __device__ __forceinline__ float2 twid(const float2 *tw, int t, bool inverse) {
     float2 w = tw[t];
@@ -120,40 +125,56 @@ __device__ __forceinline__ void fft_dif(float2 *buf, const float2 *tw) {
     }
 }
 
+// Two real rows per complex FFT: real rows 2c and 2c + 1 of the work-group
+// (reordered as above: v_a, v_b) travel as one complex row z = v_a + i v_b
+// of length n, so a work-group holds 2 * kElems / n real rows.
+//   forward: Z = DFT(z); with Zc = conj(Z[(n - m) mod n]),
+//            V_a = (Z[m] + Zc) / 2 and V_b = (Z[m] - Zc) / (2i), so
+//            X_a[m] = Re(pt[m] (Z[m] + Zc)) s(m), X_b[m] = Im(pt[m] (Z[m] - Zc)) s(m);
+//   inverse: V_a, V_b are Hermitian (V[n - m] = conj(V[m]) for real Y), so
+//            IDFT(V_a + i V_b) = v_a + i v_b: real parts row 2c, imaginary 2c + 1.
 template <int MODE, int L>
 __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
-    __shared__ __align__(16) float2 buf[kPadded];
+    __shared__ __align__(16) float2 buf[kBuf];
     extern __shared__ __align__(16) float2 tws[];  // n/2 + n twiddles (dynamic LDS)
-    float *stg = reinterpret_cast<float *>(buf);  // real staging view (2 * kPadded floats)
-    constexpr int n = 1 << L, R = kElems >> L;
+    float *stg = reinterpret_cast<float *>(buf);  // real staging view (2 * kBuf floats)
+    constexpr int n = 1 << L;
+    constexpr int R = 2 * (kElems >> L);  // real rows per work-group
     const long long g0 = (long long)blockIdx.x * R;  // first row of this work-group
     const int tid = threadIdx.x;
     for (int t = tid; t < (n >> 1) + n; t += kThreads) tws[t] = a.tw[t];
     const float2 *tw = tws;
     const float2 *pt = tws + (n >> 1);
-    float val[kPer];
+    // element q of the thread: complex row c = e >> L, column e & (n - 1),
+    // carrying real rows 2c (va) and 2c + 1 (vb)
+    float va[kPer], vb[kPer];
 
     if (MODE != kInverse) {
-        // load rows, Makhoul reorder into the complex buffer
+        // load two rows per complex row, Makhoul reorder into the buffer
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const int r = e >> L, k = e & (n - 1);
-            const long long gr = g0 + r;
-            const float x = gr < a.rows ? a.in[gr * n + k] : 0.f;
+            const int c = e >> L, k = e & (n - 1);
+            const long long ra = g0 + 2 * c;
+            const float xa = ra < a.rows ? a.in[ra * n + k] : 0.f;
+            const float xb = ra + 1 < a.rows ? a.in[(ra + 1) * n + k] : 0.f;
             const int vp = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-            buf[pad((r << L) + vp)] = make_float2(x, 0.f);
+            buf[pad((c << L) + vp)] = make_float2(xa, xb);
         }
         __syncthreads();
         fft_dif<L, false>(buf, tw);
-        // X[m] = 2 Re(pt[m] V[m]) s(m), element e = (row r, frequency m)
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const int r = e >> L, m = e & (n - 1);
-            const float2 V = buf[pad((r << L) + bitrev(m, L))];
+            const int c = e >> L, m = e & (n - 1);
+            const float2 Zm = buf[pad((c << L) + bitrev(m, L))];
+            const float2 Zn = buf[pad((c << L) + bitrev((n - m) & (n - 1), L))];
+            const float ax = Zm.x + Zn.x, ay = Zm.y - Zn.y;  // Z[m] + conj(Z[n-m])
+            const float bx = Zm.x - Zn.x, by = Zm.y + Zn.y;  // Z[m] - conj(Z[n-m])
             const float2 p = pt[m];
-            val[q] = 2.f * (p.x * V.x - p.y * V.y) * (m == 0 ? a.s0 : a.s1);
+            const float sm = m == 0 ? a.s0 : a.s1;
+            va[q] = (p.x * ax - p.y * ay) * sm;
+            vb[q] = (p.x * by + p.y * bx) * sm;
         }
         if (MODE == kFiltered) {
             // * H[j][m] with j = the row's index in its image (the frequency
@@ -161,9 +182,10 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
 #pragma unroll
             for (int q = 0; q < kPer; ++q) {
                 const int e = tid + q * kThreads;
-                const int r = e >> L, m = e & (n - 1);
-                const int j = int((g0 + r) & (n - 1));
-                val[q] *= a.H[(size_t)j * n + m];
+                const int c = e >> L, m = e & (n - 1);
+                const int ja = int((g0 + 2 * c) & (n - 1));
+                va[q] *= a.H[(size_t)ja * n + m];
+                vb[q] *= a.H[(size_t)(ja + 1) * n + m];
             }
         }
         __syncthreads();  // every read of buf is done
@@ -171,43 +193,53 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const long long gr = g0 + (e >> L);
-            val[q] = gr < a.rows ? a.in[gr * n + (e & (n - 1))] : 0.f;
+            const long long ra = g0 + 2 * (e >> L);
+            const int k = e & (n - 1);
+            va[q] = ra < a.rows ? a.in[ra * n + k] : 0.f;
+            vb[q] = ra + 1 < a.rows ? a.in[(ra + 1) * n + k] : 0.f;
         }
     }
 
     if (MODE == kInverse) __syncthreads();  // the twiddles are in LDS
     if (MODE != kForward) {
-        // inverse: Y = X / s in natural order, then V[m] = (1/2) conj(pt[m]) (Y[m] - i Y[n-m])
+        // Y = X / s in natural order (real row r at stg[r * n]), then
+        // V[m] = (1/2) conj(pt[m]) (Y[m] - i Y[n-m]) per row and W = V_a + i V_b
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const int m = e & (n - 1);
-            stg[e] = val[q] * (m == 0 ? a.r0 : a.r1);
+            const int c = e >> L, m = e & (n - 1);
+            const float rm = m == 0 ? a.r0 : a.r1;
+            stg[((2 * c) << L) + m] = va[q] * rm;
+            stg[((2 * c + 1) << L) + m] = vb[q] * rm;
         }
         __syncthreads();
-        float2 V[kPer];
+        float2 W[kPer];
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const int r = e >> L, m = e & (n - 1);
-            const float ym = stg[e];
-            const float ynm = m == 0 ? 0.f : stg[(r << L) + n - m];
-            const float2 c = pt[m];  // e^{-i pi m / 2n}; its conjugate is applied
-            V[q] = make_float2(0.5f * (c.x * ym - c.y * ynm), 0.5f * (-c.y * ym - c.x * ynm));
+            const int c = e >> L, m = e & (n - 1);
+            const float2 cc = pt[m];  // e^{-i pi m / 2n}; its conjugate is applied
+            const int ia = (2 * c) << L, ib = ia + n;
+            const float ya = stg[ia + m], yna = m == 0 ? 0.f : stg[ia + n - m];
+            const float yb = stg[ib + m], ynb = m == 0 ? 0.f : stg[ib + n - m];
+            const float vax = 0.5f * (cc.x * ya - cc.y * yna), vay = 0.5f * (-cc.y * ya - cc.x * yna);
+            const float vbx = 0.5f * (cc.x * yb - cc.y * ynb), vby = 0.5f * (-cc.y * yb - cc.x * ynb);
+            W[q] = make_float2(vax - vby, vay + vbx);
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) buf[pad(tid + q * kThreads)] = V[q];
+        for (int q = 0; q < kPer; ++q) buf[pad(tid + q * kThreads)] = W[q];
         __syncthreads();
         fft_dif<L, true>(buf, tw);
         constexpr float inv_n = 1.f / float(n);
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const int r = e >> L, k = e & (n - 1);
+            const int c = e >> L, k = e & (n - 1);
             const int j = (k & 1) ? n - 1 - (k >> 1) : (k >> 1);
-            val[q] = buf[pad((r << L) + bitrev(j, L))].x * inv_n;
+            const float2 z = buf[pad((c << L) + bitrev(j, L))];
+            va[q] = z.x * inv_n;
+            vb[q] = z.y * inv_n;
         }
     }
 
@@ -216,24 +248,27 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int e = tid + q * kThreads;
-            const long long gr = g0 + (e >> L);
-            if (gr < a.rows) a.out[gr * n + (e & (n - 1))] = val[q];
+            const long long ra = g0 + 2 * (e >> L);
+            const int k = e & (n - 1);
+            if (ra < a.rows) a.out[ra * n + k] = va[q];
+            if (ra + 1 < a.rows) a.out[(ra + 1) * n + k] = vb[q];
         }
         return;
     }
-    // transposed store: element (r, m) -> out[b][m][i], i = row index in its
-    // image; staged as stg[m * (R + 1) + r] so each m leaves as a run of rows
+    // transposed store: element (row r, m) -> out[b][m][i], i = row index in
+    // its image; staged as stg[m * (R + 1) + r] so each m leaves as a run of rows
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const int e = tid + q * kThreads;
-        const int r = e >> L, m = e & (n - 1);
-        stg[m * (R + 1) + r] = val[q];
+        const int c = e >> L, m = e & (n - 1);
+        stg[m * (R + 1) + 2 * c] = va[q];
+        stg[m * (R + 1) + 2 * c + 1] = vb[q];
     }
     __syncthreads();
-    constexpr int lr = 12 - L;  // log2 R
+    constexpr int lr = 13 - L;  // log2 R
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
+    for (int q = 0; q < 2 * kPer; ++q) {
         const int e = tid + q * kThreads;
         const int m = e >> lr, r = e & (R - 1);
         const long long gr = g0 + r;
@@ -246,7 +281,7 @@ __global__ void __launch_bounds__(kThreads) dct_rows_kernel(Args a) {
 
 template <int MODE, int L>
 static void launch_l(const Args &a, hipStream_t s) {
-    constexpr long long R = kElems >> L;
+    constexpr long long R = 2 * (kElems >> L);  // real rows per work-group
     const unsigned grid = unsigned((a.rows + R - 1) / R);
     const size_t lds = size_t(((1 << L) >> 1) + (1 << L)) * sizeof(float2);
     hipLaunchKernelGGL((dct_rows_kernel<MODE, L>), dim3(grid), dim3(kThreads), lds, s, a);
