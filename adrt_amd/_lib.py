@@ -121,6 +121,8 @@ SIGNATURES = {
     "adrt_hip_nn_loss_grad": (_INT, [_VP, _VP, _VP, _INT, _INT, _INT, ctypes.c_float, _VP]),
     "adrt_hip_nn_conv_in": (_INT, [_VP, ctypes.c_float, _VP, _INT, _VP, _VP, _INT, _INT, _INT,
                                    _INT, _VP]),
+    "adrt_hip_nn_encoder0": (_INT, [_VP, ctypes.c_float, _VP, _INT, _VP, _VP, _VP, _VP, _INT, _VP,
+                                    _INT, _INT, _INT, _INT, _VP]),
     "adrt_hip_nn_conv_out": (_INT, [_VP, _INT, _VP, _VP, _VP, _VP, _INT, _INT, _INT, _VP]),
     "adrt_hip_nn_adam": (_INT, [_VP, _VP, _VP, _VP, _VP, _SZ, ctypes.c_float, ctypes.c_float,
                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,

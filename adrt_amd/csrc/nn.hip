@@ -367,6 +367,58 @@ __device__ __forceinline__ constexpr int tap_offset(int t) {
     return ((t / 3) * kWideRow + (t % 3)) * kCCP;  // tile element offset of tap t
 }
 
+// Weight fragments of a chunk: step s, lane group g -> k = 32 s + 8 g, i.e.
+// tap s / channels 8g (ccv = 32) or tap 2s + g/2 / channels 8(g%2) (ccv = 16;
+// tap 9 of the last step is past the chunk: zero).  wl: the lane's weight
+// row (output channel co0 + r), Cin: the layer's input channels.
+template <int NT, int CCV>
+__device__ __forceinline__ void wide_load_b(const bf16 *wl, int Cin, int c0, int g,
+                                            bf16x8 (&B)[CCV == 32 ? 9 : 5][NT]) {
+    constexpr int S = CCV == 32 ? 9 : 5;
+    const int K = 9 * Cin;
+    const bf16 *wc = wl + c0 + (CCV == 32 ? 8 * g : (g >> 1) * Cin + 8 * (g & 1));
+    const int wstep = CCV == 32 ? Cin : 2 * Cin;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const bool ok = CCV == 32 || s < S - 1 || g < 2;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            B[s][nt] = keep_or_zero(load8(wc + (ok ? s * wstep : 0) + (size_t)nt * 16 * K), ok);
+    }
+}
+
+// The chunk's MFMAs on the filled tile: lane (r, g) of tile mt reads pixel
+// 16 mt + r of row `wave` shifted by its tap, channels of its group, at a
+// constant offset from the lane's base.
+template <int NT, int CCV>
+__device__ __forceinline__ void wide_mfma(const bf16 (&tile)[kTH + 2][kWideRow][kCCP], int wave,
+                                          int g, int r, const bf16x8 (&B)[CCV == 32 ? 9 : 5][NT],
+                                          f32x4 (&acc)[kMT][NT]) {
+    constexpr int S = CCV == 32 ? 9 : 5;
+    const bf16 *tl = &tile[0][0][0] + (wave * kWideRow + r) * kCCP + (CCV == 32 ? 8 * g : 8 * (g & 1));
+    const bool hi = g >= 2;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        int off;
+        if (CCV == 32) {
+            off = tap_offset(s);
+        } else {
+            const int t0 = 2 * s, t1 = 2 * s + 1 < 9 ? 2 * s + 1 : 8;
+            off = hi ? tap_offset(t1) : tap_offset(t0);
+        }
+        bf16x8 A[kMT];
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt) {
+            A[mt] = *reinterpret_cast<const bf16x8 *>(tl + off + 16 * mt * kCCP);
+            if (CCV == 16 && s == S - 1) A[mt] = keep_or_zero(A[mt], !hi);
+        }
+#pragma unroll
+        for (int mt = 0; mt < kMT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(A[mt], B[s][nt], acc[mt][nt]);
+    }
+}
+
 template <int NT, int CCV>
 __device__ __forceinline__ void wide_chunk(const ConvArgs &a, bf16 (&tile)[kTH + 2][kWideRow][kCCP],
                                            const bf16 *xb, const bf16 *x2b, const bf16 *wl, int c0,
@@ -376,23 +428,10 @@ __device__ __forceinline__ void wide_chunk(const ConvArgs &a, bf16 (&tile)[kTH +
     constexpr int LG = CCV == 32 ? 2 : 1;
     constexpr int NFILL = (kTH + 2) * kWideRow * NV;
     constexpr int FILL = (NFILL + kWgThreads - 1) / kWgThreads;
-    constexpr int S = CCV == 32 ? 9 : 5;          // K steps of the chunk
     const int tid = int(threadIdx.x);
-    const int K = 9 * a.Cin;
 
-    // weight fragments of the chunk: step s, lane group g -> k = 32 s + 8 g,
-    // i.e. tap s / channels 8g (ccv = 32) or tap 2s + g/2 / channels 8(g%2)
-    // (ccv = 16; tap 9 of the last step is past the chunk: zero)
-    bf16x8 B[S][NT];
-    const bf16 *wc = wl + c0 + (CCV == 32 ? 8 * g : (g >> 1) * a.Cin + 8 * (g & 1));
-    const int wstep = CCV == 32 ? a.Cin : 2 * a.Cin;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        const bool ok = CCV == 32 || s < S - 1 || g < 2;
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-            B[s][nt] = keep_or_zero(load8(wc + (ok ? s * wstep : 0) + (size_t)nt * 16 * K), ok);
-    }
+    bf16x8 B[CCV == 32 ? 9 : 5][NT];
+    wide_load_b<NT, CCV>(wl, a.Cin, c0, g, B);
 
     __syncthreads();  // the previous chunk's readers are done
     // tile fill: (kTH + 2) x kWideRow pixels with a zero halo; a thread's
@@ -422,30 +461,61 @@ __device__ __forceinline__ void wide_chunk(const ConvArgs &a, bf16 (&tile)[kTH +
             *reinterpret_cast<bf16x8 *>(&tile[0][0][0] + (i >> LG) * kCCP + 8 * v) = fv[j];
     }
     __syncthreads();
+    wide_mfma<NT, CCV>(tile, wave, g, r, B, acc);
+}
 
-    // A fragments: lane (r, g) of tile mt reads pixel 16 mt + r of row
-    // `wave` shifted by its tap, channels of its group
-    const bf16 *tl = &tile[0][0][0] + (wave * kWideRow + r) * kCCP + (CCV == 32 ? 8 * g : 8 * (g & 1));
-    const bool hi = g >= 2;
+// bf16 outputs (16 channels co0 .. co0 + 15 of the work-group's 4 x 64
+// pixels, one row per wave) through LDS, as in conv3x3_lds_kernel; then, if
+// `pooled`, the 2 x 32 averages of their 2 x 2 blocks as pool_kernel forms
+// them.  The tile's readers must be done (a barrier comes first).
+template <int MODE>
+__device__ __forceinline__ void wide_store(bf16 (&tile)[kTH + 2][kWideRow][kCCP],
+                                           const f32x4 (&acc)[kMT][1], const float *bias, int co0,
+                                           bf16 *y, int y_stride, long long prow, int wave, int lane,
+                                           int g, int r, bf16 *pooled, int pooled_stride,
+                                           long long pq0, int Wc) {
+    bf16 *ob0 = &tile[0][0][0];
+    bf16 *ob = ob0 + wave * (kTWmax * 16);
+    const int co = co0 + r;
+    const float bco = MODE == kReluBias ? bias[co] : 0.f;
+    __syncthreads();  // the tile's readers are done
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        int off;
-        if (CCV == 32) {
-            off = tap_offset(s);
-        } else {
-            const int t0 = 2 * s, t1 = 2 * s + 1 < 9 ? 2 * s + 1 : 8;
-            off = hi ? tap_offset(t1) : tap_offset(t0);
+    for (int mt = 0; mt < kMT; ++mt) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int px = 16 * mt + 4 * g + j;
+            float val = acc[mt][0][j];
+            if (MODE == kReluBias) {
+                val += bco;
+                val = val > 0.f ? val : 0.f;
+            }
+            ob[px * 16 + r] = bf16(val);
         }
-        bf16x8 A[kMT];
+    }
+    __syncthreads();
+    for (int e = lane; e < 32 * kMT; e += kWaveLanes) {
+        const int px = e >> 1, h = e & 1;
+        const bf16x8 val = *reinterpret_cast<const bf16x8 *>(&ob[px * 16 + 8 * h]);
+        *reinterpret_cast<bf16x8 *>(&y[(prow + px) * y_stride + co0 + 8 * h]) = val;
+    }
+    if (pooled) {
+        // pooled row pr (of 2), column pc (of 32), channels 8h .. 8h + 7:
+        // 0.25 * ((x[2pr][2pc] + x[2pr][2pc+1]) + (x[2pr+1][2pc] + x[2pr+1][2pc+1]))
+        const int e = int(threadIdx.x);
+        if (e < 2 * (kTWmax / 2) * 2) {
+            const int h = e & 1, pc = (e >> 1) & (kTWmax / 2 - 1), pr = e >> 6;
+            const bf16 *r0 = ob0 + (2 * pr) * (kTWmax * 16), *r1 = r0 + kTWmax * 16;
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(r0 + (2 * pc) * 16 + 8 * h);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(r0 + (2 * pc + 1) * 16 + 8 * h);
+            const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(r1 + (2 * pc) * 16 + 8 * h);
+            const bf16x8 a3 = *reinterpret_cast<const bf16x8 *>(r1 + (2 * pc + 1) * 16 + 8 * h);
+            bf16x8 o;
 #pragma unroll
-        for (int mt = 0; mt < kMT; ++mt) {
-            A[mt] = *reinterpret_cast<const bf16x8 *>(tl + off + 16 * mt * kCCP);
-            if (CCV == 16 && s == S - 1) A[mt] = keep_or_zero(A[mt], !hi);
+            for (int j = 0; j < 8; ++j)
+                o[j] = bf16(0.25f * ((float(a0[j]) + float(a1[j])) + (float(a2[j]) + float(a3[j]))));
+            *reinterpret_cast<bf16x8 *>(&pooled[(pq0 + (long long)pr * Wc + pc) * pooled_stride +
+                                                co0 + 8 * h]) = o;
         }
-#pragma unroll
-        for (int mt = 0; mt < kMT; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(A[mt], B[s][nt], acc[mt][nt]);
     }
 }
 
@@ -496,35 +566,125 @@ __global__ void __launch_bounds__(kWgThreads) conv3x3_wide_kernel(ConvArgs a, in
         wide_chunk<NT, 32>(a, tile, xb, x2b, wl, c0, y0, x0, wave, g, r, acc);
     if (c0 < a.Cin) wide_chunk<NT, 16>(a, tile, xb, x2b, wl, c0, y0, x0, wave, g, r, acc);
 
-    // bf16 outputs through LDS, one 16-channel slice at a time (as in
-    // conv3x3_lds_kernel)
     const long long prow = img + (long long)(y0 + wave) * a.W + x0;
-    bf16 *ob = &tile[0][0][0] + wave * (kTWmax * 16);
+    wide_store<MODE>(tile, acc, a.bias, co0, a.y, a.y_stride, prow, wave, lane, g, r, nullptr, 0,
+                     0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// The encoder's full-resolution level in one kernel (inference): e0a (the
+// first layer, 1 -> 16 channels from the float32 input), e0b (16 -> 16) and
+// the 2 x 2 average pooling.  A work-group's e0b tile is 4 x 64 pixels; the
+// 6 x 66 e0a values it reads are computed into the LDS tile from an 8 x 68
+// tile of bf16(scale * z), with conv_in_kernel's arithmetic (one fp32 fma
+// chain per output in tap order, + bias, ReLU, bf16), so e0a never goes
+// through HBM.  e0b is conv3x3_wide_kernel's (same MFMA operands), written
+// as the skip connection, and its 2 x 2 averages (pool_kernel's arithmetic)
+// go to the pooled output.  Results equal conv_in + conv3x3 + pool bit for
+// bit.
+// ---------------------------------------------------------------------------
+struct Enc0Args {
+    const float *z;
+    float scale;
+    const bf16 *w1;  // e0a weights [16][9][cinw1] (channel 0 real)
+    int cinw1;
+    const float *b1;
+    const bf16 *w2;  // e0b weights [16][9][16]
+    const float *b2;
+    bf16 *skip;
+    int skip_stride;
+    bf16 *pooled;
+    int pooled_stride;
+    int H, W;
+};
+
+__global__ void __launch_bounds__(kWgThreads) encoder0_kernel(Enc0Args e) {
+    __shared__ __align__(16) bf16 tile[kTH + 2][kWideRow][kCCP];
+    __shared__ float zt[kTH + 4][kTWmax + 4];
+    __shared__ __align__(16) float w1l[9][16];
+    const int tid = int(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = lane >> 4, r = lane & 15;
+    const int tiles_x = e.W / kTWmax, tiles_y = e.H / kTH;
+    int tt = int(blockIdx.x);
+    const int tx = tt % tiles_x;
+    tt /= tiles_x;
+    const int ty = tt % tiles_y;
+    const long long b = tt / tiles_y;
+    const int x0 = tx * kTWmax, y0 = ty * kTH;
+    const long long img = b * e.H * e.W;
+
+    bf16x8 B[5][1];
+    wide_load_b<1, 16>(e.w2 + (size_t)r * (9 * 16), 16, 0, g, B);
+    // bf16(scale * z) around the tile (zero outside the image), e0a weights
+    for (int i = tid; i < (kTH + 4) * (kTWmax + 4); i += kWgThreads) {
+        const int rr = i / (kTWmax + 4), cc = i - rr * (kTWmax + 4);
+        const int gy = y0 + rr - 2, gx = x0 + cc - 2;
+        const bool ok = unsigned(gy) < unsigned(e.H) && unsigned(gx) < unsigned(e.W);
+        zt[rr][cc] = ok ? float(bf16(e.scale * e.z[img + (long long)gy * e.W + gx])) : 0.f;
+    }
+    if (tid < 144) {
+        const int t = tid >> 4, co = tid & 15;
+        w1l[t][co] = float(e.w1[((size_t)co * 9 + t) * e.cinw1]);
+    }
+    __syncthreads();
+    // e0a at the tile's (kTH + 2) x kWideRow pixels: vector i = (pixel, 8
+    // channels 8v); pixels outside the image are e0b's zero padding
+    {
+        constexpr int NFILL = (kTH + 2) * kWideRow * 2;
+        constexpr int FILL = (NFILL + kWgThreads - 1) / kWgThreads;
+        const int v = tid & 1;
+        float bv[8];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int co = co0 + 16 * nt + r;
-        const float bco = MODE == kReluBias ? a.bias[co] : 0.f;
-        __syncthreads();  // the tile's (or the previous slice's) readers are done
+        for (int c = 0; c < 8; ++c) bv[c] = e.b1[8 * v + c];
+#pragma unroll 1
+        for (int j = 0; j < FILL; ++j) {
+            const int i = tid + j * kWgThreads;
+            if (i >= NFILL) break;
+            const int pix = i >> 1;
+            const int py = pix / kWideRow, px = pix - py * kWideRow;
+            const int gy = y0 + py - 1, gx = x0 + px - 1;
+            const bool ok = unsigned(gy) < unsigned(e.H) && unsigned(gx) < unsigned(e.W);
+            float acc[8];
 #pragma unroll
-        for (int mt = 0; mt < kMT; ++mt) {
+            for (int c = 0; c < 8; ++c) acc[c] = 0.f;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int px = 16 * mt + 4 * g + j;
-                float val = acc[mt][nt][j];
-                if (MODE == kReluBias) {
-                    val += bco;
-                    val = val > 0.f ? val : 0.f;
+            for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    const float xv = zt[py + dy][px + dx];
+                    const float4 wa = *reinterpret_cast<const float4 *>(&w1l[dy * 3 + dx][8 * v]);
+                    const float4 wb = *reinterpret_cast<const float4 *>(&w1l[dy * 3 + dx][8 * v + 4]);
+                    acc[0] = __builtin_fmaf(wa.x, xv, acc[0]);
+                    acc[1] = __builtin_fmaf(wa.y, xv, acc[1]);
+                    acc[2] = __builtin_fmaf(wa.z, xv, acc[2]);
+                    acc[3] = __builtin_fmaf(wa.w, xv, acc[3]);
+                    acc[4] = __builtin_fmaf(wb.x, xv, acc[4]);
+                    acc[5] = __builtin_fmaf(wb.y, xv, acc[5]);
+                    acc[6] = __builtin_fmaf(wb.z, xv, acc[6]);
+                    acc[7] = __builtin_fmaf(wb.w, xv, acc[7]);
                 }
-                ob[px * 16 + r] = bf16(val);
             }
-        }
-        __syncthreads();
-        for (int e = lane; e < 32 * kMT; e += kWaveLanes) {
-            const int px = e >> 1, h = e & 1;
-            const bf16x8 val = *reinterpret_cast<const bf16x8 *>(&ob[px * 16 + 8 * h]);
-            *reinterpret_cast<bf16x8 *>(&a.y[(prow + px) * a.y_stride + co0 + 16 * nt + 8 * h]) = val;
+            bf16x8 o;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float a = acc[c] + bv[c];
+                o[c] = bf16(ok && a > 0.f ? a : 0.f);
+            }
+            *reinterpret_cast<bf16x8 *>(&tile[0][0][0] + pix * kCCP + 8 * v) = o;
         }
     }
+    __syncthreads();
+    f32x4 acc[kMT][1];
+#pragma unroll
+    for (int mt = 0; mt < kMT; ++mt) acc[mt][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+    wide_mfma<1, 16>(tile, wave, g, r, B, acc);
+    const long long prow = img + (long long)(y0 + wave) * e.W + x0;
+    const int Wc = e.W >> 1;
+    const long long pq0 = (b * (e.H >> 1) + (y0 >> 1)) * Wc + (x0 >> 1);
+    wide_store<kReluBias>(tile, acc, e.b2, 0, e.skip, e.skip_stride, prow, wave, lane, g, r,
+                          e.pooled, e.pooled_stride, pq0, Wc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1435,6 +1595,36 @@ int adrt_hip_nn_conv_in(const float *z, float scale, const void *w, int cinw, co
     hipLaunchKernelGGL(conv_in_kernel, dim3(unsigned(tiles)), dim3(kWgThreads), 0,
                        static_cast<hipStream_t>(stream), z, scale, static_cast<const bf16 *>(w),
                        cinw, bias, static_cast<bf16 *>(y), y_stride, h, w_);
+    return launch_status();
+}
+
+int adrt_hip_nn_encoder0(const float *z, float scale, const void *w1, int cinw1, const float *b1,
+                         const void *w2, const float *b2, void *skip, int skip_stride,
+                         void *pooled, int pooled_stride, int batch, int h, int w_,
+                         void *stream) {
+    if (batch < 1 || h < kTH || w_ < kTWmax || h % kTH != 0 || w_ % kTWmax != 0 || cinw1 < 1 ||
+        skip_stride < 16 || skip_stride % 8 != 0 || pooled_stride < 16 || pooled_stride % 8 != 0)
+        return ADRT_HIP_ERR_ARG;
+    if (!z || !w1 || !b1 || !w2 || !b2 || !skip || !pooled || !aligned16(w2) || !aligned16(skip) ||
+        !aligned16(pooled))
+        return ADRT_HIP_ERR_NULL;
+    Enc0Args e;
+    e.z = z;
+    e.scale = scale;
+    e.w1 = static_cast<const bf16 *>(w1);
+    e.cinw1 = cinw1;
+    e.b1 = b1;
+    e.w2 = static_cast<const bf16 *>(w2);
+    e.b2 = b2;
+    e.skip = static_cast<bf16 *>(skip);
+    e.skip_stride = skip_stride;
+    e.pooled = static_cast<bf16 *>(pooled);
+    e.pooled_stride = pooled_stride;
+    e.H = h;
+    e.W = w_;
+    const long long tiles = (long long)batch * (h / kTH) * (w_ / kTWmax);
+    hipLaunchKernelGGL(encoder0_kernel, dim3(unsigned(tiles)), dim3(kWgThreads), 0,
+                       static_cast<hipStream_t>(stream), e);
     return launch_status();
 }
 

@@ -528,9 +528,10 @@ class UNet:
 
     def _infer(self, z, f, B, n):
         """Inference form of forward(): the skip connections keep their own
-        buffers and each decoder convolution reads the coarse features
-        through the upsampling in place, so no concatenation buffer is
-        written.  Same arithmetic, bit for bit."""
+        buffers, each decoder convolution reads the coarse features through
+        the upsampling in place, so no concatenation buffer is written, and
+        the full-resolution encoder level (e0a, e0b, pooling) is one kernel
+        that keeps e0a on chip.  Same arithmetic, bit for bit."""
         torch = _torch()
         lib = _lib.load()
         n1, n2, n3 = n // 2, n // 4, n // 8
@@ -538,9 +539,16 @@ class UNet:
         for key, hw, c in (("s0", n, 16), ("s1", n1, 32), ("s2", n2, 64)):
             if key not in f:
                 f[key] = torch.empty((B, hw, hw, c), dtype=torch.bfloat16, device=self.device)
-        self._conv_in(z, f["t0"], B, n)
-        self._conv(f["t0"], 16, "e0b", f["s0"], 16, B, n, 0)
-        _check(lib.adrt_hip_nn_pool(_p(f["s0"]), 16, _p(f["p0"]), 16, B, n, n, 16, s), "pool")
+        if n % 64 == 0:  # e0a, e0b and the pooling in one kernel
+            _, _, cin, _ = self.offsets["e0a"]
+            _check(lib.adrt_hip_nn_encoder0(_p(z), 1.0, _p(self.weight("e0a")), cin,
+                                            _p(self.bias("e0a")), _p(self.weight("e0b")),
+                                            _p(self.bias("e0b")), _p(f["s0"]), 16, _p(f["p0"]), 16,
+                                            B, n, n, s), "encoder0")
+        else:
+            self._conv_in(z, f["t0"], B, n)
+            self._conv(f["t0"], 16, "e0b", f["s0"], 16, B, n, 0)
+            _check(lib.adrt_hip_nn_pool(_p(f["s0"]), 16, _p(f["p0"]), 16, B, n, n, 16, s), "pool")
         self._conv(f["p0"], 16, "e1a", f["t1"], 32, B, n1, 0)
         self._conv(f["t1"], 32, "e1b", f["s1"], 32, B, n1, 0)
         _check(lib.adrt_hip_nn_pool(_p(f["s1"]), 32, _p(f["p1"]), 32, B, n1, n1, 32, s), "pool")

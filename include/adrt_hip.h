@@ -454,6 +454,15 @@ int adrt_hip_nn_conv_in(const float *z, float scale, const void *w, int cinw, co
                         void *y, int y_stride, int batch, int h, int w_, void *stream);
 int adrt_hip_nn_conv_out(const void *x, int x_stride, const void *w, const float *bias, float *y32,
                          const float *resid, int batch, int h, int w_, void *stream);
+/* The encoder's full-resolution level for inference in one launch: e0a
+ * (adrt_hip_nn_conv_in's arithmetic, never stored), e0b (16 -> 16, ReLU,
+ * stored as the skip connection) and its 2x2 average pooling (stored as the
+ * next level's input); bit-identical to conv_in + conv3x3 + pool.  h a
+ * multiple of 4, w_ a multiple of 64. */
+int adrt_hip_nn_encoder0(const float *z, float scale, const void *w1, int cinw1, const float *b1,
+                         const void *w2, const float *b2, void *skip, int skip_stride,
+                         void *pooled, int pooled_stride, int batch, int h, int w_,
+                         void *stream);
 
 /* 16-channel bf16 gradient: channel 0 = scale * (pred - target), others 0. */
 int adrt_hip_nn_loss_grad(const float *pred, const float *target, void *dz, int batch, int h,
