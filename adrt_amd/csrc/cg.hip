@@ -42,6 +42,25 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return s;
 }
 
+// Order of a chunk's sum: element i of [lo, hi) goes to thread
+// ((i - lo) / 4) mod kThreads and to that thread's accumulator (i - lo) mod 4;
+// the thread returns (s0 + s1) + (s2 + s3), and block_sum adds the threads.
+// The order depends on the chunk alone, so an image's sum does not depend on
+// the batch, and dot_kernel and update_kernel (r.r) sum alike.  A thread's
+// elements are independent, so its loads are all in flight together.
+constexpr int kSub = 4;
+
+// Four consecutive elements as one vector (16 bytes of float, 32 of double):
+// used when m is a multiple of 4, so that every chunk starts and ends on a
+// whole vector and a lane's four elements are one coalesced access.
+template <typename T>
+using vec4 = T __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ bool vec_ok(const T *p) {
+    return (reinterpret_cast<unsigned long long>(p) & (sizeof(vec4<T>) - 1)) == 0;
+}
+
 // Chunk c of image b: elements [c * chunk, min((c+1) * chunk, m)).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -53,8 +72,24 @@ __global__ void __launch_bounds__(kThreads)
     const size_t lo = size_t(c) * chunk;
     const size_t hi = lo + chunk < m ? lo + chunk : m;
     const T *ub = u + b * m, *vb = v + b * m;
-    double s = 0.0;
-    for (size_t i = lo + threadIdx.x; i < hi; i += kThreads) s += double(ub[i]) * double(vb[i]);
+    double sj[kSub] = {0.0, 0.0, 0.0, 0.0};
+    if (m % kSub == 0 && vec_ok(ub) && vec_ok(vb)) {
+#pragma unroll 4
+        for (size_t base = lo + kSub * threadIdx.x; base < hi; base += kSub * kThreads) {
+            const vec4<T> a = *reinterpret_cast<const vec4<T> *>(ub + base);
+            const vec4<T> c = *reinterpret_cast<const vec4<T> *>(vb + base);
+#pragma unroll
+            for (int j = 0; j < kSub; ++j) sj[j] += double(a[j]) * double(c[j]);
+        }
+    } else {
+#pragma unroll 4
+        for (size_t base = lo + kSub * threadIdx.x; base < hi; base += kSub * kThreads) {
+#pragma unroll
+            for (int j = 0; j < kSub; ++j)
+                if (base + j < hi) sj[j] += double(ub[base + j]) * double(vb[base + j]);
+        }
+    }
+    double s = (sj[0] + sj[1]) + (sj[2] + sj[3]);
     s = block_sum(s, red);
     if (threadIdx.x == 0) partial[b * chunks + c] = s;
 }
@@ -72,15 +107,43 @@ __global__ void __launch_bounds__(kThreads)
     const T a = T(alpha[b]);
     T *xb = x + b * m, *rb = r + b * m;
     const T *pb = p + b * m, *qb = q + b * m;
-    double s = 0.0;
-    for (size_t i = lo + threadIdx.x; i < hi; i += kThreads) {
-        const T ap = a * pb[i];
-        const T aq = a * qb[i];
-        xb[i] = xb[i] + ap;
-        const T rn = rb[i] - aq;
-        rb[i] = rn;
-        s += double(rn) * double(rn);
+    double sj[kSub] = {0.0, 0.0, 0.0, 0.0};
+    if (m % kSub == 0 && vec_ok(pb) && vec_ok(qb) && vec_ok(xb) && vec_ok(rb)) {
+#pragma unroll 2
+        for (size_t base = lo + kSub * threadIdx.x; base < hi; base += kSub * kThreads) {
+            const vec4<T> p4 = *reinterpret_cast<const vec4<T> *>(pb + base);
+            const vec4<T> q4 = *reinterpret_cast<const vec4<T> *>(qb + base);
+            vec4<T> x4 = *reinterpret_cast<const vec4<T> *>(xb + base);
+            vec4<T> r4 = *reinterpret_cast<const vec4<T> *>(rb + base);
+#pragma unroll
+            for (int j = 0; j < kSub; ++j) {
+                const T ap = a * p4[j];
+                const T aq = a * q4[j];
+                x4[j] = x4[j] + ap;
+                r4[j] = r4[j] - aq;
+                sj[j] += double(r4[j]) * double(r4[j]);
+            }
+            *reinterpret_cast<vec4<T> *>(xb + base) = x4;
+            *reinterpret_cast<vec4<T> *>(rb + base) = r4;
+        }
+    } else {
+#pragma unroll 4
+        for (size_t base = lo + kSub * threadIdx.x; base < hi; base += kSub * kThreads) {
+#pragma unroll
+            for (int j = 0; j < kSub; ++j) {
+                const size_t i = base + j;
+                if (i < hi) {
+                    const T ap = a * pb[i];
+                    const T aq = a * qb[i];
+                    xb[i] = xb[i] + ap;
+                    const T rn = rb[i] - aq;
+                    rb[i] = rn;
+                    sj[j] += double(rn) * double(rn);
+                }
+            }
+        }
     }
+    double s = (sj[0] + sj[1]) + (sj[2] + sj[3]);
     s = block_sum(s, red);
     if (threadIdx.x == 0) partial[b * chunks + c] = s;
 }
@@ -95,6 +158,7 @@ __global__ void __launch_bounds__(kThreads)
     const T c = T(beta[b]);
     T *pb = p + b * m;
     const T *zb = z + b * m;
+#pragma unroll 4
     for (size_t i = lo + threadIdx.x; i < hi; i += kThreads) {
         const T cp = c * pb[i];
         pb[i] = zb[i] + cp;
@@ -226,6 +290,7 @@ __global__ void __launch_bounds__(kThreads)
     const T c = T(s.beta[b]);
     T *pb = p + b * m;
     const T *zb = (z != nullptr && s.use_p[b]) ? z + b * m : r + b * m;
+#pragma unroll 4
     for (size_t i = lo + threadIdx.x; i < hi; i += kThreads) {
         const T cp = c * pb[i];
         pb[i] = zb[i] + cp;

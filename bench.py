@@ -91,6 +91,10 @@ def parse():
     p.add_argument("--iterations", type=int, default=0,
                    help="--summarize a kernel trace per solver iteration over the last K "
                         "iterations (e.g. --op pcg: --match pcg_dir_kernel --iterations 5)")
+    p.add_argument("--sequence", type=int, default=0,
+                   help="--summarize a kernel trace as the K dispatches that start at the "
+                        "last one whose name contains --match (e.g. --op unet: --match "
+                        "encoder0 --sequence 16, one row per layer)")
     p.add_argument("--op", default="adrt",
                    choices=["adrt", "bdrt", "iadrt", "unet", "pcg", "uneven", "lsq"],
                    help="operation a --profile run calls (bdrt/iadrt on the forward output)")
@@ -422,6 +426,24 @@ def summarize_iterations(dirs, marker, last):
     return "\n".join(lines)
 
 
+def summarize_sequence(dirs, marker, count):
+    """The ``count`` dispatches of a traced run (rocprofv3 --kernel-trace csv)
+    that start at the last one whose name contains ``marker``, in order,
+    with their durations."""
+    rows = sorted(_csv_rows(dirs, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+    starts = [i for i, r in enumerate(rows) if marker in r["Kernel_Name"]]
+    if not starts:
+        return f"no dispatch of {marker}"
+    lines = ["| # | kernel | µs |", "|---|---|---|"]
+    total = 0.0
+    for k, r in enumerate(rows[starts[-1]:starts[-1] + count]):
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        total += d
+        lines.append(f"| {k} | `{_short(r['Kernel_Name'])}` | {d:.1f} |")
+    lines.append(f"\nkernel_us_total: {total:.1f}")
+    return "\n".join(lines)
+
+
 def profiled_traffic():
     """HBM bytes per forward call from the committed rocprofv3 summary."""
     try:
@@ -526,6 +548,8 @@ def main():
     if args.summarize:
         if args.iterations:
             print(summarize_iterations(args.summarize, args.match, args.iterations))
+        elif args.sequence:
+            print(summarize_sequence(args.summarize, args.match, args.sequence))
         else:
             print(summarize(args.summarize, args.calls, args.match))
         return
