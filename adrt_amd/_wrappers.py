@@ -26,23 +26,25 @@ def _set_module(module):
     return decorate
 
 
-def _format_object_type(obj, /):
-    """_wrappers.py:82-93."""
-    t = type(obj)
-    if t.__module__ == "builtins":
-        return str(t.__qualname__)
-    return f"{t.__module__}.{t.__qualname__}"
+def _type_name(obj, /):
+    """The name of ``obj``'s type as the reference's messages print it
+    (_wrappers.py:82-93): the qualified name alone for built-in types, else
+    prefixed by its module."""
+    cls = type(obj)
+    if cls.__module__ == "builtins":
+        return cls.__qualname__
+    return ".".join((cls.__module__, cls.__qualname__))
 
 
 def _normalize_array(a, /):
+    """What the native layer takes: a contiguous device tensor, or a base-class
+    NumPy array in C order, native byte order and aligned (copied only when
+    one of those fails); anything else raises the reference's TypeError."""
     if dev.is_tensor(a):
         return a.contiguous()
     if not isinstance(a, np.ndarray):
-        raise TypeError(f"array must be numpy.ndarray, but got {_format_object_type(a)}")
-    a = np.asarray(a, a.dtype.newbyteorder("="), "C")
-    if not a.flags.aligned:
-        return a.copy("C")
-    return a
+        raise TypeError(f"array must be numpy.ndarray, but got {_type_name(a)}")
+    return np.require(a, a.dtype.newbyteorder("="), ["C", "A", "E"])
 
 
 @_set_module("adrt_amd")

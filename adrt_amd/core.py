@@ -13,7 +13,7 @@ import numpy as np
 from . import _device as dev
 from . import _kernels
 from ._wrappers import (
-    _format_object_type,
+    _type_name,
     adrt_step,
     bdrt_step,
     threading_enabled,
@@ -53,7 +53,7 @@ def adrt_init(a, /):
     ``(..., 4, 2N-1, N)`` with the quadrant flips in the top N offsets.
     """
     if not (isinstance(a, np.ndarray) or dev.is_tensor(a)):
-        raise TypeError(f"array must be numpy.ndarray, but got {_format_object_type(a)}")
+        raise TypeError(f"array must be numpy.ndarray, but got {_type_name(a)}")
     if a.ndim > 3 or a.ndim < 2:
         raise ValueError(f"array must have between 2 and 3 dimensions, but had {a.ndim}")
     if a.shape[-1] != a.shape[-2] or bin(operator.index(a.shape[-1])).count("1") != 1:
