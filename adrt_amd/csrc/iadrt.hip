@@ -34,6 +34,12 @@ namespace iadrt_native {
 constexpr int kLanes = 64;
 constexpr int kRing = 64;   // rows in flight: lag 0..63
 constexpr int kSteps = 8;   // steps per prefetch block
+// Row pitch of the LDS rings, in elements.  Lagged lanes access different
+// rows of a ring at one step: at pitch 65 the 16 odd chains of a group (rows
+// s - col, columns c0 + col) all fall in one bank.  At 68 the tail's accesses
+// of a step conflict for about one cycle in all (64 four-byte banks), and,
+// unlike at 64, a row address stays one 24-bit multiply-add.
+constexpr int kPitch = kLanes + 4;
 
 // Wave-uniform parameters of one wavefront of a stage.
 struct Wave {
@@ -192,11 +198,31 @@ __host__ __device__ inline bool pair_step(const PairLane &pp, int y, int R, T p0
 // its first row reached in the sweep, are +0 (their sources are +0 rows);
 // the histories start at +0 for them.  Output rows lag by type and column;
 // each type stores, at step y, its row y - minlag (the row its least-lagged
-// lane produces now, which every lane of the type has produced by then)
-// through a ring per type in LDS of 16 (B), 32 (C) or 64 (D) rows (lag
-// spreads 15, 30, 45); A stores its row y directly.  The rows above the
+// lane produces now, which every lane of the type has produced by then), so
+// a lane's value of step t is stored at step t + d, d = lag - minlag (0 for A
+// and for the first lane of each type, up to 15, 30 and 45 for B, C and D).
+// The values wait in a per-lane delay line in LDS, indexed by step: slot
+// t & mask of the lane's column in its type's region.  The rows above the
 // first flushed row are filled with +0 after the sweep.
-constexpr int kQuadRing = 16 * (16 + 32 + 64);  // ring elements of B, C and D
+//
+// Steps run in blocks of 16 that start at multiples of 16, so a block writes
+// 16 consecutive slots of each line (one address and 16 immediate offsets);
+// its 16 reads start at slot (t0 - d) & mask and may run past the last slot,
+// so the first 16 slots of B, C and D are mirrored after the last (written
+// twice in the blocks that start at slot 0).  Regions (16 columns each): A 16
+// slots, B and C 32 + 16, D 64 + 16; their bases lie 16 elements apart
+// modulo 64, so the four types' columns of one slot fall in distinct banks.
+constexpr int kQuadAhead = 32;         // pair registers: loads run kQuadAhead - 1 steps ahead
+constexpr int kQuadMirror = 16;        // mirrored slots
+constexpr int kQuadMirrorPeriod = 32;  // smallest mirrored line: blocks at t0 % 32 == 0 mirror
+constexpr int kQuadRing = 1840 + (64 + kQuadMirror) * 16;  // ring elements (D's region ends it)
+
+__host__ __device__ constexpr int quad_slots(int type) {
+    return type == 0 ? 16 : (type == 3 ? 64 : 32);
+}
+__host__ __device__ constexpr int quad_region(int type) {
+    return type == 0 ? 0 : (type == 1 ? 272 : (type == 2 ? 1056 : 1840));
+}
 
 struct QuadLane {
     PairLane p;  // the lane's stage-k chain
@@ -205,7 +231,7 @@ struct QuadLane {
     int lag;     // stage-(k+1) output row = y - lag
     int minlag;  // smallest lag of the type in the wave (flush row = y - minlag)
     int oc;      // stage-(k+1) output column
-    int rbase, rmask, ri;  // ring element of row r: rbase + (r & rmask) * 16 + ri
+    int i;       // column of the lane's delay line in its type's region
 };
 
 __host__ __device__ inline QuadLane make_quad_lane(int n, int Li, int wave_in_plane, int lane,
@@ -220,42 +246,131 @@ __host__ __device__ inline QuadLane make_quad_lane(int n, int Li, int wave_in_pl
     q.colq = c0 / 2 + i;
     q.oc = (4 * pl + q.type) * Coq + q.colq;
     const int cm = c0 / 2;
-    const int k = q.type == 0 ? 0 : (q.type == 1 ? 1 : (q.type == 2 ? 2 : 3));
-    const int mult = k == 3 ? 3 : k;        // lag = add + mult * col'
-    const int add = k == 0 ? 0 : (k == 3 ? 2 : 1);
+    const int mult = q.type;  // lag = add + mult * col'
+    const int add = q.type == 0 ? 0 : (q.type == 3 ? 2 : 1);
     q.lag = add + mult * q.colq;
     q.minlag = add + mult * cm;
-    q.ri = i;
-    q.rmask = k == 1 ? 15 : (k == 2 ? 31 : 63);
-    q.rbase = k == 1 ? 0 : (k == 2 ? 16 * 16 : 16 * (16 + 32));
+    q.i = i;
     return q;
 }
 
-// The value this lane sends its partner at a step (h0 = this step's stage-k
-// value, h1 the previous, h2 the one before).
+// The wave's largest flush lag (type D's minlag): stores address row
+// y - quad_mlmax(c0) plus a per-lane offset of quad_store_off elements.
+__host__ __device__ inline int quad_mlmax(int c0) { return 2 + 3 * (c0 / 2); }
+__host__ __device__ inline size_t quad_store_off(const QuadLane &q, int n, int mlmax) {
+    return size_t(mlmax - q.minlag) * size_t(n) + size_t(q.oc);
+}
+// Whether every store offset of an n-column plane, in bytes, fits 32 bits
+// (mlmax <= 3n/4, oc < n).
 template <typename T>
-__host__ __device__ __forceinline__ T quad_send(const QuadLane &q, T h0, T h1, T h2) {
-    return q.type == 1 ? h1 : (q.type == 3 ? h2 : h0);
+__host__ __device__ inline bool quad_narrow(size_t n) {
+    return (3 * (n / 4) + 3) * n * sizeof(T) <= size_t(0xffffffffu);
 }
 
-// Stage k + 1 at step y: a = the lane's own operand (D: h1, else h0), b =
-// the partner's; the reference's order of additions.
+// Last block start t0 at which every lane of the wave produces a stage-k
+// value, a stage-(k+1) row and a flush row at all 16 steps t0 .. t0 + 15:
+// odd stage-k chains need y >= col + 1 (col <= c0 + 31), stage-(k+1) chains
+// y >= lag (lag <= 3 c0 / 2 + 47), flushes y >= minlag.  With t0 >= 16 as
+// well (row y + 1 < R for every chain), such a block needs no checks.
+__host__ __device__ inline int quad_bulk_last(int R, int c0) { return R - 63 - 3 * (c0 / 2); }
+
+// Per-lane constants of the branch-free step (t = step, y = R - 1 - t).
+struct QuadConst {
+    bool odd;      // the stage-k chain is odd
+    bool send1;    // B sends h1
+    bool send2;    // D sends h2 (others h0)
+    bool own1;     // D: a = h1 (others h0)
+    bool first_a;  // A: at t = 0 (row R - 1) neither b nor acc enters
+    int made_lim;  // stage k produces a value while t <= made_lim
+    int made2_lim; // stage k + 1 produces a row while t <= made2_lim
+    int flush_lim; // the flush row y - minlag exists while t <= flush_lim
+    int col0;      // element of slot 0 of the lane's delay line
+    int mask;      // slots - 1
+    int delay;     // lag - minlag
+    int mirror;    // elements from slot s to its mirror (0: A, never read past its end)
+};
+
+__host__ __device__ inline QuadConst make_quad_const(const QuadLane &q, int R) {
+    QuadConst c;
+    c.odd = q.p.odd;
+    c.send1 = q.type == 1;
+    c.send2 = q.type == 3;
+    c.own1 = q.type == 3;
+    c.first_a = q.type == 0;
+    c.made_lim = q.p.odd ? R - 2 - q.p.col : R;  // odd: row y - 1 - col >= 0
+    c.made2_lim = R - 1 - q.lag;
+    c.flush_lim = R - 1 - q.minlag;
+    c.col0 = quad_region(q.type) + q.i;
+    c.mask = quad_slots(q.type) - 1;
+    c.delay = q.lag - q.minlag;
+    c.mirror = q.type == 0 ? 0 : quad_slots(q.type) * 16;
+    return c;
+}
+
+// Ring elements of a block's first step (step k adds 16 k): the write, the
+// mirror write (= the write where none is due) and the read.
+struct QuadBlock {
+    int w, m, r;
+};
+
+__host__ __device__ inline QuadBlock quad_block(const QuadConst &c, int t0) {
+    QuadBlock b;
+    const int ws = t0 & c.mask;
+    b.w = c.col0 + ws * 16;
+    b.m = ws == 0 ? b.w + c.mirror : b.w;
+    b.r = c.col0 + ((t0 - c.delay) & c.mask) * 16;
+    return b;
+}
+
 template <typename T>
-__host__ __device__ __forceinline__ bool quad_step(const QuadLane &q, int y, int R, T a, T b,
-                                                   T &acc, T &val, int &row) {
-    row = y - q.lag;
-    if (row < 0) return false;
-    T v = T(0);
-    v += a;
-    if (q.type == 0 || q.type == 2) {  // even chains: b enters while row + 1 < R
-        if (row + 1 < R) v -= b;
-    } else {  // odd chains: row + 1 + col' < R at every row the sweep reaches
-        v -= b;
+struct QuadState {
+    T acc, prev1;  // stage k: running value, the previous pair's second element
+    T h0, h1, h2;  // stage-k values of this step and the two before (+0 where none)
+    T acc2;        // stage k + 1 running value
+};
+
+// Stage k at step t from the lane's source pair (p0, p1) at row y, in the
+// reference's order: even  v = 0 + p0 - prev1 + acc  (only p0 at t = 0),
+// odd  v = 0 + p1 - p0 + acc.  Returns the value the lane sends its partner.
+// Without kChecked every condition must hold (a bulk block).
+template <bool kChecked, typename T>
+__host__ __device__ __forceinline__ T quad_stage_k(const QuadConst &c, int t, T p0, T p1,
+                                                   QuadState<T> &s) {
+    const T x = c.odd ? p1 : p0;
+    const T z = c.odd ? p0 : s.prev1;
+    const T u = T(0) + x;
+    T v = (u - z) + s.acc;
+    s.prev1 = p1;
+    if (kChecked) {
+        v = (c.odd || t != 0) ? v : u;
+        const bool made = t <= c.made_lim;
+        s.acc = made ? v : s.acc;
+        v = made ? v : T(0);
+    } else {
+        s.acc = v;
     }
-    if (row + 1 < R) v += acc;
-    acc = v;
-    val = v;
-    return true;
+    s.h2 = s.h1;
+    s.h1 = s.h0;
+    s.h0 = v;
+    return c.send2 ? s.h2 : (c.send1 ? s.h1 : s.h0);
+}
+
+// Stage k + 1 at step t: a = the lane's own operand, b = the partner's;
+// v = 0 + a - b + acc2 (A at t = 0: only a).  Returns v (junk where the row
+// is negative: it is never flushed).
+template <bool kChecked, typename T>
+__host__ __device__ __forceinline__ T quad_stage_k1(const QuadConst &c, int t, T b,
+                                                    QuadState<T> &s) {
+    const T a = c.own1 ? s.h1 : s.h0;
+    const T u = T(0) + a;
+    T v = (u - b) + s.acc2;
+    if (kChecked) {
+        v = (c.first_a && t == 0) ? u : v;
+        s.acc2 = t <= c.made2_lim ? v : s.acc2;
+    } else {
+        s.acc2 = v;
+    }
+    return v;
 }
 
 // Fused tail.  Once a stage's input groups are at most 64 columns wide
@@ -296,7 +411,7 @@ __host__ __device__ constexpr int tail_ring_base(int S, int t) {
 __host__ __device__ constexpr int tail_rows_total(int S) { return tail_ring_base(S, S - 1); }
 
 // The operands of row x of a later stage's chain in its producer's ring
-// (rows base .. base + mask, pitch kLanes + 1):
+// (rows base .. base + mask, pitch kPitch):
 //   even: a = src[2col][x],          b = src[2col+1][x+1]  (b used if x+1 < R)
 //   odd:  a = src[2col+1][x+1+col],  b = src[2col][x+1+col] (both if x+1+col < R)
 // As per-lane constants: a sits at ring row x + offa, column ca, and enters
@@ -340,8 +455,8 @@ struct TailOperands {
 __host__ __device__ __forceinline__ TailOperands tail_operands(const TailLane &tl, int x,
                                                                int base, int mask) {
     TailOperands o;
-    o.ia = (base + ((x + tl.offa) & mask)) * (kLanes + 1) + tl.ca;
-    o.ib = (base + ((x + tl.offb) & mask)) * (kLanes + 1) + tl.cb;
+    o.ia = (base + ((x + tl.offa) & mask)) * kPitch + tl.ca;
+    o.ib = (base + ((x + tl.offb) & mask)) * kPitch + tl.cb;
     o.use_a = x < tl.lima;
     o.use_b = x < tl.limb;
     return o;
@@ -376,7 +491,7 @@ __global__ void __launch_bounds__(kLanes)
     stage_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
                  int waves_per_plane) {
     using P2 = typename Pair<T>::type;
-    __shared__ T ring[kRing][kLanes + 1];
+    __shared__ T ring[kRing][kPitch];
     const int lane = threadIdx.x;
     const size_t plane = blockIdx.x / waves_per_plane;
     const Wave w = make_wave(R, n, Li, int(blockIdx.x % waves_per_plane));
@@ -487,81 +602,199 @@ __global__ void __launch_bounds__(kLanes)
         for (int r = R - 1 - c0; r < R; ++r) dst[size_t(r) * n] = T(0);
 }
 
+// Global-memory bytes: a pointer that passes through an asm statement keeps
+// its address space (and so global rather than flat instructions) only when
+// its type names it.
+using GByte = __attribute__((address_space(1))) char;
+
+// The value of lane ^ 1 (DPP quad_perm [1, 0, 3, 2]; every lane active).
 template <typename T>
-__device__ __forceinline__ T lane_swap1(T v);
+__device__ __forceinline__ T partner(T v);
 template <>
-__device__ __forceinline__ float lane_swap1<float>(float v) {
-    return __builtin_bit_cast(float, __shfl_xor(__builtin_bit_cast(int, v), 1));
+__device__ __forceinline__ float partner<float>(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
 }
 template <>
-__device__ __forceinline__ double lane_swap1<double>(double v) {
-    return __shfl_xor(v, 1);
+__device__ __forceinline__ double partner<double>(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u)), 0xB1, 0xF, 0xF, true));
+    const unsigned hi =
+        unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u >> 32)), 0xB1, 0xF, 0xF, true));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
 // Two paired stages (stage k has Co >= 32): grid = planes * (n / 64)
 // single-wave work-groups; `in` holds stage k's input, `out` receives stage
 // k + 1's output.
-template <typename T>
+//
+// Addressing.  A row is a wave-uniform (scalar) row address plus a per-lane
+// byte offset, so a load or store costs no vector instruction for its
+// address: the offsets pass through an empty asm statement once per block,
+// which keeps them in the block beside the accesses (hoisted out of the loop,
+// their 64-bit extension would cost an add per access), and the row address
+// steps by one row through another, which keeps the compiler from folding
+// the steps into one per-lane address.  Flush stores wait
+// to the end of their block, off the serial chain of the LDS reads they
+// store.  Bulk blocks store at row y - mlmax plus the lane's offset of
+// mlmax - minlag rows (Off: uint32_t where quad_narrow holds, else size_t);
+// checked blocks store per lane, where their rows exist.
+template <typename T, typename Off>
 __global__ void __launch_bounds__(kLanes)
     quad_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
                 int waves_per_plane) {
-    using P2 = typename Pair<T>::type;
+    using P2 = T __attribute__((ext_vector_type(2)));  // a native vector: loads from address space 1
+    using False = std::integral_constant<bool, false>;
+    using True = std::integral_constant<bool, true>;
     __shared__ T ring[kQuadRing];
     const int lane = threadIdx.x;
     const size_t plane = blockIdx.x / waves_per_plane;
     int c0;
     const QuadLane q = make_quad_lane(n, Li, int(blockIdx.x % waves_per_plane), lane, c0);
-    const T *src = in + plane * size_t(R) * n + q.p.pc;
-    T *dst = out + plane * size_t(R) * n + q.oc;
+    const QuadConst c = make_quad_const(q, R);
+    const int mlmax = quad_mlmax(c0);
+    const size_t row_bytes = size_t(n) * sizeof(T);
+    const GByte *gsrc = (const GByte *)(in + plane * size_t(R) * n);
+    GByte *gdst = (GByte *)(out + plane * size_t(R) * n);
+    T *dst_lane = out + plane * size_t(R) * n + q.oc;
+    const uint32_t src_off = uint32_t(q.p.pc) * uint32_t(sizeof(T));
+    const Off dst_off = Off(quad_store_off(q, n, mlmax) * sizeof(T));
     for (int e = lane; e < kQuadRing; e += kLanes) ring[e] = T(0);
     __syncthreads();
 
-    auto load = [&](int t) -> P2 {
-        const int y = R - 1 - t;
-        return *reinterpret_cast<const P2 *>(src + size_t(y < 0 ? 0 : y) * n);
+    QuadState<T> s;
+    s.acc = s.prev1 = s.h0 = s.h1 = s.h2 = s.acc2 = T(0);
+    // step t0 + k: both stages, the delay-line write(s), the read of the flush value
+    auto step = [&](auto checked, auto mirrored, int t0, int k, const P2 &p,
+                    const QuadBlock &b) -> T {
+        constexpr bool kChecked = decltype(checked)::value;
+        const int t = t0 + k;
+        const T send = quad_stage_k<kChecked, T>(c, t, p.x, p.y, s);
+        const T v2 = quad_stage_k1<kChecked, T>(c, t, partner<T>(send), s);
+        ring[b.w + 16 * k] = v2;
+        if (decltype(mirrored)::value) ring[b.m + 16 * k] = v2;
+        return ring[b.r + 16 * k];
     };
-    T acc = T(0), prev1 = T(0), acc2 = T(0);
-    T h0 = T(0), h1 = T(0), h2 = T(0);
-    auto run_step = [&](int t, const P2 &p) {
-        const int y = R - 1 - t;
-        T val;
-        int row;
-        // stage k (values of rows a stage-k odd chain does not reach stay +0)
-        const bool made = pair_step<T>(q.p, y, R, p.x, p.y, prev1, acc, val, row);
-        prev1 = p.y;
-        h2 = h1;
-        h1 = h0;
-        h0 = made ? val : T(0);
-        // stage k + 1
-        const T b = lane_swap1<T>(quad_send<T>(q, h0, h1, h2));
-        const T a = q.type == 3 ? h1 : h0;
-        T v2;
-        int row2;
-        const bool made2 = quad_step<T>(q, y, R, a, b, acc2, v2, row2);
-        if (q.type != 0 && made2) ring[q.rbase + (row2 & q.rmask) * 16 + q.ri] = v2;
-        // one store per step: A its row y, the others their type's row y - minlag
-        const int f = y - q.minlag;
-        const T ov = q.type == 0 ? v2 : ring[q.rbase + (f & q.rmask) * 16 + q.ri];
-        if (f >= 0) dst[size_t(f) * n] = ov;
+    // a bulk block's flushes: rows R - 1 - t0 - k - minlag, all present
+    auto flush_bulk = [&](int t0, const T *ov) {
+        Off off = dst_off;
+        asm volatile("" : "+v"(off));
+        GByte *row = gdst + size_t(R - 1 - t0 - mlmax) * row_bytes;
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) {
+            if (k > 0) {
+                row -= row_bytes;
+                asm volatile("" : "+s"(row));
+            }
+            *reinterpret_cast<__attribute__((address_space(1))) T *>(row + off) = ov[k];
+        }
+    };
+    // a checked block's flushes, steps t0 .. t0 + count - 1
+    auto flush_checked = [&](int t0, int count, const T *ov) {
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k)
+            if (k < count && t0 + k <= c.flush_lim)
+                dst_lane[size_t(R - 1 - t0 - k - q.minlag) * n] = ov[k];
     };
 
-    P2 cur[kPairSteps];
-    int t = 0;
+    // Loads run kQuadAhead - 1 steps ahead in a ring of registers: slot
+    // t % kQuadAhead holds the pair of step t; step t reads it, and slot
+    // (t - 1) % kQuadAhead (step t - 1's pair, whose second element step t
+    // still reads as prev1) is refilled after step t with the pair of step
+    // t + kQuadAhead - 1.  Each refill comes after its slot's last reader in
+    // program order, and the refills' row addresses form one chain through
+    // asm statements, so every slot keeps one register across iterations (no
+    // copies, which would wait for every load in flight).  An iteration is
+    // two 16-step blocks from t0 % 32 == 0 (the first may mirror, the second
+    // never does).
+    using GP2 = const __attribute__((address_space(1))) P2;
+    P2 buf[kQuadAhead];
+    {
+        uint32_t off = src_off;
+        asm volatile("" : "+v"(off));
 #pragma unroll
-    for (int k = 0; k < kPairSteps; ++k) cur[k] = load(k);
-    for (; t + kPairSteps <= R; t += kPairSteps) {  // whole blocks
-        P2 nxt[kPairSteps];
-#pragma unroll
-        for (int k = 0; k < kPairSteps; ++k) nxt[k] = load(t + kPairSteps + k);
-#pragma unroll
-        for (int k = 0; k < kPairSteps; ++k) run_step(t + k, cur[k]);
-#pragma unroll
-        for (int k = 0; k < kPairSteps; ++k) cur[k] = nxt[k];
+        for (int k = 0; k < kQuadAhead - 1; ++k) {  // steps 0 .. kQuadAhead - 2 (clamped)
+            const int y = R - 1 - k;
+            buf[k] = *reinterpret_cast<GP2 *>(gsrc + size_t(y < 0 ? 0 : y) * row_bytes + off);
+        }
     }
+    // an iteration's refills: steps t0 + k + kQuadAhead - 1, rows y0 - k
+    // (clamped to row 0 in the last iterations; never needed where checks are)
+    auto refill_rows = [&](auto clamped, int t0, auto body) {
+        const int y0 = R - 1 - t0 - (kQuadAhead - 1);
+        uint32_t off = src_off;
+        asm volatile("" : "+v"(off));
+        const GByte *row = gsrc + size_t(y0 < 0 ? 0 : y0) * row_bytes;
+        auto refill = [&](int k) {
+            if (k > 0) {
+                if (decltype(clamped)::value)
+                    row = y0 - k >= 0 ? row - row_bytes : row;
+                else
+                    row -= row_bytes;
+                asm volatile("" : "+s"(row));
+            }
+            buf[(k + kQuadAhead - 1) % kQuadAhead] = *reinterpret_cast<GP2 *>(row + off);
+            // no instruction crosses a step: hoisting later steps' operand
+            // selects would wait for their loads early and empty the ring
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        body(refill);
+    };
+    auto checked_iteration = [&](int t0) {
+        refill_rows(True(), t0, [&](auto &refill) {
 #pragma unroll
-    for (int k = 0; k < kPairSteps; ++k)  // tail: steps t .. R-1
-        if (t + k < R) run_step(t + k, cur[k]);
-    for (int r = R - q.minlag; r < R; ++r) dst[size_t(r) * n] = T(0);  // the +0 rows
+            for (int h = 0; h < kQuadAhead; h += kPairSteps) {
+                const QuadBlock b = quad_block(c, t0 + h);
+                T ov[kPairSteps];
+#pragma unroll
+                for (int k = 0; k < kPairSteps; ++k) {
+                    ov[k] = step(True(), True(), t0 + h, k, buf[h + k], b);
+                    refill(h + k);
+                }
+                flush_checked(t0 + h, kPairSteps, ov);
+            }
+        });
+    };
+    auto bulk_iteration = [&](int t0) {
+        refill_rows(False(), t0, [&](auto &refill) {  // rows y0 - 31 >= 0: see quad_bulk_last
+            T ov[kPairSteps];
+            QuadBlock b = quad_block(c, t0);
+#pragma unroll
+            for (int k = 0; k < kPairSteps; ++k) {
+                ov[k] = step(False(), True(), t0, k, buf[k], b);
+                refill(k);
+            }
+            flush_bulk(t0, ov);
+            b = quad_block(c, t0 + kPairSteps);
+#pragma unroll
+            for (int k = 0; k < kPairSteps; ++k) {
+                ov[k] = step(False(), False(), t0 + kPairSteps, k, buf[kPairSteps + k], b);
+                refill(kPairSteps + k);
+            }
+            flush_bulk(t0 + kPairSteps, ov);
+        });
+    };
+    // whole iterations: checked until the first check-free one (t0 >= 16,
+    // both blocks within quad_bulk_last), check-free while they last, checked
+    // after; three loops, so that no loop body branches
+    const int bulk_last = quad_bulk_last(R, c0);
+    int t0 = 0;
+    for (; t0 + kQuadAhead <= R && !(t0 >= kPairSteps && t0 + kPairSteps <= bulk_last);
+         t0 += kQuadAhead)
+        checked_iteration(t0);
+    for (; t0 + kQuadAhead <= R && t0 + kPairSteps <= bulk_last; t0 += kQuadAhead)
+        bulk_iteration(t0);
+    for (; t0 + kQuadAhead <= R; t0 += kQuadAhead) checked_iteration(t0);
+#pragma unroll
+    for (int h = 0; h < kQuadAhead; h += kPairSteps) {  // tail: steps t0 .. R - 1
+        const QuadBlock b = quad_block(c, t0 + h);
+        T ov[kPairSteps];
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k)
+            if (t0 + h + k < R) ov[k] = step(True(), True(), t0 + h, k, buf[h + k], b);
+        flush_checked(t0 + h, R - t0 - h, ov);
+    }
+    for (int r = R - q.minlag; r < R; ++r) dst_lane[size_t(r) * n] = T(0);  // the +0 rows
 }
 
 // The last S stages (the first with Li0 groups, Ci = n / Li0 = 2^S), one
@@ -572,7 +805,7 @@ __global__ void __launch_bounds__(kLanes)
                 int waves_per_plane) {
     using P2 = typename Pair<T>::type;
     constexpr int kM0 = tail_ring_rows(S, 0);
-    __shared__ T rbuf[tail_rows_total(S) * (kLanes + 1)];
+    __shared__ T rbuf[tail_rows_total(S) * kPitch];
     const int lane = threadIdx.x;
     const size_t plane = blockIdx.x / waves_per_plane;
     const Wave w = make_wave(R, n, Li0, int(blockIdx.x % waves_per_plane));
@@ -613,7 +846,7 @@ __global__ void __launch_bounds__(kLanes)
         T val;
         int row;
         if (step<T>(w, ln, sk, p.x, p.y, prev1, acc0, val, row))
-            rbuf[(row & (kM0 - 1)) * (kLanes + 1) + lane] = val;
+            rbuf[(row & (kM0 - 1)) * kPitch + lane] = val;
         prev1 = p.y;
         // later stages: row x = sk + t is wave-uniform
 #pragma unroll
@@ -623,7 +856,7 @@ __global__ void __launch_bounds__(kLanes)
             const T v = tail_combine<T>(o[t - 1], a[t - 1], b[t - 1], acc[t - 1], x + 1 < R);
             acc[t - 1] = v;
             if (t < S - 1)
-                rbuf[(tail_ring_base(S, t) + (x & (tail_ring_rows(S, t) - 1))) * (kLanes + 1) +
+                rbuf[(tail_ring_base(S, t) + (x & (tail_ring_rows(S, t) - 1))) * kPitch +
                      lane] = v;
             else if (tl[t - 1].live)
                 dst[size_t(x) * n] = v;
@@ -717,9 +950,14 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
         T *dst = bufs[pass & 1];
         const bool paired = int(n) / Li / 2 >= iadrt_native::kHalf;  // Co >= 32
         if (paired && quad && k + 1 < k0) {  // stages k and k + 1 in one pass
-            hipLaunchKernelGGL(iadrt_native::quad_kernel<T>, dim3(grid),
-                               dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n), Li,
-                               waves_per_plane);
+            if (iadrt_native::quad_narrow<T>(n))
+                hipLaunchKernelGGL((iadrt_native::quad_kernel<T, uint32_t>), dim3(grid),
+                                   dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
+                                   Li, waves_per_plane);
+            else
+                hipLaunchKernelGGL((iadrt_native::quad_kernel<T, size_t>), dim3(grid),
+                                   dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
+                                   Li, waves_per_plane);
             k += 2;
             Li *= 4;
         } else {

@@ -26,7 +26,7 @@ template <typename T>
 static void emulate_stage(const std::vector<T> &in, std::vector<T> &out, int planes, int R, int n,
                           int Li) {
     const int waves = (n + kLanes - 1) / kLanes;
-    std::vector<T> ring(size_t(kRing) * (kLanes + 1));
+    std::vector<T> ring(size_t(kRing) * kPitch);
     for (int plane = 0; plane < planes; ++plane) {
         const T *src = in.data() + size_t(plane) * R * n;
         T *dst = out.data() + size_t(plane) * R * n;
@@ -49,7 +49,7 @@ static void emulate_stage(const std::vector<T> &in, std::vector<T> &out, int pla
                     T val;
                     int row;
                     if (step<T>(w, ln[l], s, p0, p1, prev1[l], acc[l], val, row))
-                        ring[size_t(row % kRing) * (kLanes + 1) + l] = val;
+                        ring[size_t(row % kRing) * kPitch + l] = val;
                     prev1[l] = p1;
                 }
                 // the kernel's unconditional flush: steps above the last row
@@ -58,7 +58,7 @@ static void emulate_stage(const std::vector<T> &in, std::vector<T> &out, int pla
                 for (int l = 0; l < kLanes; ++l) {
                     const int fl = w.ch0 + l < n ? l : n - 1 - w.ch0;
                     const int fr = s < R ? s : R - 1;
-                    dst[size_t(fr) * n + w.ch0 + fl] = ring[size_t(s % kRing) * (kLanes + 1) + fl];
+                    dst[size_t(fr) * n + w.ch0 + fl] = ring[size_t(s % kRing) * kPitch + fl];
                 }
             }
         }
@@ -126,76 +126,114 @@ static void emulate_pair_stage(const std::vector<T> &in, std::vector<T> &out, in
     }
 }
 
-// Two paired stages in one pass (the quad kernel): lanes in lockstep, each
-// keeping its last three stage-k values; the partner exchange is lane ^ 1;
-// the B / C / D rings are exactly the kernel's, every slot tagged with the
-// row it holds (+0 slots with their own tag), and every flush checks that it
-// reads its row or a +0 row above the lane's first real row.
+// Two paired stages in one pass (the quad kernel): lanes in lockstep, in the
+// kernel's blocks of 16 steps (two per iteration of kQuadAhead steps, the
+// first with mirror writes where check-free); a check-free block runs the unchecked step
+// only after verifying, for every lane and step, each condition that step
+// drops.  The partner exchange is lane ^ 1.  The delay lines are the
+// kernel's ring (its element indices, its mirror writes; blocks without
+// mirror writes verify that none was due), every element tagged with the
+// step that wrote it, and every flush checks that it reads the value of step
+// t - delay, or an element no step has written when t - delay < 0 (the +0
+// rows above the lane's first row).  Check-free blocks store through the
+// kernel's addressing (row y - mlmax plus the lane's offset, which must fit
+// 32 bits where the kernel takes it so).
 template <typename T>
 static void emulate_quad_stage(const std::vector<T> &in, std::vector<T> &out, int planes, int R,
                                int n, int Li) {
     const int waves = n / kLanes;
     const int kZero = -(1 << 30);
+    const bool narrow = quad_narrow<T>(size_t(n));
+    for (int type = 0; type < 3; ++type)  // regions in order, disjoint, inside the ring
+        if (quad_region(type) + 16 * (quad_slots(type) + (type ? kQuadMirror : 0)) >
+            quad_region(type + 1))
+            std::abort();
+    if (quad_region(3) + 16 * (quad_slots(3) + kQuadMirror) != kQuadRing) std::abort();
     for (int plane = 0; plane < planes; ++plane) {
         const T *src = in.data() + size_t(plane) * R * n;
         T *dst = out.data() + size_t(plane) * R * n;
         for (int wv = 0; wv < waves; ++wv) {
             QuadLane q[kLanes];
+            QuadConst c[kLanes];
             int c0 = 0;
             for (int l = 0; l < kLanes; ++l) {
                 q[l] = make_quad_lane(n, Li, wv, l, c0);
-                if (q[l].p.pc < 0 || q[l].p.pc + 1 >= n || q[l].oc < 0 || q[l].oc >= n)
+                c[l] = make_quad_const(q[l], R);
+                if (q[l].p.pc < 0 || q[l].p.pc + 1 >= n || q[l].oc < 0 || q[l].oc >= n ||
+                    q[l].i < 0 || q[l].i > 15)
                     std::abort();
-                if (q[l].type != 0 &&
-                    (q[l].rbase + q[l].rmask * 16 + q[l].ri >= kQuadRing || q[l].ri > 15))
+                if (c[l].mirror != (q[l].type ? 16 * (c[l].mask + 1) : 0)) std::abort();
+            }
+            const int mlmax = quad_mlmax(c0);
+            for (int l = 0; l < kLanes; ++l) {
+                if (q[l].minlag > mlmax) std::abort();
+                if (narrow && quad_store_off(q[l], n, mlmax) * sizeof(T) > 0xffffffffull)
                     std::abort();
             }
             std::vector<T> ring(kQuadRing, T(0));
             std::vector<int> tag(kQuadRing, kZero);
-            T acc[kLanes], prev1[kLanes], acc2[kLanes], h0[kLanes], h1[kLanes], h2[kLanes];
-            for (int l = 0; l < kLanes; ++l)
-                acc[l] = prev1[l] = acc2[l] = h0[l] = h1[l] = h2[l] = T(0);
-            for (int t = 0; t < R; ++t) {
-                const int y = R - 1 - t;
+            QuadState<T> st[kLanes];
+            for (auto &x : st) x.acc = x.prev1 = x.h0 = x.h1 = x.h2 = x.acc2 = T(0);
+            const int bulk_last = quad_bulk_last(R, c0);
+            // iterations of kQuadAhead steps (two blocks), check-free as a whole
+            // (the kernel's three loops take the iterations where this holds,
+            // an interval of t0, in the middle one)
+            for (int t0 = 0; t0 < R; t0 += kPairSteps) {
+                const int it0 = t0 - t0 % kQuadAhead;
+                const bool bulk = it0 + kQuadAhead <= R && it0 >= kPairSteps &&
+                                  it0 + kPairSteps <= bulk_last;
+                const bool mirrored = !bulk || t0 == it0;
+                QuadBlock b[kLanes];
                 for (int l = 0; l < kLanes; ++l) {
-                    const size_t at = size_t(y) * n + q[l].p.pc;
-                    const T p0 = src[at], p1 = src[at + 1];
-                    T val;
-                    int row;
-                    const bool made = pair_step<T>(q[l].p, y, R, p0, p1, prev1[l], acc[l], val, row);
-                    prev1[l] = p1;
-                    h2[l] = h1[l];
-                    h1[l] = h0[l];
-                    h0[l] = made ? val : T(0);
-                }
-                T send[kLanes];
-                for (int l = 0; l < kLanes; ++l) send[l] = quad_send<T>(q[l], h0[l], h1[l], h2[l]);
-                for (int l = 0; l < kLanes; ++l) {
-                    const T b = send[l ^ 1];
-                    const T a = q[l].type == 3 ? h1[l] : h0[l];
-                    T v2;
-                    int row2;
-                    const bool made2 = quad_step<T>(q[l], y, R, a, b, acc2[l], v2, row2);
-                    const int f = y - q[l].minlag;
-                    if (q[l].type == 0) {
-                        if (!made2 || row2 != y) std::abort();
-                        dst[size_t(y) * n + q[l].oc] = v2;
-                        continue;
-                    }
-                    if (made2) {
-                        const size_t slot = size_t(q[l].rbase + (row2 & q[l].rmask) * 16 + q[l].ri);
-                        ring.at(slot) = v2;
-                        tag.at(slot) = row2;
-                    }
-                    if (f < 0) continue;
-                    const size_t slot = size_t(q[l].rbase + (f & q[l].rmask) * 16 + q[l].ri);
-                    const int first = R - 1 - q[l].lag;  // the lane's first row in the sweep
-                    const bool skipped = tag[slot] == kZero && f > first;
-                    if (tag[slot] != f && !skipped) {
-                        std::fprintf(stderr, "stale quad flush: row %d lane %d\n", f, l);
+                    b[l] = quad_block(c[l], t0);
+                    if (!mirrored && b[l].m != b[l].w) {
+                        std::fprintf(stderr, "mirror write skipped: step %d lane %d\n", t0, l);
                         std::abort();
                     }
-                    dst[size_t(f) * n + q[l].oc] = ring[slot];
+                    for (int k = 0; bulk && k < kPairSteps; ++k) {
+                        const int t = t0 + k;
+                        if (t == 0 || t > c[l].made_lim || t > c[l].made2_lim ||
+                            t > c[l].flush_lim) {
+                            std::fprintf(stderr, "check-free block with a check due: step %d lane %d\n",
+                                         t, l);
+                            std::abort();
+                        }
+                    }
+                }
+                for (int k = 0; k < kPairSteps && t0 + k < R; ++k) {
+                    const int t = t0 + k, y = R - 1 - t;
+                    T send[kLanes];
+                    for (int l = 0; l < kLanes; ++l) {
+                        const size_t at = size_t(y) * n + q[l].p.pc;
+                        const T p0 = src[at], p1 = src[at + 1];
+                        send[l] = bulk ? quad_stage_k<false, T>(c[l], t, p0, p1, st[l])
+                                       : quad_stage_k<true, T>(c[l], t, p0, p1, st[l]);
+                    }
+                    for (int l = 0; l < kLanes; ++l) {
+                        const T v2 = bulk ? quad_stage_k1<false, T>(c[l], t, send[l ^ 1], st[l])
+                                          : quad_stage_k1<true, T>(c[l], t, send[l ^ 1], st[l]);
+                        ring.at(size_t(b[l].w + 16 * k)) = v2;
+                        tag.at(size_t(b[l].w + 16 * k)) = t;
+                        if (mirrored) {
+                            ring.at(size_t(b[l].m + 16 * k)) = v2;
+                            tag.at(size_t(b[l].m + 16 * k)) = t;
+                        }
+                        const size_t e = size_t(b[l].r + 16 * k);
+                        const T ov = ring.at(e);
+                        if (t > c[l].flush_lim) continue;
+                        const int want = t - c[l].delay;
+                        if (want >= 0 ? tag[e] != want : tag[e] != kZero) {
+                            std::fprintf(stderr, "stale quad flush: step %d lane %d\n", t, l);
+                            std::abort();
+                        }
+                        const int f = y - q[l].minlag;
+                        size_t at = size_t(f) * n + q[l].oc;
+                        if (bulk) {  // the kernel's row y - mlmax plus the lane's offset
+                            const size_t at2 = size_t(y - mlmax) * n + quad_store_off(q[l], n, mlmax);
+                            if (y < mlmax || at2 != at) std::abort();
+                        }
+                        dst[at] = ov;
+                    }
                 }
             }
             for (int l = 0; l < kLanes; ++l)
@@ -214,7 +252,7 @@ template <typename T>
 static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int planes, int R, int n,
                          int Li0, int S) {
     const int waves = (n + kLanes - 1) / kLanes;
-    const int P = kLanes + 1;
+    const int P = kPitch;
     const int M0 = tail_ring_rows(S, 0);
     std::vector<T> rbuf(size_t(tail_rows_total(S)) * P);
     std::vector<int> tag(rbuf.size());
