@@ -16,7 +16,7 @@ namespace fused {
 template <typename T, int KB>
 __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
                                amdgpu_waves_per_eu(lds_waves_per_simd<Pass1<T, KB>, T>(
-                                   sizeof(T) == 4 ? 8 : 4))))
+                                   sizeof(T) == 4 ? Pass1<T, KB>::kSimdWaves : 4))))
 fwd_pass1(const T *__restrict__ img_base, T *__restrict__ X_base, int n, int first_image,
           int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];

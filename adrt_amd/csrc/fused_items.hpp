@@ -389,6 +389,10 @@ __host__ __device__ __forceinline__ int x_pitch(int n, int k1) {
 //   phase 1 (level B): strip mA (8 of them): LDS rows (g, mA) skewed by
 //     mA*g, base c0 - kLead -> X[J][mA*2^KB + tB][c] for c in [c0, c0+kOut)
 // With KB == 0 phase 0 writes X directly and phase 1 is empty.
+// With KB == 3 (64 rows) the LDS rows go in two parts of tA in [0, 4) and
+// [4, 8): strips mA = 0..3 load the first, then every strip stores its
+// second half over it and strips 4..7 load that (40 KB of LDS in fp32
+// instead of 80, so three work-groups fit per CU instead of two).
 // ---------------------------------------------------------------------------
 template <typename T, int KB>
 struct Pass1 {
@@ -400,7 +404,14 @@ struct Pass1 {
     static constexpr int R0 = RA, R1 = RB;
     static constexpr int tasks0 = RB;
     static constexpr int tasks1 = KB == 0 ? 0 : RA;
-    static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) * kRow;
+    static constexpr int kParts = KB == 3 ? 2 : 1;     // LDS parts of the level A -> B rows
+    static constexpr int kPartRows = RA / kParts;      // rows tA of a strip per part
+    static constexpr size_t lds_elems = KB == 0 ? 0 : size_t(1 << k1) / kParts * kRow;
+    // waves per SIMD for the register budget: three 8-wave work-groups per CU
+    // with parts (80 VGPRs; at 64, four work-groups, it spills 9 registers and
+    // measured 69 against 50 us per 2-image group at 2048^2), else the budget
+    // the LDS allows
+    static constexpr int kSimdWaves = kParts > 1 ? 6 : 8;
 
 #ifndef ADRT_HOST_EMULATION
     __device__ __forceinline__ static void strip0(T (&v)[R0][kW]) { strip_adrt<KA, T>(v); }
@@ -557,29 +568,40 @@ struct Pass1 {
         }
     }
 
+    // Rows tA of part `part` of strip g: LDS row g * kPartRows + tA % kPartRows.
+    __host__ __device__ __forceinline__ static void store0_part(const Item &it, int g, int lane,
+                                                                const T (&o)[RA][kW], T *lds,
+                                                                int part) {
+        (void)it;
+#pragma unroll
+        for (int u = 0; u < kPartRows; ++u)
+#pragma unroll
+            for (int i = 0; i < kW; ++i)
+                lds[(g * kPartRows + u) * kRow + lds_uniform(i) + lane] = o[part * kPartRows + u][i];
+    }
+
     __host__ __device__ __forceinline__ static void store0(const Item &it, int g, int lane,
                                                            const T (&o)[RA][kW], T *lds) {
         if constexpr (KB == 0) {
             (void)lds;
             store_x<RA>(it, g, lane, o);
         } else {
-#pragma unroll
-            for (int t = 0; t < RA; ++t)
-#pragma unroll
-                for (int i = 0; i < kW; ++i) lds[(g * RA + t) * kRow + lds_uniform(i) + lane] = o[t][i];
+            static_assert(kParts == 1, "parted rows are stored by store0_part");
+            store0_part(it, g, lane, o, lds, 0);
         }
     }
 
     // Level B input: LDS rows (g, mA), skewed by mA*g.  Strip column of
     // offset c0 - kLead + 4*lane + i - mA*g relative to base0(g):
     // 4*lane + i - kLead + HB + 7 + (7 - mA)*g, always in [-kPad, kStrip).
+    // With parts, the LDS holds part mA / kPartRows.
     __host__ __device__ __forceinline__ static void load1(const Item &it, int mA, int lane,
                                                           const T *lds, T (&v)[RB][kW]) {
         (void)it;
 #pragma unroll
         for (int g = 0; g < RB; ++g) {
             const int ub = -kLead + HB + (RA - 1) + (RA - 1 - mA) * g;
-            const T *row = lds + (g * RA + mA) * kRow + lane;
+            const T *row = lds + (g * kPartRows + mA % kPartRows) * kRow + lane;
 #pragma unroll
             for (int i = 0; i < kW; ++i) v[g][i] = row[lds_uniform(ub + i)];
         }
@@ -608,6 +630,8 @@ struct Pass1 {
 template <typename T, int KD>
 struct Pass2 {
     static constexpr bool kStaged = false;
+    static constexpr int kParts = 1;
+    static constexpr int kSimdWaves = 8;
     static constexpr int KC = 3;
     static constexpr int k2 = KC + KD;
     static constexpr int RC = 1 << KC, RD = 1 << KD;
@@ -800,6 +824,8 @@ __host__ __device__ __forceinline__ int lds_col(int u) { return (u & (kW - 1)) *
 template <typename T, int K2>
 struct BdrtA {
     static constexpr int kA = 3 + K2;
+    static constexpr int kParts = 1;
+    static constexpr int kSimdWaves = 8;
     static constexpr int R0 = 8, R1 = 1 << K2;
     static constexpr int tasks0 = R1;                // level-1 strips g
     static constexpr int tasks1 = K2 == 0 ? 0 : 8;  // level-2 strips s1
@@ -1013,6 +1039,8 @@ struct BdrtA {
 
 template <typename T, int K4>
 struct BdrtB {
+    static constexpr int kParts = 1;
+    static constexpr int kSimdWaves = 8;
     static constexpr bool kStaged = false;
     static constexpr int kB = 3 + K4;
     static constexpr int R0 = 8, R1 = 1 << K4;
@@ -1176,6 +1204,27 @@ __device__ __forceinline__ void run_item_at(int idx, T *lds, A... args) {
             P::strip0(v);
             P::store0(it, wave, lane, v, lds);
         }
+    } else if constexpr (P::kParts == 2) {
+        // level A in every strip; the level B strips of part 0 load their
+        // rows, every strip stores its part-1 rows over them, then the level
+        // B strips of part 1 load (a strip holds its part-1 rows and its
+        // level B rows together for one barrier only)
+        static_assert(P::tasks0 == kWaves && P::tasks1 == kWaves, "one strip per wave");
+        T v[P::R0][kW];
+        P::load0(it, wave, lane, v);
+        P::strip0(v);
+        P::store0_part(it, wave, lane, v, lds, 0);
+        __syncthreads();
+        T w[P::R1][kW];
+        const bool first = wave < P::kPartRows;  // level B strip of part 0
+        if (first) P::load1(it, wave, lane, lds, w);
+        __syncthreads();
+        P::store0_part(it, wave, lane, v, lds, 1);
+        __syncthreads();
+        if (!first) P::load1(it, wave, lane, lds, w);
+        P::strip1(w);
+        P::store1(it, wave, lane, w, lds);
+        return;
     } else if (wave < P::tasks0) {
         T v[P::R0][kW];
         P::load0(it, wave, lane, v);
@@ -1204,7 +1253,7 @@ __device__ __forceinline__ void run_item(int items, T *lds, A... args) {
 
 // Register budget: waves per SIMD the kernel's LDS allows (160 KB per CU,
 // at most 4 work-groups per CU), capped by what the strips' registers need
-// without spilling (`cap` waves: 64, 128 or 256 VGPRs).
+// without spilling (`cap` waves: 64, 80, 128 or 256 VGPRs).
 template <typename P, typename T>
 constexpr int lds_waves_per_simd(int cap) {
     constexpr size_t bytes = P::lds_elems * sizeof(T);

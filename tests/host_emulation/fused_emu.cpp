@@ -60,6 +60,30 @@ static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args.
                     P::strip0_host(v0);
                     for (int lane = 0; lane < kLanes; ++lane) P::store0(it, task, lane, v0[lane], l);
                 }
+            } else if constexpr (P::kParts == 2) {
+                // the kernel's two parts: level A everywhere; part 0 rows, the
+                // first strips' level B loads; part 1 rows over them (LDS
+                // poisoned in between), the other strips' loads
+                std::vector<T> keepA(size_t(P::tasks0) * buf0.size());
+                for (int task = 0; task < P::tasks0; ++task) {
+                    for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
+                    P::strip0_host(v0);
+                    std::memcpy(keepA.data() + size_t(task) * buf0.size(), buf0.data(), buf0.size() * sizeof(T));
+                }
+                for (int part = 0; part < 2; ++part) {
+                    for (auto &x : lds) x = T(NAN);
+                    for (int task = 0; task < P::tasks0; ++task) {
+                        std::memcpy(buf0.data(), keepA.data() + size_t(task) * buf0.size(), buf0.size() * sizeof(T));
+                        for (int lane = 0; lane < kLanes; ++lane)
+                            P::store0_part(it, task, lane, v0[lane], l, part);
+                    }
+                    for (int task = part * P::kPartRows; task < (part + 1) * P::kPartRows; ++task) {
+                        for (int lane = 0; lane < kLanes; ++lane) P::load1(it, task, lane, l, v1[lane]);
+                        P::strip1_host(v1);
+                        for (int lane = 0; lane < kLanes; ++lane) P::store1(it, task, lane, v1[lane], l);
+                    }
+                }
+                continue;
             } else {
                 for (int task = 0; task < P::tasks0; ++task) {
                     for (int lane = 0; lane < kLanes; ++lane) P::load0(it, task, lane, v0[lane]);
