@@ -631,7 +631,6 @@ template <typename T, int KD>
 struct Pass2 {
     static constexpr bool kStaged = false;
     static constexpr int kParts = 1;
-    static constexpr int kSimdWaves = 8;
     static constexpr int KC = 3;
     static constexpr int k2 = KC + KD;
     static constexpr int RC = 1 << KC, RD = 1 << KD;
@@ -825,7 +824,6 @@ template <typename T, int K2>
 struct BdrtA {
     static constexpr int kA = 3 + K2;
     static constexpr int kParts = 1;
-    static constexpr int kSimdWaves = 8;
     static constexpr int R0 = 8, R1 = 1 << K2;
     static constexpr int tasks0 = R1;                // level-1 strips g
     static constexpr int tasks1 = K2 == 0 ? 0 : 8;  // level-2 strips s1
@@ -1040,7 +1038,6 @@ struct BdrtA {
 template <typename T, int K4>
 struct BdrtB {
     static constexpr int kParts = 1;
-    static constexpr int kSimdWaves = 8;
     static constexpr bool kStaged = false;
     static constexpr int kB = 3 + K4;
     static constexpr int R0 = 8, R1 = 1 << K4;
