@@ -181,10 +181,11 @@ def test_iadrt_fused_tail_emulation(iadrt_emulator, tmp_path, dt, n, tail):
     stage, LDS rings between stages, each later stage one row behind) run on
     the CPU with every ring read checked against the row it must hold, equals
     the oracle byte for byte, for every tail length either dtype allows.  At
-    n = 1024 the planes have 16 waves, so the early stages run with the
-    paired (XCD) block order, checked to be a bijection."""
+    n = 1024 (16 waves per plane) the stages before the tail run as
+    two-stage passes (or, with no tail, as two-stage passes and one paired
+    stage)."""
     if n == 1024 and dt == "float64":
-        pytest.skip("one dtype suffices for the block order at this size")
+        pytest.skip("one dtype suffices at this size (float64 runs at 512 and 2048 elsewhere)")
     rng = np.random.default_rng(n + tail)
     b = 2 if n <= 16 else 1
     x = (rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)
