@@ -185,7 +185,7 @@ def test_iadrt_fused_tail_emulation(iadrt_emulator, tmp_path, dt, n, tail):
     two-stage passes (or, with no tail, as two-stage passes and one paired
     stage)."""
     if n == 1024 and dt == "float64":
-        pytest.skip("one dtype suffices at this size (float64 runs at 512 and 2048 elsewhere)")
+        pytest.skip("one dtype suffices at this size (float64 two-stage passes: 128 and 512 below)")
     rng = np.random.default_rng(n + tail)
     b = 2 if n <= 16 else 1
     x = (rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)
