@@ -373,6 +373,180 @@ __host__ __device__ __forceinline__ T quad_stage_k1(const QuadConst &c, int t, T
     return v;
 }
 
+// Wide two-stage pass (float32, stage k with Co >= 64): grid = planes *
+// (n / 128) single-wave work-groups.  A wave covers stage-k columns c0 ..
+// c0 + 63 of the group pair pl and lane j (< 32) holds two stage-k chains of
+// the even group (P = column 2col', Q = 2col' + 1, col' = c0/2 + j), lane
+// 32 + j the same two of the odd group; each lane loads the four source
+// values of its two pairs as one 16-byte vector.  The lane then holds both
+// sources of its stage-(k+1) chains E and F, so no value crosses lanes:
+//   lanes 0-31:  E = type A: a = P now,  b = Q one step ago
+//                F = type B: a = Q now,  b = P now
+//   lanes 32-63: E = type C: a = P now,  b = Q two steps ago
+//                F = type D: a = Q one step ago, b = P now
+// (the operand timing of the quad kernel, within one lane).  Each type's 32
+// output columns are consecutive, so a flush store writes whole 128-byte
+// lines (in the quad kernel a type's 16 columns are half a line, the other
+// half written by the neighbouring wave 16-48 steps later).  The delay lines
+// (per lane, one per chain E / F) span 0, 62, 31 and 93 steps for A, C, B, D:
+// A 16 slots, C 64, B 32, D 128, plus 16 mirrored slots for C, B, D; a
+// region is [slot][32 lanes], and C and D start 32 elements (mod 64) after A
+// and B, so the two halves of a wave's access fall in distinct banks.
+constexpr int kWideRing = 4704 + (128 + 16) * 32;  // ring elements (D's region ends it)
+constexpr int kWideAhead = 16;  // source vectors in the register ring (loads 15 steps ahead)
+
+__host__ __device__ constexpr int wide_slots(int type) {  // 0 A, 1 B, 2 C, 3 D
+    return type == 0 ? 16 : (type == 1 ? 32 : (type == 2 ? 64 : 128));
+}
+__host__ __device__ constexpr int wide_region(int type) {
+    return type == 0 ? 0 : (type == 2 ? 544 : (type == 1 ? 3136 : 4704));
+}
+
+struct WideLane {
+    bool odd;          // lanes 32-63: the odd stage-k group (types C, D)
+    int j;             // lane & 31
+    int colq;          // stage-(k+1) column c0/2 + j
+    int src;           // first of the four source columns (2 * 2col')
+    int ocE, ocF;      // output columns of E and F
+    int lagE, lagF;    // output row = y - lag
+    int minE, minF;    // smallest lag of each type in the wave (flush row = y - min)
+};
+
+__host__ __device__ inline WideLane make_wide_lane(int n, int Li, int wave_in_plane, int lane,
+                                                   int &c0) {
+    const int Ci = n / Li, Co = Ci / 2, Coq = Co / 2;
+    const int wpg = Co / kLanes;  // waves per group pair
+    const int pl = wave_in_plane / wpg;
+    c0 = (wave_in_plane % wpg) * kLanes;
+    const int cm = c0 / 2;
+    WideLane w;
+    w.odd = lane >= kHalf;
+    w.j = lane & (kHalf - 1);
+    w.colq = cm + w.j;
+    w.src = pl * Ci + 4 * w.colq;
+    w.ocE = (4 * pl + (w.odd ? 2 : 0)) * Coq + w.colq;
+    w.ocF = (4 * pl + (w.odd ? 3 : 1)) * Coq + w.colq;
+    w.lagE = w.odd ? 1 + 2 * w.colq : 0;
+    w.lagF = w.odd ? 2 + 3 * w.colq : 1 + w.colq;
+    w.minE = w.odd ? 1 + 2 * cm : 0;
+    w.minF = w.odd ? 2 + 3 * cm : 1 + cm;
+    return w;
+}
+
+// The wave's largest flush lags (types C and D): E stores address row
+// y - wide_mlE(c0), F row y - wide_mlF(c0), plus a per-lane offset.
+__host__ __device__ inline int wide_mlE(int c0) { return 1 + 2 * (c0 / 2); }
+__host__ __device__ inline int wide_mlF(int c0) { return 2 + 3 * (c0 / 2); }
+
+// Last iteration start t0 (two 16-step blocks, t0 .. t0 + 31) at which every
+// lane produces both stage-k values, both stage-(k+1) rows and both flush
+// rows: odd chains need y >= 2col' + 2 (col' <= c0/2 + 31), D chains y >= lag
+// (<= 3 c0/2 + 95), so t0 + 31 <= R - 96 - 3 c0/2; with t0 >= 16 as well
+// no check is needed.
+__host__ __device__ inline int wide_bulk_last(int R, int c0) { return R - 127 - 3 * (c0 / 2); }
+
+struct WideConst {
+    bool odd;
+    int limP, limQ;    // stage-k chains P / Q produce a value while t <= lim
+    int limE, limF;    // stage-(k+1) rows exist while t <= lim
+    int flushE, flushF;  // flush rows exist while t <= flush
+    int colE, colF;    // element of slot 0 of each delay line
+    int maskE, maskF;  // slots - 1
+    int dE, dF;        // delays
+    int mirE, mirF;    // elements from slot s to its mirror (0: none)
+};
+
+__host__ __device__ inline WideConst make_wide_const(const WideLane &w, int R) {
+    WideConst c;
+    c.odd = w.odd;
+    c.limP = w.odd ? R - 2 - 2 * w.colq : R;      // odd P: row y - 1 - 2col' >= 0
+    c.limQ = w.odd ? R - 3 - 2 * w.colq : R;      // odd Q: row y - 2 - 2col' >= 0
+    c.limE = R - 1 - w.lagE;
+    c.limF = R - 1 - w.lagF;
+    c.flushE = R - 1 - w.minE;
+    c.flushF = R - 1 - w.minF;
+    const int tE = w.odd ? 2 : 0, tF = w.odd ? 3 : 1;
+    c.colE = wide_region(tE) + w.j;
+    c.colF = wide_region(tF) + w.j;
+    c.maskE = wide_slots(tE) - 1;
+    c.maskF = wide_slots(tF) - 1;
+    c.dE = w.lagE - w.minE;
+    c.dF = w.lagF - w.minF;
+    c.mirE = w.odd ? wide_slots(tE) * 32 : 0;
+    c.mirF = wide_slots(tF) * 32;
+    return c;
+}
+
+struct WideBlock {
+    int wE, mE, rE, wF, mF, rF;  // write, mirror write, read elements of step 0 (step k: + 32 k)
+};
+
+__host__ __device__ inline WideBlock wide_block(const WideConst &c, int t0) {
+    WideBlock b;
+    const int sE = t0 & c.maskE, sF = t0 & c.maskF;
+    b.wE = c.colE + sE * 32;
+    b.mE = sE == 0 ? b.wE + c.mirE : b.wE;
+    b.rE = c.colE + ((t0 - c.dE) & c.maskE) * 32;
+    b.wF = c.colF + sF * 32;
+    b.mF = sF == 0 ? b.wF + c.mirF : b.wF;
+    b.rF = c.colF + ((t0 - c.dF) & c.maskF) * 32;
+    return b;
+}
+
+template <typename T>
+struct WideState {
+    T accP, accQ, prevP, prevQ;  // stage k: running values, the previous second elements
+    T hP, hQ0, hQ1, hQ2;         // stage-k values (+0 where none): P now, Q now and before
+    T accE, accF;                // stage k + 1
+};
+
+// One step of a lane from its source values (p0, p1, p2, p3) at row y, both
+// stages in the reference's order; returns E's and F's values (junk where
+// their row is negative: never flushed).
+template <bool kChecked, typename T>
+__host__ __device__ __forceinline__ void wide_step(const WideConst &c, int t, T p0, T p1, T p2,
+                                                   T p3, WideState<T> &s, T &vE, T &vF) {
+    // stage k: even  v = 0 + x0 - prev + acc (x0 only at t = 0), odd  v = 0 + x1 - x0 + acc
+    const T xP = c.odd ? p1 : p0, zP = c.odd ? p0 : s.prevP;
+    const T xQ = c.odd ? p3 : p2, zQ = c.odd ? p2 : s.prevQ;
+    const T uP = T(0) + xP, uQ = T(0) + xQ;
+    T vP = (uP - zP) + s.accP;
+    T vQ = (uQ - zQ) + s.accQ;
+    s.prevP = p1;
+    s.prevQ = p3;
+    if (kChecked) {
+        const bool full = c.odd || t != 0;
+        vP = full ? vP : uP;
+        vQ = full ? vQ : uQ;
+        const bool mP = t <= c.limP, mQ = t <= c.limQ;
+        s.accP = mP ? vP : s.accP;
+        s.accQ = mQ ? vQ : s.accQ;
+        vP = mP ? vP : T(0);
+        vQ = mQ ? vQ : T(0);
+    } else {
+        s.accP = vP;
+        s.accQ = vQ;
+    }
+    s.hP = vP;
+    s.hQ2 = s.hQ1;
+    s.hQ1 = s.hQ0;
+    s.hQ0 = vQ;
+    // stage k + 1: v = 0 + a - b + acc (A at t = 0: only a)
+    const T bE = c.odd ? s.hQ2 : s.hQ1;
+    const T aF = c.odd ? s.hQ1 : s.hQ0;
+    const T uE = T(0) + s.hP, uF = T(0) + aF;
+    vE = (uE - bE) + s.accE;
+    vF = (uF - s.hP) + s.accF;
+    if (kChecked) {
+        vE = (!c.odd && t == 0) ? uE : vE;
+        s.accE = t <= c.limE ? vE : s.accE;
+        s.accF = t <= c.limF ? vF : s.accF;
+    } else {
+        s.accE = vE;
+        s.accF = vF;
+    }
+}
+
 // Fused tail.  Once a stage's input groups are at most 64 columns wide
 // (Ci <= 64), one wavefront's 64 columns hold whole groups of that stage and
 // of every later one, so the last S stages run in one pass: the first of
@@ -797,6 +971,180 @@ __global__ void __launch_bounds__(kLanes)
     for (int r = R - q.minlag; r < R; ++r) dst_lane[size_t(r) * n] = T(0);  // the +0 rows
 }
 
+// The wide two-stage pass (float32; see WideLane), addressed and pipelined as
+// the quad kernel: scalar row addresses plus per-lane offsets, source loads
+// kWideAhead - 1 steps ahead in a register ring, flush stores at the end of
+// each 16-step block, three loops of 32-step iterations so that no loop body
+// branches.
+template <typename T, typename Off>
+__global__ void __launch_bounds__(kLanes)
+    wide_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
+                int waves_per_plane) {
+    using P4 = T __attribute__((ext_vector_type(4)));
+    using GP4 = const __attribute__((address_space(1))) P4;
+    using GT = __attribute__((address_space(1))) T;
+    using False = std::integral_constant<bool, false>;
+    using True = std::integral_constant<bool, true>;
+    __shared__ T ring[kWideRing];
+    const int lane = threadIdx.x;
+    const size_t plane = blockIdx.x / waves_per_plane;
+    int c0;
+    const WideLane wl = make_wide_lane(n, Li, int(blockIdx.x % waves_per_plane), lane, c0);
+    const WideConst c = make_wide_const(wl, R);
+    // wave-uniform (scalar) flush-row lags
+    const int mlE = __builtin_amdgcn_readfirstlane(wide_mlE(c0));
+    const int mlF = __builtin_amdgcn_readfirstlane(wide_mlF(c0));
+    const size_t row_bytes = size_t(n) * sizeof(T);
+    const GByte *gsrc = (const GByte *)(in + plane * size_t(R) * n);
+    GByte *gdst = (GByte *)(out + plane * size_t(R) * n);
+    T *dst_plane = out + plane * size_t(R) * n;
+    const uint32_t src_off = uint32_t(wl.src) * uint32_t(sizeof(T));
+    const Off offE = Off((size_t(mlE - wl.minE) * n + size_t(wl.ocE)) * sizeof(T));
+    const Off offF = Off((size_t(mlF - wl.minF) * n + size_t(wl.ocF)) * sizeof(T));
+    for (int e = lane; e < kWideRing; e += kLanes) ring[e] = T(0);
+    __syncthreads();
+
+    WideState<T> st;
+    st.accP = st.accQ = st.prevP = st.prevQ = st.hP = st.hQ0 = st.hQ1 = st.hQ2 = T(0);
+    st.accE = st.accF = T(0);
+    // step t0 + k: both stages, the delay-line writes, the reads of the flush values
+    auto step = [&](auto checked, auto mirrored, int t0, int k, const P4 &p, const WideBlock &b,
+                    T &oE, T &oF) {
+        constexpr bool kChecked = decltype(checked)::value;
+        T vE, vF;
+        wide_step<kChecked, T>(c, t0 + k, p.x, p.y, p.z, p.w, st, vE, vF);
+        ring[b.wE + 32 * k] = vE;
+        ring[b.wF + 32 * k] = vF;
+        if (decltype(mirrored)::value) {
+            ring[b.mE + 32 * k] = vE;
+            ring[b.mF + 32 * k] = vF;
+        }
+        oE = ring[b.rE + 32 * k];
+        oF = ring[b.rF + 32 * k];
+    };
+    // a check-free block's flushes: E rows R - 1 - t0 - k - minE (stored at
+    // row R - 1 - t0 - k - mlE plus the lane's offset), F likewise
+    auto flush_bulk = [&](int t0, const T *oE, const T *oF) {
+        Off e = offE, f = offF;
+        asm volatile("" : "+v"(e), "+v"(f));
+        GByte *rowE = gdst + size_t(R - 1 - t0 - mlE) * row_bytes;
+        GByte *rowF = gdst + size_t(R - 1 - t0 - mlF) * row_bytes;
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) {
+            if (k > 0) {
+                rowE -= row_bytes;
+                rowF -= row_bytes;
+                asm volatile("" : "+s"(rowE));
+                asm volatile("" : "+s"(rowF));
+            }
+            *reinterpret_cast<GT *>(rowE + e) = oE[k];
+            *reinterpret_cast<GT *>(rowF + f) = oF[k];
+        }
+    };
+    // a checked block's flushes, steps t0 .. t0 + count - 1
+    auto flush_checked = [&](int t0, int count, const T *oE, const T *oF) {
+#pragma unroll
+        for (int k = 0; k < kPairSteps; ++k) {
+            const int t = t0 + k;
+            if (k < count && t <= c.flushE)
+                dst_plane[size_t(R - 1 - t - wl.minE) * n + wl.ocE] = oE[k];
+            if (k < count && t <= c.flushF)
+                dst_plane[size_t(R - 1 - t - wl.minF) * n + wl.ocF] = oF[k];
+        }
+    };
+
+    // the register ring of source vectors, as in the quad kernel but 16
+    // vectors deep (64 registers; at 32 the allocator splits the slots)
+    P4 buf[kWideAhead];
+    {
+        uint32_t off = src_off;
+        asm volatile("" : "+v"(off));
+#pragma unroll
+        for (int k = 0; k < kWideAhead - 1; ++k) {
+            const int y = R - 1 - k;
+            buf[k] = *reinterpret_cast<GP4 *>(gsrc + size_t(y < 0 ? 0 : y) * row_bytes + off);
+        }
+    }
+    auto refill_rows = [&](auto clamped, int t0, auto body) {
+        const int y0 = R - 1 - t0 - (kWideAhead - 1);
+        uint32_t off = src_off;
+        asm volatile("" : "+v"(off));
+        const GByte *row = gsrc + size_t(y0 < 0 ? 0 : y0) * row_bytes;
+        auto refill = [&](int k) {
+            if (k > 0) {
+                if (decltype(clamped)::value)
+                    row = y0 - k >= 0 ? row - row_bytes : row;
+                else
+                    row -= row_bytes;
+                asm volatile("" : "+s"(row));
+            }
+            buf[(k + kWideAhead - 1) % kWideAhead] = *reinterpret_cast<GP4 *>(row + off);
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        body(refill);
+    };
+    auto checked_iteration = [&](int t0) {
+        refill_rows(True(), t0, [&](auto &refill) {
+#pragma unroll
+            for (int h = 0; h < kQuadAhead; h += kPairSteps) {
+                const WideBlock b = wide_block(c, t0 + h);
+                T oE[kPairSteps], oF[kPairSteps];
+#pragma unroll
+                for (int k = 0; k < kPairSteps; ++k) {
+                    step(True(), True(), t0 + h, k, buf[(h + k) % kWideAhead], b, oE[k], oF[k]);
+                    refill(h + k);
+                }
+                flush_checked(t0 + h, kPairSteps, oE, oF);
+            }
+        });
+    };
+    auto bulk_iteration = [&](int t0) {
+        refill_rows(False(), t0, [&](auto &refill) {  // rows y0 - 31 >= 0: see wide_bulk_last
+            T oE[kPairSteps], oF[kPairSteps];
+            WideBlock b = wide_block(c, t0);
+#pragma unroll
+            for (int k = 0; k < kPairSteps; ++k) {
+                step(False(), True(), t0, k, buf[k % kWideAhead], b, oE[k], oF[k]);
+                refill(k);
+            }
+            flush_bulk(t0, oE, oF);
+            b = wide_block(c, t0 + kPairSteps);
+#pragma unroll
+            for (int k = 0; k < kPairSteps; ++k) {
+                step(False(), False(), t0 + kPairSteps, k, buf[(kPairSteps + k) % kWideAhead], b,
+                     oE[k], oF[k]);
+                refill(kPairSteps + k);
+            }
+            flush_bulk(t0 + kPairSteps, oE, oF);
+        });
+    };
+    const int bulk_last = wide_bulk_last(R, c0);
+    int t0 = 0;
+    for (; t0 + kQuadAhead <= R && !(t0 >= kPairSteps && t0 <= bulk_last); t0 += kQuadAhead)
+        checked_iteration(t0);
+    for (; t0 + kQuadAhead <= R && t0 <= bulk_last; t0 += kQuadAhead) bulk_iteration(t0);
+    for (; t0 + kQuadAhead <= R; t0 += kQuadAhead) checked_iteration(t0);
+    // tail: steps t0 .. R - 1 (up to 31, more than the ring holds ahead, so
+    // the refills go on, clamped into the rows)
+    refill_rows(True(), t0, [&](auto &refill) {
+#pragma unroll
+        for (int h = 0; h < kQuadAhead; h += kPairSteps) {
+            const WideBlock b = wide_block(c, t0 + h);
+            T oE[kPairSteps], oF[kPairSteps];
+#pragma unroll
+            for (int k = 0; k < kPairSteps; ++k) {
+                if (t0 + h + k < R)
+                    step(True(), True(), t0 + h, k, buf[(h + k) % kWideAhead], b, oE[k], oF[k]);
+                refill(h + k);
+            }
+            flush_checked(t0 + h, R - t0 - h, oE, oF);
+        }
+    });
+    // the +0 rows above each type's first flushed row
+    for (int r = R - wl.minE; r < R; ++r) dst_plane[size_t(r) * n + wl.ocE] = T(0);
+    for (int r = R - wl.minF; r < R; ++r) dst_plane[size_t(r) * n + wl.ocF] = T(0);
+}
+
 // The last S stages (the first with Li0 groups, Ci = n / Li0 = 2^S), one
 // single-wave work-group per 64 columns of a plane.
 template <typename T, int S>
@@ -950,7 +1298,18 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
         T *dst = bufs[pass & 1];
         const bool paired = int(n) / Li / 2 >= iadrt_native::kHalf;  // Co >= 32
         if (paired && quad && k + 1 < k0) {  // stages k and k + 1 in one pass
-            if (iadrt_native::quad_narrow<T>(n))
+            bool launched = false;
+            if constexpr (sizeof(T) == 4) {  // Co >= 64: the wide pass
+                if (int(n) / Li / 2 >= iadrt_native::kLanes && iadrt_native::quad_narrow<T>(n)) {
+                    hipLaunchKernelGGL((iadrt_native::wide_kernel<T, uint32_t>),
+                                       dim3(unsigned(B * 4 * (n / 128))),
+                                       dim3(iadrt_native::kLanes), 0, s, src, dst, int(R),
+                                       int(n), Li, int(n / 128));
+                    launched = true;
+                }
+            }
+            if (launched) {
+            } else if (iadrt_native::quad_narrow<T>(n))
                 hipLaunchKernelGGL((iadrt_native::quad_kernel<T, uint32_t>), dim3(grid),
                                    dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
                                    Li, waves_per_plane);

@@ -126,6 +126,28 @@ static void emulate_pair_stage(const std::vector<T> &in, std::vector<T> &out, in
     }
 }
 
+// The register ring of source rows that the quad and wide kernels keep:
+// `depth` slots; the prologue loads steps 0 .. depth - 2 into slots 0 ..
+// depth - 2; each step t reads slot t % depth and then, where the kernel
+// refills (whole iterations; in the wide kernel the final partial one too),
+// refills slot (t - 1) % depth with step t + depth - 1.  `use` checks that
+// the slot read at step t holds step t.
+struct RowRing {
+    std::vector<int> step;
+    explicit RowRing(int depth) : step(size_t(depth), -1) {
+        for (int k = 0; k < depth - 1; ++k) step[size_t(k)] = k;
+    }
+    void use(int t, bool refill) {
+        const int d = int(step.size());
+        if (step[size_t(t % d)] != t) {
+            std::fprintf(stderr, "register ring: step %d reads slot %d holding step %d\n", t, t % d,
+                         step[size_t(t % d)]);
+            std::abort();
+        }
+        if (refill) step[size_t((t + d - 1) % d)] = t + d - 1;
+    }
+};
+
 // Two paired stages in one pass (the quad kernel): lanes in lockstep, in the
 // kernel's blocks of 16 steps (two per iteration of kQuadAhead steps, the
 // first with mirror writes where check-free); a check-free block runs the unchecked step
@@ -175,6 +197,7 @@ static void emulate_quad_stage(const std::vector<T> &in, std::vector<T> &out, in
             QuadState<T> st[kLanes];
             for (auto &x : st) x.acc = x.prev1 = x.h0 = x.h1 = x.h2 = x.acc2 = T(0);
             const int bulk_last = quad_bulk_last(R, c0);
+            RowRing rows(kQuadAhead);
             // iterations of kQuadAhead steps (two blocks), check-free as a whole
             // (the kernel's three loops take the iterations where this holds,
             // an interval of t0, in the middle one)
@@ -202,6 +225,7 @@ static void emulate_quad_stage(const std::vector<T> &in, std::vector<T> &out, in
                 }
                 for (int k = 0; k < kPairSteps && t0 + k < R; ++k) {
                     const int t = t0 + k, y = R - 1 - t;
+                    rows.use(t, it0 + kQuadAhead <= R);  // no refills in the final partial iteration
                     T send[kLanes];
                     for (int l = 0; l < kLanes; ++l) {
                         const size_t at = size_t(y) * n + q[l].p.pc;
@@ -238,6 +262,133 @@ static void emulate_quad_stage(const std::vector<T> &in, std::vector<T> &out, in
             }
             for (int l = 0; l < kLanes; ++l)
                 for (int r = R - q[l].minlag; r < R; ++r) dst[size_t(r) * n + q[l].oc] = T(0);
+        }
+    }
+}
+
+// The wide two-stage pass (float32, Co >= 64): lanes in lockstep in the
+// kernel's 32-step iterations of two 16-step blocks; a check-free iteration
+// runs the unchecked step only after verifying every condition it drops.
+// Each lane holds both chains of its two stage-(k+1) chains (no exchange).
+// The two delay lines per lane are the kernel's ring (its element indices and
+// mirror writes; blocks without mirror writes verify none was due), every
+// element tagged with the step that wrote it; each flush checks it reads the
+// value of step t - delay, or an unwritten +0 element when t - delay < 0.
+// Check-free blocks store through the kernel's addressing (row y - ml plus
+// the lane's offset, which must fit 32 bits).
+template <typename T>
+static void emulate_wide_stage(const std::vector<T> &in, std::vector<T> &out, int planes, int R,
+                               int n, int Li) {
+    const int waves = n / 128;
+    const int kZero = -(1 << 30);
+    if (!quad_narrow<T>(size_t(n))) std::abort();  // the library takes the wide pass only then
+    // regions in order, disjoint, inside the ring, C and D 32 elements (mod 64) after A and B
+    const int order[4] = {0, 2, 1, 3};
+    for (int i = 0; i < 3; ++i) {
+        const int t = order[i], u = order[i + 1];
+        if (wide_region(t) + 32 * (wide_slots(t) + (t ? 16 : 0)) > wide_region(u)) std::abort();
+    }
+    if (wide_region(3) + 32 * (wide_slots(3) + 16) != kWideRing) std::abort();
+    if ((wide_region(2) - wide_region(0)) % 64 != 32 || (wide_region(3) - wide_region(1)) % 64 != 32)
+        std::abort();
+    for (int plane = 0; plane < planes; ++plane) {
+        const T *src = in.data() + size_t(plane) * R * n;
+        T *dst = out.data() + size_t(plane) * R * n;
+        for (int wv = 0; wv < waves; ++wv) {
+            WideLane w[kLanes];
+            WideConst c[kLanes];
+            int c0 = 0;
+            for (int l = 0; l < kLanes; ++l) {
+                w[l] = make_wide_lane(n, Li, wv, l, c0);
+                c[l] = make_wide_const(w[l], R);
+                if (w[l].src < 0 || w[l].src + 3 >= n || w[l].src % 4 != 0 || w[l].ocE < 0 ||
+                    w[l].ocE >= n || w[l].ocF < 0 || w[l].ocF >= n)
+                    std::abort();
+            }
+            const int mlE = wide_mlE(c0), mlF = wide_mlF(c0);
+            for (int l = 0; l < kLanes; ++l) {
+                if (w[l].minE > mlE || w[l].minF > mlF) std::abort();
+                const size_t oE = (size_t(mlE - w[l].minE) * n + w[l].ocE) * sizeof(T);
+                const size_t oF = (size_t(mlF - w[l].minF) * n + w[l].ocF) * sizeof(T);
+                if (oE > 0xffffffffull || oF > 0xffffffffull) std::abort();
+            }
+            std::vector<T> ring(kWideRing, T(0));
+            std::vector<int> tag(kWideRing, kZero);
+            WideState<T> st[kLanes];
+            for (auto &x : st) {
+                x.accP = x.accQ = x.prevP = x.prevQ = x.hP = x.hQ0 = x.hQ1 = x.hQ2 = T(0);
+                x.accE = x.accF = T(0);
+            }
+            const int bulk_last = wide_bulk_last(R, c0);
+            RowRing rows(kWideAhead);
+            auto put = [&](size_t e, T v, int t) {
+                ring.at(e) = v;
+                tag.at(e) = t;
+            };
+            for (int t0 = 0; t0 < R; t0 += kPairSteps) {
+                const int it0 = t0 - t0 % kQuadAhead;
+                const bool bulk = it0 + kQuadAhead <= R && it0 >= kPairSteps && it0 <= bulk_last;
+                const bool mirrored = !bulk || t0 == it0;
+                WideBlock b[kLanes];
+                for (int l = 0; l < kLanes; ++l) {
+                    b[l] = wide_block(c[l], t0);
+                    if (!mirrored && (b[l].mE != b[l].wE || b[l].mF != b[l].wF)) {
+                        std::fprintf(stderr, "wide mirror write skipped: step %d lane %d\n", t0, l);
+                        std::abort();
+                    }
+                    for (int k = 0; bulk && k < kPairSteps; ++k) {
+                        const int t = t0 + k;
+                        if (t == 0 || t > c[l].limP || t > c[l].limQ || t > c[l].limE ||
+                            t > c[l].limF || t > c[l].flushE || t > c[l].flushF) {
+                            std::fprintf(stderr, "wide check-free block with a check due: step %d lane %d\n",
+                                         t, l);
+                            std::abort();
+                        }
+                    }
+                }
+                for (int k = 0; k < kPairSteps && t0 + k < R; ++k) {
+                    const int t = t0 + k, y = R - 1 - t;
+                    rows.use(t, true);
+                    for (int l = 0; l < kLanes; ++l) {
+                        const T *p = src + size_t(y) * n + w[l].src;
+                        T vE, vF;
+                        if (bulk)
+                            wide_step<false, T>(c[l], t, p[0], p[1], p[2], p[3], st[l], vE, vF);
+                        else
+                            wide_step<true, T>(c[l], t, p[0], p[1], p[2], p[3], st[l], vE, vF);
+                        put(size_t(b[l].wE + 32 * k), vE, t);
+                        put(size_t(b[l].wF + 32 * k), vF, t);
+                        if (mirrored) {
+                            put(size_t(b[l].mE + 32 * k), vE, t);
+                            put(size_t(b[l].mF + 32 * k), vF, t);
+                        }
+                        const int line[2][4] = {{b[l].rE, c[l].flushE, c[l].dE, w[l].minE},
+                                                {b[l].rF, c[l].flushF, c[l].dF, w[l].minF}};
+                        const int oc[2] = {w[l].ocE, w[l].ocF}, ml[2] = {mlE, mlF};
+                        for (int h = 0; h < 2; ++h) {
+                            const size_t e = size_t(line[h][0] + 32 * k);
+                            const T ov = ring.at(e);
+                            if (t > line[h][1]) continue;
+                            const int want = t - line[h][2];
+                            if (want >= 0 ? tag[e] != want : tag[e] != kZero) {
+                                std::fprintf(stderr, "stale wide flush: step %d lane %d line %d\n", t, l, h);
+                                std::abort();
+                            }
+                            const size_t at = size_t(y - line[h][3]) * n + oc[h];
+                            if (bulk) {
+                                const size_t at2 = size_t(y - ml[h]) * n +
+                                                   (size_t(ml[h] - line[h][3]) * n + oc[h]);
+                                if (y < ml[h] || at2 != at) std::abort();
+                            }
+                            dst[at] = ov;
+                        }
+                    }
+                }
+            }
+            for (int l = 0; l < kLanes; ++l) {
+                for (int r = R - w[l].minE; r < R; ++r) dst[size_t(r) * n + w[l].ocE] = T(0);
+                for (int r = R - w[l].minF; r < R; ++r) dst[size_t(r) * n + w[l].ocF] = T(0);
+            }
         }
     }
 }
@@ -362,7 +513,10 @@ static int run(int B, int n, const char *fin, const char *fout, int tail, bool q
     for (int k = 0; k < k0;) {
         const bool paired = n / Li / 2 >= kHalf;
         if (paired && quad && k + 1 < k0) {
-            emulate_quad_stage<T>(a, b, B * 4, R, n, Li);
+            if (sizeof(T) == 4 && n / Li / 2 >= kLanes && quad_narrow<T>(size_t(n)))
+                emulate_wide_stage<T>(a, b, B * 4, R, n, Li);  // the library's choice
+            else
+                emulate_quad_stage<T>(a, b, B * 4, R, n, Li);
             k += 2;
             Li *= 4;
         } else {
