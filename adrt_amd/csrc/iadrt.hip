@@ -571,6 +571,19 @@ __host__ __device__ constexpr int pow2_at_least(int v) {
     while (p < v) p <<= 1;
     return p;
 }
+// Highest step s0 at which a block s0 .. s0 - 15 of the tail runs without
+// checks: every later stage t's row x = s + t has x + 1 < R, and an odd
+// chain's operands (row x + 1 + col, col < Co_t = 2^(S-1-t)) lie inside the
+// rows, so every operand and the running value enter each addition:
+// s + t + Co_t < R for every t.
+__host__ __device__ constexpr int tail_free_top(int R, int S) {
+    int m = 0;
+    for (int t = 1; t < S; ++t) {
+        const int v = t + (1 << (S - 1 - t));
+        m = v > m ? v : m;
+    }
+    return R - 1 - m;
+}
 // rows of the ring stage t (0 <= t <= S - 2) writes for stage t + 1
 __host__ __device__ constexpr int tail_ring_rows(int S, int t) {
     return t == 0 ? pow2_at_least(3 * (1 << (S - 1)) / 2 + 1)
@@ -1201,7 +1214,10 @@ __global__ void __launch_bounds__(kLanes)
         for (int t = 1; t < S; ++t) {
             const int x = sk + t;
             if (kChecked && (x < 0 || x >= R)) continue;
-            const T v = tail_combine<T>(o[t - 1], a[t - 1], b[t - 1], acc[t - 1], x + 1 < R);
+            // unchecked: both operands and acc enter (see tail_free_top)
+            const T v = kChecked
+                            ? tail_combine<T>(o[t - 1], a[t - 1], b[t - 1], acc[t - 1], x + 1 < R)
+                            : ((T(0) + a[t - 1]) - b[t - 1]) + acc[t - 1];
             acc[t - 1] = v;
             if (t < S - 1)
                 rbuf[(tail_ring_base(S, t) + (x & (tail_ring_rows(S, t) - 1))) * kPitch +
@@ -1224,7 +1240,7 @@ __global__ void __launch_bounds__(kLanes)
         P2 nxt[kTailSteps];
 #pragma unroll
         for (int k = 0; k < kTailSteps; ++k) nxt[k] = load(s - kTailSteps - k);
-        if (s <= R - S && s - (kTailSteps - 1) >= 0) {  // rows s+1 .. s+S-1 inside
+        if (s <= tail_free_top(R, S) && s - (kTailSteps - 1) >= 0) {
 #pragma unroll
             for (int k = 0; k < kTailSteps; ++k) run_step(s - k, cur[k], Unchecked{});
         } else {

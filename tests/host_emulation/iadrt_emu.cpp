@@ -434,12 +434,24 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
             for (auto &x : tag) x = -1 << 20;
             // the kernel's blocks of kTailSteps steps: a whole block that it
             // runs without range checks must have every later stage's row
-            // inside [0, R)
+            // inside [0, R), with x + 1 < R and both operands of every live
+            // lane entering (the unchecked combine adds them all)
             for (int sb = w.top; sb - (kTailSteps - 1) >= -(S - 1); sb -= kTailSteps)
-                if (sb <= R - S && sb - (kTailSteps - 1) >= 0)
+                if (sb <= tail_free_top(R, S) && sb - (kTailSteps - 1) >= 0)
                     for (int k = 0; k < kTailSteps; ++k)
-                        for (int t = 1; t < S; ++t)
-                            if (sb - k + t < 0 || sb - k + t >= R) std::abort();
+                        for (int t = 1; t < S; ++t) {
+                            const int x = sb - k + t;
+                            if (x < 0 || x + 1 >= R) std::abort();
+                            for (int l = 0; l < kLanes; ++l) {
+                                const TailLane &u = tl[size_t(t) * kLanes + l];
+                                const TailOperands o = tail_operands(u, x, 0, 0);
+                                if (u.live && (!o.use_a || !o.use_b)) {
+                                    std::fprintf(stderr, "tail unchecked block drops an operand: x %d lane %d\n",
+                                                 x, l);
+                                    std::abort();
+                                }
+                            }
+                        }
             for (int s = w.top; s >= -(S - 1); --s) {
                 std::vector<TailOperands> ops(size_t(S) * kLanes);
                 std::vector<T> va(size_t(S) * kLanes), vb(size_t(S) * kLanes);
