@@ -584,6 +584,13 @@ __host__ __device__ constexpr int tail_free_top(int R, int S) {
     }
     return R - 1 - m;
 }
+// Whether the tail's block of steps s0 .. s0 - 15 runs without checks: the
+// later stages as above, and every stage-0 row s - lag in [0, R - 1) with its
+// source row s + roff < R (s0 - 15 >= maxlag); all the wave's columns must
+// exist as well.
+__host__ __device__ constexpr bool tail_block_free(int s0, int R, int S, int maxlag) {
+    return s0 <= tail_free_top(R, S) && s0 - (kTailSteps - 1) - maxlag >= 0;
+}
 // rows of the ring stage t (0 <= t <= S - 2) writes for stage t + 1
 __host__ __device__ constexpr int tail_ring_rows(int S, int t) {
     return t == 0 ? pow2_at_least(3 * (1 << (S - 1)) / 2 + 1)
@@ -1160,7 +1167,7 @@ __global__ void __launch_bounds__(kLanes)
 
 // The last S stages (the first with Li0 groups, Ci = n / Li0 = 2^S), one
 // single-wave work-group per 64 columns of a plane.
-template <typename T, int S>
+template <typename T, int S, bool kAllLive>
 __global__ void __launch_bounds__(kLanes)
     tail_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li0,
                 int waves_per_plane) {
@@ -1187,9 +1194,9 @@ __global__ void __launch_bounds__(kLanes)
     T acc[S - 1];
 #pragma unroll
     for (int t = 0; t < S - 1; ++t) acc[t] = T(0);
-    // kChecked: some later stage's row sk + t may lie outside [0, R) (the
-    // first and last steps); a block of steps with every stage inside runs
-    // without the per-stage range branches
+    // kChecked: some stage's row may lie outside the rows (the first and last
+    // steps); a block with every stage inside (tail_block_free) runs without
+    // the range branches and selects
     auto run_step = [&](int sk, const P2 &p, auto checked) {
         constexpr bool kChecked = decltype(checked)::value;
         // every later stage's operands first (rows written at earlier steps),
@@ -1204,10 +1211,17 @@ __global__ void __launch_bounds__(kLanes)
             b[t - 1] = rbuf[o[t - 1].ib];
         }
         // stage 0 (the step function on the loaded pair)
-        T val;
-        int row;
-        if (step<T>(w, ln, sk, p.x, p.y, prev1, acc0, val, row))
-            rbuf[(row & (kM0 - 1)) * kPitch + lane] = val;
+        if constexpr (kChecked) {
+            T val;
+            int row;
+            if (step<T>(w, ln, sk, p.x, p.y, prev1, acc0, val, row))
+                rbuf[(row & (kM0 - 1)) * kPitch + lane] = val;
+        } else {  // the lane's row sk - lag: even  0 + p0 - prev1 + acc, odd  0 + p1 - p0 + acc
+            const T x0 = ln.odd ? p.y : p.x, z0 = ln.odd ? p.x : prev1;
+            const T v0 = ((T(0) + x0) - z0) + acc0;
+            acc0 = v0;
+            rbuf[((sk - ln.lag) & (kM0 - 1)) * kPitch + lane] = v0;
+        }
         prev1 = p.y;
         // later stages: row x = sk + t is wave-uniform
 #pragma unroll
@@ -1222,7 +1236,7 @@ __global__ void __launch_bounds__(kLanes)
             if (t < S - 1)
                 rbuf[(tail_ring_base(S, t) + (x & (tail_ring_rows(S, t) - 1))) * kPitch +
                      lane] = v;
-            else if (tl[t - 1].live)
+            else if (kAllLive || tl[t - 1].live)
                 dst[size_t(x) * n] = v;
         }
     };
@@ -1240,7 +1254,7 @@ __global__ void __launch_bounds__(kLanes)
         P2 nxt[kTailSteps];
 #pragma unroll
         for (int k = 0; k < kTailSteps; ++k) nxt[k] = load(s - kTailSteps - k);
-        if (s <= tail_free_top(R, S) && s - (kTailSteps - 1) >= 0) {
+        if (kAllLive && tail_block_free(s, R, S, w.maxlag)) {
 #pragma unroll
             for (int k = 0; k < kTailSteps; ++k) run_step(s - k, cur[k], Unchecked{});
         } else {
@@ -1353,12 +1367,22 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
         T *dst = bufs[pass & 1];
         const dim3 g(grid), blk(iadrt_native::kLanes);
         const int r = int(R), nn = int(n);
-        switch (S) {
-            case 2: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 2>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
-            case 3: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 3>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
-            case 4: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 4>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
-            case 5: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 5>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
-            default: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 6>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+        // every column of every wave exists when n is a multiple of 64
+        if (n % iadrt_native::kLanes == 0) {
+            switch (S) {
+                case 2: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 2, true>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                case 3: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 3, true>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                case 4: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 4, true>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                case 5: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 5, true>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                default: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 6, true>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+            }
+        } else {  // n < 64: one wave with n live lanes, L = log2 n <= 5 stages
+            switch (S) {
+                case 2: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 2, false>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                case 3: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 3, false>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                case 4: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 4, false>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+                default: hipLaunchKernelGGL((iadrt_native::tail_kernel<T, 5, false>), g, blk, 0, s, src, dst, r, nn, Li, waves_per_plane); break;
+            }
         }
     }
     return launch_status();

@@ -436,9 +436,19 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
             // runs without range checks must have every later stage's row
             // inside [0, R), with x + 1 < R and both operands of every live
             // lane entering (the unchecked combine adds them all)
+            const bool all_live = n % kLanes == 0;
             for (int sb = w.top; sb - (kTailSteps - 1) >= -(S - 1); sb -= kTailSteps)
-                if (sb <= tail_free_top(R, S) && sb - (kTailSteps - 1) >= 0)
-                    for (int k = 0; k < kTailSteps; ++k)
+                if (all_live && tail_block_free(sb, R, S, w.maxlag))
+                    for (int k = 0; k < kTailSteps; ++k) {
+                        const int sk = sb - k;
+                        for (int l = 0; l < kLanes; ++l) {  // the branch-free stage 0
+                            const int row = sk - ln[l].lag;
+                            if (!ln[l].live || row < 0 || row + 1 >= R || sk + ln[l].roff >= R) {
+                                std::fprintf(stderr, "tail unchecked block with a stage-0 check due: s %d lane %d\n",
+                                             sk, l);
+                                std::abort();
+                            }
+                        }
                         for (int t = 1; t < S; ++t) {
                             const int x = sb - k + t;
                             if (x < 0 || x + 1 >= R) std::abort();
@@ -452,6 +462,7 @@ static void emulate_tail(const std::vector<T> &in, std::vector<T> &out, int plan
                                 }
                             }
                         }
+                    }
             for (int s = w.top; s >= -(S - 1); --s) {
                 std::vector<TailOperands> ops(size_t(S) * kLanes);
                 std::vector<T> va(size_t(S) * kLanes), vb(size_t(S) * kLanes);
