@@ -16,8 +16,20 @@
 // transposes.  One wavefront owns 64 consecutive output columns and sweeps
 // the rows downward, each lane carrying its chain's running value.
 //
-// Coalescing.  An even chain reads its src pair at rows s and s+1; an odd
-// chain at row r+1+col, which differs per lane.  Odd lane col therefore runs
+// Kernels, by the stages a pass runs (run_native picks them per stage):
+//   stage_kernel  one stage, any group width (below, and where paired
+//                 chains do not fit: Co < 32);
+//   pair_kernel   one stage with Co >= 32: the even and odd chains of 32
+//                 columns in one wave, each source pair loaded once;
+//   quad_kernel   two stages (stage k with Co >= 32): float64, and float32
+//                 where the wide pass does not apply (Co = 32, or store
+//                 offsets past 32 bits);
+//   wide_kernel   two stages (float32, Co >= 64), two chains per lane and
+//                 whole-line stores;
+//   tail_kernel   the last S <= 6 stages (groups of at most 64 columns).
+//
+// Coalescing (stage_kernel).  An even chain reads its src pair at rows s and
+// s+1; an odd chain at row r+1+col, which differs per lane.  Odd lane col therefore runs
 // (col - c0) rows behind (c0 = the wave's first column when the wave holds
 // one group, else 0), so at step s every odd lane reads src row s + 1 + c0:
 // each step is one 8-byte (fp32) pair load per lane from one or two rows.
@@ -1300,6 +1312,16 @@ inline int iadrt_passes(int n, int k0, bool quad) {
 }
 
 #ifndef ADRT_HOST_EMULATION
+// The wide two-stage pass: grid = planes * (n / 128) waves (float32 only).
+template <typename T>
+static void launch_wide(const T *src, T *dst, size_t B, size_t n, int R, int Li, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        const int wpp = int(n / 128);
+        hipLaunchKernelGGL((iadrt_native::wide_kernel<T, uint32_t>), dim3(unsigned(B * 4 * wpp)),
+                           dim3(iadrt_native::kLanes), 0, s, src, dst, R, int(n), Li, wpp);
+    }
+}
+
 template <typename T>
 static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
     const size_t R = 2 * n - 1;
@@ -1328,18 +1350,12 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
         T *dst = bufs[pass & 1];
         const bool paired = int(n) / Li / 2 >= iadrt_native::kHalf;  // Co >= 32
         if (paired && quad && k + 1 < k0) {  // stages k and k + 1 in one pass
-            bool launched = false;
-            if constexpr (sizeof(T) == 4) {  // Co >= 64: the wide pass
-                if (int(n) / Li / 2 >= iadrt_native::kLanes && iadrt_native::quad_narrow<T>(n)) {
-                    hipLaunchKernelGGL((iadrt_native::wide_kernel<T, uint32_t>),
-                                       dim3(unsigned(B * 4 * (n / 128))),
-                                       dim3(iadrt_native::kLanes), 0, s, src, dst, int(R),
-                                       int(n), Li, int(n / 128));
-                    launched = true;
-                }
-            }
-            if (launched) {
-            } else if (iadrt_native::quad_narrow<T>(n))
+            // float32 with Co >= 64 (and 32-bit store offsets): the wide pass
+            const bool wide = sizeof(T) == 4 && int(n) / Li / 2 >= iadrt_native::kLanes &&
+                              iadrt_native::quad_narrow<T>(n);
+            if (wide)
+                launch_wide<T>(src, dst, B, n, int(R), Li, s);
+            else if (iadrt_native::quad_narrow<T>(n))
                 hipLaunchKernelGGL((iadrt_native::quad_kernel<T, uint32_t>), dim3(grid),
                                    dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
                                    Li, waves_per_plane);
