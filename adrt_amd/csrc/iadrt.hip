@@ -843,9 +843,10 @@ __device__ __forceinline__ double partner<double>(double v) {
 // the steps into one per-lane address.  Flush stores wait
 // to the end of their block, off the serial chain of the LDS reads they
 // store.  Bulk blocks store at row y - mlmax plus the lane's offset of
-// mlmax - minlag rows (Off: uint32_t where quad_narrow holds, else size_t);
+// mlmax - minlag rows (32 bits: the library runs two-stage passes only
+// where quad_narrow holds);
 // checked blocks store per lane, where their rows exist.
-template <typename T, typename Off>
+template <typename T>
 __global__ void __launch_bounds__(kLanes)
     quad_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
                 int waves_per_plane) {
@@ -864,7 +865,7 @@ __global__ void __launch_bounds__(kLanes)
     GByte *gdst = (GByte *)(out + plane * size_t(R) * n);
     T *dst_lane = out + plane * size_t(R) * n + q.oc;
     const uint32_t src_off = uint32_t(q.p.pc) * uint32_t(sizeof(T));
-    const Off dst_off = Off(quad_store_off(q, n, mlmax) * sizeof(T));
+    const uint32_t dst_off = uint32_t(quad_store_off(q, n, mlmax) * sizeof(T));
     for (int e = lane; e < kQuadRing; e += kLanes) ring[e] = T(0);
     __syncthreads();
 
@@ -883,7 +884,7 @@ __global__ void __launch_bounds__(kLanes)
     };
     // a bulk block's flushes: rows R - 1 - t0 - k - minlag, all present
     auto flush_bulk = [&](int t0, const T *ov) {
-        Off off = dst_off;
+        uint32_t off = dst_off;
         asm volatile("" : "+v"(off));
         GByte *row = gdst + size_t(R - 1 - t0 - mlmax) * row_bytes;
 #pragma unroll
@@ -1008,7 +1009,7 @@ __global__ void __launch_bounds__(kLanes)
 // kWideAhead - 1 steps ahead in a register ring, flush stores at the end of
 // each 16-step block, three loops of 32-step iterations so that no loop body
 // branches.
-template <typename T, typename Off>
+template <typename T>
 __global__ void __launch_bounds__(kLanes)
     wide_kernel(const T *__restrict__ in, T *__restrict__ out, int R, int n, int Li,
                 int waves_per_plane) {
@@ -1031,8 +1032,8 @@ __global__ void __launch_bounds__(kLanes)
     GByte *gdst = (GByte *)(out + plane * size_t(R) * n);
     T *dst_plane = out + plane * size_t(R) * n;
     const uint32_t src_off = uint32_t(wl.src) * uint32_t(sizeof(T));
-    const Off offE = Off((size_t(mlE - wl.minE) * n + size_t(wl.ocE)) * sizeof(T));
-    const Off offF = Off((size_t(mlF - wl.minF) * n + size_t(wl.ocF)) * sizeof(T));
+    const uint32_t offE = uint32_t((size_t(mlE - wl.minE) * n + size_t(wl.ocE)) * sizeof(T));
+    const uint32_t offF = uint32_t((size_t(mlF - wl.minF) * n + size_t(wl.ocF)) * sizeof(T));
     for (int e = lane; e < kWideRing; e += kLanes) ring[e] = T(0);
     __syncthreads();
 
@@ -1057,7 +1058,7 @@ __global__ void __launch_bounds__(kLanes)
     // a check-free block's flushes: E rows R - 1 - t0 - k - minE (stored at
     // row R - 1 - t0 - k - mlE plus the lane's offset), F likewise
     auto flush_bulk = [&](int t0, const T *oE, const T *oF) {
-        Off e = offE, f = offF;
+        uint32_t e = offE, f = offF;
         asm volatile("" : "+v"(e), "+v"(f));
         GByte *rowE = gdst + size_t(R - 1 - t0 - mlE) * row_bytes;
         GByte *rowF = gdst + size_t(R - 1 - t0 - mlF) * row_bytes;
@@ -1317,7 +1318,7 @@ template <typename T>
 static void launch_wide(const T *src, T *dst, size_t B, size_t n, int R, int Li, hipStream_t s) {
     if constexpr (sizeof(T) == 4) {
         const int wpp = int(n / 128);
-        hipLaunchKernelGGL((iadrt_native::wide_kernel<T, uint32_t>), dim3(unsigned(B * 4 * wpp)),
+        hipLaunchKernelGGL((iadrt_native::wide_kernel<T>), dim3(unsigned(B * 4 * wpp)),
                            dim3(iadrt_native::kLanes), 0, s, src, dst, R, int(n), Li, wpp);
     }
 }
@@ -1336,7 +1337,9 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
     // pairs of paired stages run as one pass each (not in STREAMING)
     const int variant = fused_variant(ADRT_HIP_OP_IADRT, n);
     const int S = tail_stages<T>(L, variant);
-    const bool quad = variant != ADRT_HIP_VARIANT_STREAMING;
+    // two stages per pass unless STREAMING, and only where every store offset
+    // of a plane fits 32 bits (else each stage runs as its own paired pass)
+    const bool quad = variant != ADRT_HIP_VARIANT_STREAMING && iadrt_native::quad_narrow<T>(n);
     const int k0 = L - (S >= 2 ? S : 0);  // stages before the tail
     const int passes = iadrt_passes(int(n), k0, quad) + (S >= 2 ? 1 : 0);
     // ping-pong so that the last pass writes `out`
@@ -1350,17 +1353,11 @@ static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_
         T *dst = bufs[pass & 1];
         const bool paired = int(n) / Li / 2 >= iadrt_native::kHalf;  // Co >= 32
         if (paired && quad && k + 1 < k0) {  // stages k and k + 1 in one pass
-            // float32 with Co >= 64 (and 32-bit store offsets): the wide pass
-            const bool wide = sizeof(T) == 4 && int(n) / Li / 2 >= iadrt_native::kLanes &&
-                              iadrt_native::quad_narrow<T>(n);
-            if (wide)
+            // float32 with Co >= 64: the wide pass
+            if (sizeof(T) == 4 && int(n) / Li / 2 >= iadrt_native::kLanes)
                 launch_wide<T>(src, dst, B, n, int(R), Li, s);
-            else if (iadrt_native::quad_narrow<T>(n))
-                hipLaunchKernelGGL((iadrt_native::quad_kernel<T, uint32_t>), dim3(grid),
-                                   dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
-                                   Li, waves_per_plane);
             else
-                hipLaunchKernelGGL((iadrt_native::quad_kernel<T, size_t>), dim3(grid),
+                hipLaunchKernelGGL((iadrt_native::quad_kernel<T>), dim3(grid),
                                    dim3(iadrt_native::kLanes), 0, s, src, dst, int(R), int(n),
                                    Li, waves_per_plane);
             k += 2;

@@ -535,9 +535,11 @@ static int run(int B, int n, const char *fin, const char *fout, int tail, bool q
     int Li = 1;
     for (int k = 0; k < k0;) {
         const bool paired = n / Li / 2 >= kHalf;
-        if (paired && quad && k + 1 < k0) {
-            if (sizeof(T) == 4 && n / Li / 2 >= kLanes && quad_narrow<T>(size_t(n)))
-                emulate_wide_stage<T>(a, b, B * 4, R, n, Li);  // the library's choice
+        // the library's choice: two stages per pass where store offsets fit 32
+        // bits, the wide pass for float32 with Co >= 64
+        if (paired && quad && quad_narrow<T>(size_t(n)) && k + 1 < k0) {
+            if (sizeof(T) == 4 && n / Li / 2 >= kLanes)
+                emulate_wide_stage<T>(a, b, B * 4, R, n, Li);
             else
                 emulate_quad_stage<T>(a, b, B * 4, R, n, Li);
             k += 2;
