@@ -45,6 +45,20 @@ def bdrt(t):
     return _ws_call("adrt_hip_bdrt", _lib.OP_BDRT, t, dev.empty_like_dtype(t, t.shape))
 
 
+def bdrt_rows(t, rows):
+    """bdrt with rows [0, rows) of every quadrant computed; the rows from
+    `rows` on are unspecified (adrt_hip_bdrt_rows)."""
+    lib = _lib.load()
+    code = _code(t)
+    b, n = _batch_n(t, 3)
+    out = dev.empty_like_dtype(t, t.shape)
+    nbytes = lib.adrt_hip_workspace_bytes(_lib.OP_BDRT, code, b, n)
+    ws = dev.workspace(t, nbytes)
+    _lib.check(lib.adrt_hip_bdrt_rows(dev.ptr(t), dev.ptr(out), dev.ptr(ws), nbytes, code, b, n,
+                                      int(rows), dev.stream_ptr(t)), "adrt_hip_bdrt_rows")
+    return out
+
+
 def iadrt(t):
     return _ws_call("adrt_hip_iadrt", _lib.OP_IADRT, t, dev.empty_like_dtype(t, t.shape))
 
@@ -158,9 +172,16 @@ def truncate_combine(t, divisor=1.0, mode=MEAN):
     return out
 
 
+def adjoint(t, divisor=1.0, mode=MEAN):
+    """truncate_combine(bdrt(t), divisor, mode) with only the rows [0, n) of
+    each bdrt quadrant computed (bdrt_rows), the rows truncate keeps."""
+    return truncate_combine(bdrt_rows(t, int(t.shape[-1])), divisor, mode)
+
+
 # ---------------------------------------------------------------------------
 # batched CG vector operations: (B, ...) tensors, one system per leading index
 # ---------------------------------------------------------------------------
+
 def _bm(t):
     b = int(t.shape[0])
     return b, int(t.numel() // max(b, 1))

@@ -73,6 +73,7 @@ SIGNATURES = {
     "adrt_hip_adrt_noalloc": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _VP]),
     "adrt_hip_adrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
     "adrt_hip_bdrt": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _VP]),
+    "adrt_hip_bdrt_rows": (_INT, [_VP, _VP, _VP, _SZ, _INT, _SZ, _SZ, _SZ, _VP]),
     "adrt_hip_bdrt_step": (_INT, [_VP, _VP, _INT, _SZ, _SZ, _INT, _VP]),
     "adrt_hip_adrt_bdrt_small": (_INT, [ctypes.POINTER(Batch), _INT, _INT, _INT, _VP]),
     "adrt_hip_queue_workspace_bytes": (_SZ, [ctypes.POINTER(Batch), _INT, _INT]),

@@ -69,7 +69,7 @@ def _check_adrt_data(y):
 
 def _adjoint(g):
     """A^T g = sum over quadrants of truncate(bdrt(g)): (..., 4, 2n-1, n) -> (..., n, n)."""
-    return _kernels.truncate_combine(_kernels.bdrt(g.contiguous()), 1.0, _kernels.SUM)
+    return _kernels.adjoint(g.contiguous(), 1.0, _kernels.SUM)
 
 
 def _stage_chain(h):

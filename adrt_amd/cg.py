@@ -41,7 +41,7 @@ class ADRTNormalOperator:
         self.dtype = dtype
 
     def __call__(self, x):
-        return _kernels.truncate_combine(_kernels.bdrt(_kernels.adrt(x)), 1.0, _kernels.MEAN)
+        return _kernels.adjoint(_kernels.adrt(x), 1.0, _kernels.MEAN)
 
     matvec = __call__
 
@@ -62,7 +62,7 @@ class ADRTRidgeOperator(ADRTNormalOperator):
 def normal_rhs(b):
     """mean_q(truncate(bdrt(b))): the right-hand side of the recipe for ADRT
     data b of shape (..., 4, 2n-1, n) (docs/examples.cginverse.md:62)."""
-    return _kernels.truncate_combine(_kernels.bdrt(b), 1.0, _kernels.MEAN)
+    return _kernels.adjoint(b, 1.0, _kernels.MEAN)
 
 
 def _dot(u, v):

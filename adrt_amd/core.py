@@ -100,9 +100,10 @@ def _fmg_step_device(t):
     """Press FMG inverse estimate on a device tensor (core.py:265-331).
 
     At level n the correction is
-    ``highpass(mean_q(truncate(bdrt(adrt(ret) - a_n)) / (n - 1)))``; the fused
-    truncate_combine kernel divides per element, sums ((q0+q1)+q2)+q3 and
-    divides by 4, matching NumPy's mean over the quadrant axis.
+    ``highpass(mean_q(truncate(bdrt(adrt(ret) - a_n)) / (n - 1)))``; bdrt is
+    computed for the rows truncate keeps only, and the fused truncate_combine
+    kernel divides per element, sums ((q0+q1)+q2)+q3 and divides by 4,
+    matching NumPy's mean over the quadrant axis.
     """
     stack = []
     for _ in range(num_iters(t.shape[-1])):
@@ -115,7 +116,7 @@ def _fmg_step_device(t):
         ret = _kernels.fmg_prolongation(ret)
         resid = _kernels.adrt(ret) - stack.pop()
         ret -= _kernels.fmg_highpass(
-            _kernels.truncate_combine(_kernels.bdrt(resid), divisor=n - 1, mode=_kernels.MEAN)
+            _kernels.adjoint(resid, divisor=n - 1, mode=_kernels.MEAN)
         )
     return ret
 

@@ -102,10 +102,10 @@ size_t bdrt_workspace(int dtype, size_t B, size_t n) {
 // log2 n in [6, 12]: the fused two-pass back-projection (fused.hip);
 // other sizes stream one stage per kernel.
 int launch_bdrt(const void *in, void *out, void *ws, size_t, int dtype, size_t B, size_t n,
-                hipStream_t s) {
+                int rows, hipStream_t s) {
     if (use_small(ADRT_HIP_OP_BDRT, B, n))
         return launch_small(in, out, ADRT_HIP_OP_BDRT, dtype, B, n, s);
-    if (use_fused(ADRT_HIP_OP_BDRT, n)) return run_fused_bdrt(in, out, ws, dtype, B, n, s);
+    if (use_fused(ADRT_HIP_OP_BDRT, n)) return run_fused_bdrt(in, out, ws, dtype, B, n, rows, s);
     if (dtype == ADRT_HIP_F32)
         return run_streaming(static_cast<const float *>(in), static_cast<float *>(out),
                              static_cast<float *>(ws), B, n, s);

@@ -147,18 +147,29 @@ int adrt_hip_adrt_step(const void *in, void *out, int dtype, size_t batch, size_
     return launch_adrt_step(in, out, dtype, batch, n, step, static_cast<hipStream_t>(stream));
 }
 
-int adrt_hip_bdrt(const void *in, void *out, void *workspace, size_t workspace_bytes, int dtype,
-                  size_t batch, size_t n, void *stream) {
+static int bdrt_checked(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                        int dtype, size_t batch, size_t n, size_t rows, void *stream) {
     const size_t shape[4] = {batch, 4, 2 * n - 1, n};
     if (n == 0 || !adrt_hip_valid_output_shape(shape)) return ADRT_HIP_ERR_SHAPE;
     size_t elems;
     if (!output_elems(batch, n, &elems) || !mul_ok(elems, 8, &elems)) return ADRT_HIP_ERR_TOO_BIG;
+    if (rows == 0 || rows > 2 * n - 1) return ADRT_HIP_ERR_ARG;
     if (!dtype_ok(dtype)) return ADRT_HIP_ERR_DTYPE;
     if (!in || !out) return ADRT_HIP_ERR_NULL;
     const size_t need = bdrt_workspace(dtype, batch, n);
     if (workspace_bytes < need || (need && !workspace)) return ADRT_HIP_ERR_WORKSPACE;
-    return launch_bdrt(in, out, workspace, workspace_bytes, dtype, batch, n,
+    return launch_bdrt(in, out, workspace, workspace_bytes, dtype, batch, n, int(rows),
                        static_cast<hipStream_t>(stream));
+}
+
+int adrt_hip_bdrt(const void *in, void *out, void *workspace, size_t workspace_bytes, int dtype,
+                  size_t batch, size_t n, void *stream) {
+    return bdrt_checked(in, out, workspace, workspace_bytes, dtype, batch, n, 2 * n - 1, stream);
+}
+
+int adrt_hip_bdrt_rows(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                       int dtype, size_t batch, size_t n, size_t rows, void *stream) {
+    return bdrt_checked(in, out, workspace, workspace_bytes, dtype, batch, n, rows, stream);
 }
 
 int adrt_hip_bdrt_step(const void *in, void *out, int dtype, size_t batch, size_t n, int step,

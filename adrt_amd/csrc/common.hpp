@@ -92,12 +92,14 @@ int run_fused_budget(const void *in, void *out, void *ws, int dtype, size_t B, s
                      size_t group_bytes, hipStream_t s);
 size_t fused_workspace_per_image(int dtype, size_t n);
 size_t fused_bdrt_workspace(int dtype, size_t B, size_t n);
-int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
+int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, int rows,
                    hipStream_t s);
 int launch_adrt_step(const void *in, void *out, int dtype, size_t B, size_t N, int step,
                      hipStream_t s);
+// Stores at least output rows [0, rows) of every quadrant (rows <= 2N - 1);
+// the fused path stores exactly those and does only the work they need.
 int launch_bdrt(const void *in, void *out, void *ws, size_t ws_bytes, int dtype, size_t B,
-                size_t N, hipStream_t s);
+                size_t N, int rows, hipStream_t s);
 size_t bdrt_workspace(int dtype, size_t B, size_t N);
 int launch_bdrt_step(const void *in, void *out, int dtype, size_t B, size_t N, int step,
                      hipStream_t s);

@@ -39,10 +39,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
                                amdgpu_waves_per_eu(lds_waves_per_simd<BdrtA<T, K2>, T>(
                                    sizeof(T) == 4 ? 8 : 4))))
 bdrt_passA(const T *__restrict__ y_base, T *__restrict__ Z_base, int n, int first_image,
-           int items) {
+           int zrows, int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     run_item<BdrtA<T, K2>, T>(items, reinterpret_cast<T *>(smem_raw), y_base, Z_base, n,
-                              first_image);
+                              first_image, zrows);
 }
 
 template <typename T, int K4>
@@ -50,10 +50,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
                                amdgpu_waves_per_eu(lds_waves_per_simd<BdrtB<T, K4>, T>(
                                    sizeof(T) == 4 ? 8 : 4))))
 bdrt_passB(const T *__restrict__ Z_base, T *__restrict__ out_base, int n, int kA,
-           int first_image, int items) {
+           int first_image, int rows, int items) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     run_item<BdrtB<T, K4>, T>(items, reinterpret_cast<T *>(smem_raw), Z_base, out_base, n, kA,
-                              first_image);
+                              first_image, rows);
 }
 #endif  // ADRT_HOST_EMULATION
 
@@ -129,47 +129,51 @@ static hipError_t run_group(const T *img, T *X, T *out, int n, int L, int k1, in
 }
 
 template <typename T, int K2>
-static hipError_t launch_bdrt_a(const T *y, T *Z, int n, int first, int count, hipStream_t s) {
+static hipError_t launch_bdrt_a(const T *y, T *Z, int n, int first, int count, int zrows,
+                                hipStream_t s) {
     using P = BdrtA<T, K2>;
     const int items = P::num_items(n, count);
     constexpr size_t lds = P::lds_elems * sizeof(T);
     const hipError_t e = lds_limit<bdrt_passA<T, K2>>(lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((bdrt_passA<T, K2>), dim3(launch_grid(items)), dim3(kThreads), lds, s, y,
-                       Z, n, first, items);
+                       Z, n, first, zrows, items);
     return hipSuccess;
 }
 
 template <typename T, int K4>
-static hipError_t launch_bdrt_b(const T *Z, T *out, int n, int kA, int first, int count,
+static hipError_t launch_bdrt_b(const T *Z, T *out, int n, int kA, int first, int count, int rows,
                                 hipStream_t s) {
     using P = BdrtB<T, K4>;
-    const int items = P::num_items(n, kA, count);
+    const int items = P::num_items(kA, count, rows);
     constexpr size_t lds = P::lds_elems * sizeof(T);
     const hipError_t e = lds_limit<bdrt_passB<T, K4>>(lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((bdrt_passB<T, K4>), dim3(launch_grid(items)), dim3(kThreads), lds, s, Z,
-                       out, n, kA, first, items);
+                       out, n, kA, first, rows, items);
     return hipSuccess;
 }
 
-// bdrt split: pass A takes kA stages, pass B kB = L - kA.
+// bdrt split: pass A takes kA stages, pass B kB = L - kA.  Output rows
+// [0, rows) of every quadrant are stored (rows = 2n - 1: all of them);
+// pass A then skips the items whose Z rows those do not read.
 template <typename T>
 static hipError_t run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int kA, int first,
-                                 int count, hipStream_t s) {
+                                 int count, int rows, hipStream_t s) {
     hipError_t e;
+    const int zrows = bdrt_z_rows(n, rows);
     switch (kA - 3) {
-        case 0: e = launch_bdrt_a<T, 0>(y, Z, n, first, count, s); break;
-        case 1: e = launch_bdrt_a<T, 1>(y, Z, n, first, count, s); break;
-        case 2: e = launch_bdrt_a<T, 2>(y, Z, n, first, count, s); break;
-        default: e = launch_bdrt_a<T, 3>(y, Z, n, first, count, s); break;
+        case 0: e = launch_bdrt_a<T, 0>(y, Z, n, first, count, zrows, s); break;
+        case 1: e = launch_bdrt_a<T, 1>(y, Z, n, first, count, zrows, s); break;
+        case 2: e = launch_bdrt_a<T, 2>(y, Z, n, first, count, zrows, s); break;
+        default: e = launch_bdrt_a<T, 3>(y, Z, n, first, count, zrows, s); break;
     }
     if (e != hipSuccess) return e;
     switch (L - kA - 3) {
-        case 0: return launch_bdrt_b<T, 0>(Z, out, n, kA, first, count, s);
-        case 1: return launch_bdrt_b<T, 1>(Z, out, n, kA, first, count, s);
-        case 2: return launch_bdrt_b<T, 2>(Z, out, n, kA, first, count, s);
-        default: return launch_bdrt_b<T, 3>(Z, out, n, kA, first, count, s);
+        case 0: return launch_bdrt_b<T, 0>(Z, out, n, kA, first, count, rows, s);
+        case 1: return launch_bdrt_b<T, 1>(Z, out, n, kA, first, count, rows, s);
+        case 2: return launch_bdrt_b<T, 2>(Z, out, n, kA, first, count, rows, s);
+        default: return launch_bdrt_b<T, 3>(Z, out, n, kA, first, count, rows, s);
     }
 }
 #endif  // ADRT_HOST_EMULATION
@@ -300,7 +304,8 @@ size_t fused_bdrt_workspace(int dtype, size_t B, size_t n) {
 }
 
 template <typename T>
-static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
+static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, int rows,
+                            hipStream_t s) {
     const int L = num_iters(n);
     const int kA = fused::bdrt_passA_stages(L, fused_variant(ADRT_HIP_OP_BDRT, n));
     const size_t per = fused::z_bytes_per_image(int(n), sizeof(T));
@@ -308,7 +313,7 @@ static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, hipS
     for (size_t first = 0; first < B; first += G) {
         const size_t count = (B - first) < G ? (B - first) : G;
         const hipError_t e =
-            fused::run_bdrt_group<T>(in, ws, out, int(n), L, kA, int(first), int(count), s);
+            fused::run_bdrt_group<T>(in, ws, out, int(n), L, kA, int(first), int(count), rows, s);
         if (e != hipSuccess) {
             set_last_error(e);
             return ADRT_HIP_ERR_DEVICE;
@@ -317,13 +322,13 @@ static int run_fused_bdrt_t(const T *in, T *out, T *ws, size_t B, size_t n, hipS
     return launch_status();
 }
 
-int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n,
+int run_fused_bdrt(const void *in, void *out, void *ws, int dtype, size_t B, size_t n, int rows,
                    hipStream_t s) {
     if (dtype == ADRT_HIP_F32)
         return run_fused_bdrt_t(static_cast<const float *>(in), static_cast<float *>(out),
-                                static_cast<float *>(ws), B, n, s);
+                                static_cast<float *>(ws), B, n, rows, s);
     return run_fused_bdrt_t(static_cast<const double *>(in), static_cast<double *>(out),
-                            static_cast<double *>(ws), B, n, s);
+                            static_cast<double *>(ws), B, n, rows, s);
 }
 #endif  // ADRT_HOST_EMULATION
 

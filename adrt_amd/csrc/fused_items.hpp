@@ -430,6 +430,7 @@ struct Pass1 {
     __host__ __device__ __forceinline__ static int num_items(int n, int nimg) {
         return nimg * 4 * (n >> k1) * num_chunks(n);
     }
+    __host__ __device__ __forceinline__ static bool skip(const Item &) { return false; }
 
     // Item `block` in [0, num_items) -> (image, quadrant, row block J, chunk),
     // ordered so that the two quadrants reading the same image data are
@@ -665,6 +666,7 @@ struct Pass2 {
     __host__ __device__ __forceinline__ static int num_items(int n, int k1, int nimg) {
         return nimg * 4 * (1 << k1) * num_chunks(n);
     }
+    __host__ __device__ __forceinline__ static bool skip(const Item &) { return false; }
 
     // Item `block` in [0, num_items): (image, quadrant, angle s, chunk).
     __host__ __device__ __forceinline__ static Item item(const T *X_base, T *out_base, int n, int k1,
@@ -816,6 +818,16 @@ struct Pass2 {
 constexpr int kZL = 32;
 __host__ __device__ __forceinline__ int z_pitch(int n) { return kZL + (2 * n - 1 + 8 + 31) / 32 * 32; }
 
+// Z rows that pass B reads when it stores output rows [0, rows): each of its
+// ceil(rows / kOut) row chunks reads kLanes * kW rows from the chunk start,
+// and no row past C + 4 (the last margin row pass A writes).  With rows =
+// C this is every row pass A writes.
+__host__ __device__ __forceinline__ int bdrt_z_rows(int n, int rows) {
+    const int reach = ((rows + kOut - 1) / kOut - 1) * kOut + kLanes * kW;
+    const int full = 2 * n - 1 + 5;
+    return reach < full ? reach : full;
+}
+
 // LDS column c = 4*lane + u of a row (u >= 0, c < kRow): interleaved by
 // residue so that one wave-uniform u is bank-consecutive across lanes.
 __host__ __device__ __forceinline__ int lds_col(int u) { return (u & (kW - 1)) * kSlice + (u >> 2); }
@@ -851,6 +863,7 @@ struct BdrtA {
         const T *y;  // input plane (C, n)
         T *Z;        // row 0 of section 0 of group p; section s at + s * PZ
         int n, C, p, c0, PZ;
+        int zrows;  // Z rows pass B reads (bdrt_z_rows)
     };
 
     // Chunks of input rows: outputs reach row C + 4 (margins) after the
@@ -866,11 +879,12 @@ struct BdrtA {
 
     // Item `block` -> (image, quadrant, group p, chunk), chunk fastest.
     __host__ __device__ __forceinline__ static Item item(const T *y_base, T *Z_base, int n,
-                                                         int first_image, int block) {
+                                                         int first_image, int zrows, int block) {
         Item it;
         it.n = n;
         it.C = 2 * n - 1;
         it.PZ = z_pitch(n);
+        it.zrows = zrows;
         const int nP = n >> kA, nchunk = num_chunks(n);
         const int chunk = block % nchunk;
         block /= nchunk;
@@ -937,6 +951,13 @@ struct BdrtA {
             col[2 * kSP] = v[q][2];
             col[3 * kSP] = v[q][3];
         }
+    }
+
+    // Section s of the item goes to Z rows from c0 - p*s on, so with s <
+    // 2^kA every row it stores lies at or past c0 - p*(2^kA - 1).  When that
+    // is at or past the rows pass B reads, nothing it stores is read.
+    __host__ __device__ __forceinline__ static bool skip(const Item &it) {
+        return it.c0 - it.p * ((1 << kA) - 1) >= it.zrows;
     }
 
     // An item whose input rows all lie past the last row (c0 > C) sees only
@@ -1060,24 +1081,29 @@ struct BdrtB {
         const T *Z;  // row 0 of (group 0, section s); group p at + p * pstride
         T *out;      // output plane (C, n)
         int n, C, d0, abase;
+        int rows;    // output rows stored: [0, rows), rows <= C
         size_t pstride;
     };
 
-    __host__ __device__ __forceinline__ static int num_chunks(int n) {
-        return (2 * n - 1 + kOut - 1) / kOut;
+    // Chunks of kOut output rows covering rows [0, rows).
+    __host__ __device__ __forceinline__ static int num_chunks(int rows) {
+        return (rows + kOut - 1) / kOut;
     }
-    __host__ __device__ __forceinline__ static int num_items(int n, int kA, int nimg) {
-        return nimg * 4 * (1 << kA) * num_chunks(n);
+    __host__ __device__ __forceinline__ static int num_items(int kA, int nimg, int rows) {
+        return nimg * 4 * (1 << kA) * num_chunks(rows);
     }
+
+    __host__ __device__ __forceinline__ static bool skip(const Item &) { return false; }
 
     // Item `block` -> (image, quadrant, section s, chunk), chunk fastest.
     __host__ __device__ __forceinline__ static Item item(const T *Z_base, T *out_base, int n, int kA,
-                                                         int first_image, int block) {
+                                                         int first_image, int rows, int block) {
         Item it;
         it.n = n;
         it.C = 2 * n - 1;
+        it.rows = rows;
         const int PZ = z_pitch(n);
-        const int nS = 1 << kA, nchunk = num_chunks(n);
+        const int nS = 1 << kA, nchunk = num_chunks(rows);
         const int chunk = block % nchunk;
         block /= nchunk;
         const int s = block % nS;
@@ -1160,7 +1186,7 @@ struct BdrtB {
         for (int f = tid; f < kOut * G; f += kThreads) {
             const int d = f / G, grp = f % G;
             const int od = it.d0 + d;
-            if (od >= it.C) continue;
+            if (od >= it.rows) continue;
             const T *p = tile + tile_pos(d, 4 * grp);
             store_vec4_stream(it.out + (size_t)od * it.n + it.abase + 4 * grp, p[0], p[1], p[2], p[3]);
         }
@@ -1183,6 +1209,7 @@ __device__ __forceinline__ void run_item_at(int idx, T *lds, A... args) {
     static_assert(P::tasks0 <= kWaves, "at most one phase-0 strip per wave");
     static_assert(P::tasks1 == 0 || P::tasks1 == kWaves, "one phase-1 strip per wave");
     const typename P::Item it = P::item(args..., idx);
+    if (P::skip(it)) return;  // work-group uniform
     const int wave = wave_uniform(threadIdx.x / kLanes);
     const int lane = threadIdx.x % kLanes;
     if constexpr (P::kStaged) {

@@ -153,8 +153,11 @@ __device__ __forceinline__ void run_fused(int pass, const T *in, T *out, int n, 
     switch (pass) {
         case kPass1: fused::run_item_at<fused::Pass1<T, K>, T>(item, lds, in, out, n, 0); break;
         case kPass2: fused::run_item_at<fused::Pass2<T, K>, T>(item, lds, in, out, n, k, 0); break;
-        case kPassA: fused::run_item_at<fused::BdrtA<T, K>, T>(item, lds, in, out, n, 0); break;
-        default: fused::run_item_at<fused::BdrtB<T, K>, T>(item, lds, in, out, n, k, 0); break;
+        case kPassA:
+            fused::run_item_at<fused::BdrtA<T, K>, T>(item, lds, in, out, n, 0,
+                                                      fused::bdrt_z_rows(n, 2 * n - 1));
+            break;
+        default: fused::run_item_at<fused::BdrtB<T, K>, T>(item, lds, in, out, n, k, 0, 2 * n - 1); break;
     }
 }
 
@@ -341,9 +344,9 @@ inline int items_per_image(int pass, int n, int k1, int kA) {
             }
         default:
             switch (num_iters(size_t(n)) - kA - 3) {
-                case 0: return fused::BdrtB<float, 0>::num_items(n, kA, 1);
-                case 1: return fused::BdrtB<float, 1>::num_items(n, kA, 1);
-                default: return fused::BdrtB<float, 2>::num_items(n, kA, 1);
+                case 0: return fused::BdrtB<float, 0>::num_items(kA, 1, 2 * n - 1);
+                case 1: return fused::BdrtB<float, 1>::num_items(kA, 1, 2 * n - 1);
+                default: return fused::BdrtB<float, 2>::num_items(kA, 1, 2 * n - 1);
             }
     }
 }

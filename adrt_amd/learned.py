@@ -66,11 +66,11 @@ def _check(status, what):
 
 def normal_op(x):
     """M x = mean_q truncate(bdrt(adrt(x))) for (B, n, n) float32 device images."""
-    return _kernels.truncate_combine(_kernels.bdrt(_kernels.adrt(x)), 1.0, _kernels.MEAN)
+    return _kernels.adjoint(_kernels.adrt(x), 1.0, _kernels.MEAN)
 
 
 def normal_rhs(d):
-    return _kernels.truncate_combine(_kernels.bdrt(d), 1.0, _kernels.MEAN)
+    return _kernels.adjoint(d, 1.0, _kernels.MEAN)
 
 
 def bdot(u, v):

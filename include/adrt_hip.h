@@ -132,6 +132,17 @@ int adrt_hip_adrt_step(const void *in, void *out, int dtype, size_t batch, size_
 int adrt_hip_bdrt(const void *in, void *out, void *workspace, size_t workspace_bytes,
                   int dtype, size_t batch, size_t n, void *stream);
 
+/* adrt_hip_bdrt for the rows [0, rows) of every quadrant only (1 <= rows <=
+ * 2n-1, else ADRT_HIP_ERR_ARG): those rows equal adrt_hip_bdrt's, the rows
+ * from `rows` on are left unspecified (written or not).  Same workspace.  No
+ * reference counterpart: the reference's transpose of adrt,
+ * truncate(bdrt(y)).mean (docs/examples.cginverse.md:51,63; truncate at
+ * utils.py:191-245), reads rows [0, n) of each quadrant only, and the fused
+ * kernels then do only the work those rows need (about half of the second
+ * pass, a part of the first). */
+int adrt_hip_bdrt_rows(const void *in, void *out, void *workspace, size_t workspace_bytes,
+                       int dtype, size_t batch, size_t n, size_t rows, void *stream);
+
 /* Forward ADRT and back-projection of several batches of small sides in one
  * persistent launch (no reference counterpart: the reference runs one call
  * per array; this is the uneven-size batch of many small images).  For each

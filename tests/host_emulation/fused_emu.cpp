@@ -8,9 +8,11 @@
 // exactly as the library's workspace, so any out-of-bounds access aborts
 // under AddressSanitizer and any read of an unwritten or invalid slot shows
 // up as NaN in the result.  Usage:
-//   fused_emu f32|f64 B n in.bin out.bin [slots] [fwd|bdrt]
+//   fused_emu f32|f64 B n in.bin out.bin [slots] [fwd|bdrt] [variant] [group_bytes] [rows]
 // `slots` (a multiple of 8) caps the grid so that work-groups run several
 // items each; 0 (default) is the library's one-item-per-work-group grid.
+// `rows` > 0 runs the back-projection for output rows [0, rows) only, as
+// adrt_hip_bdrt_rows does (the other rows stay NaN in the output).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -38,6 +40,7 @@ static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args.
         for (int idx = sc.idx; idx < sc.hi; idx += sc.step) {
             seen[size_t(idx)]++;
             const auto it = P::item(args..., idx);
+            if (P::skip(it)) continue;  // the kernel's early return
             for (auto &x : lds) x = T(NAN);
             T *l = P::lds_elems > 0 ? lds.data() : nullptr;
             if constexpr (P::kStaged) {
@@ -126,6 +129,7 @@ static unsigned grid_for_slots(int items, unsigned slots) {
 static unsigned g_slots = 0;
 static int g_variant = 0;  // stage split (ADRT_HIP_VARIANT_DEFAULT / _ALT_SPLIT)
 static size_t g_budget = kDefaultGroupBytes;  // intermediate bytes per launch group
+static int g_rows = 0;  // back-projection output rows stored (0: all 2n - 1)
 
 template <typename T, int KB, int KD>
 static int emulate_group(const T *img, T *X, T *out, int n, int first, int count) {
@@ -147,12 +151,14 @@ static int emulate_bdrt_group(const T *y, T *Z, T *out, int n, int first, int co
     using PA = BdrtA<T, K2>;
     using PB = BdrtB<T, K4>;
     std::vector<int> seen;
+    const int rows = g_rows > 0 ? g_rows : 2 * n - 1;
     const int itemsA = PA::num_items(n, count);
-    emulate_grid<PA, T>(itemsA, grid_for_slots(itemsA, g_slots), seen, y, Z, n, first);
+    emulate_grid<PA, T>(itemsA, grid_for_slots(itemsA, g_slots), seen, y, Z, n, first,
+                        bdrt_z_rows(n, rows));
     if (!once_each(seen)) return 5;
-    const int itemsB = PB::num_items(n, PA::kA, count);
+    const int itemsB = PB::num_items(PA::kA, count, rows);
     emulate_grid<PB, T>(itemsB, grid_for_slots(itemsB, g_slots), seen, (const T *)Z, out, n, PA::kA,
-                        first);
+                        first, rows);
     if (!once_each(seen)) return 6;
     return 0;
 }
@@ -234,16 +240,18 @@ static int run(int B, int n, const char *fin, const char *fout) {
 }
 
 int main(int argc, char **argv) {
-    if (argc < 6 || argc > 10) {
+    if (argc < 6 || argc > 11) {
         std::fprintf(stderr,
-                     "usage: %s f32|f64 B n in.bin out.bin [slots] [fwd|bdrt] [variant] [group_bytes]\n",
+                     "usage: %s f32|f64 B n in.bin out.bin [slots] [fwd|bdrt] [variant] [group_bytes]"
+                     " [rows]\n",
                      argv[0]);
         return 1;
     }
     if (argc >= 7) g_slots = unsigned(std::atoi(argv[6]));
     const bool bdrt = argc >= 8 && std::strcmp(argv[7], "bdrt") == 0;
     if (argc >= 9) g_variant = std::atoi(argv[8]);
-    if (argc == 10 && std::atoll(argv[9]) > 0) g_budget = size_t(std::atoll(argv[9]));
+    if (argc >= 10 && std::atoll(argv[9]) > 0) g_budget = size_t(std::atoll(argv[9]));
+    if (argc == 11) g_rows = std::atoi(argv[10]);
     const int B = std::atoi(argv[2]), n = std::atoi(argv[3]);
     if (std::strcmp(argv[1], "f32") == 0)
         return bdrt ? run_bdrt<float>(B, n, argv[4], argv[5]) : run<float>(B, n, argv[4], argv[5]);

@@ -130,6 +130,45 @@ class TestBdrtCompute:
         assert canonical_sha256(out) == golden_hashes()["bdrt_f32_of_adrt_8x128_seed0"]
 
 
+@gpu
+class TestBdrtRows:
+    """adrt_hip_bdrt_rows (bdrt for the rows [0, rows) of every quadrant)
+    through the C ABI, and the adjoint built on it."""
+
+    @pytest.mark.parametrize("dt,n,b", [("float32", 16, 4), ("float64", 64, 8),
+                                        ("float32", 128, 4), ("float64", 256, 2),
+                                        ("float32", 512, 3), ("float64", 1024, 1),
+                                        ("float32", 2048, 2)])
+    def test_rows_equal_bdrt(self, cuda, dt, n, b):
+        """Kept rows are bit-identical to the full back-projection's for row
+        limits inside the first chunk, at chunk boundaries and at n and 2n-1."""
+        import torch
+        from adrt_amd import _kernels
+
+        rng = np.random.default_rng(3 * n + b)
+        y = torch.from_numpy((rng.standard_normal((b, 4, 2 * n - 1, n)) * 10).astype(dt)).to(cuda)
+        full = _kernels.bdrt(y)
+        ity = torch.int32 if dt == "float32" else torch.int64
+        for rows in sorted({1, min(192, 2 * n - 1), min(384, 2 * n - 1), n - 1 or 1, n,
+                            2 * n - 1}):
+            got = _kernels.bdrt_rows(y, rows)
+            assert torch.equal(got[:, :, :rows].view(ity), full[:, :, :rows].view(ity)), rows
+
+    @pytest.mark.parametrize("dt,n", [("float32", 512), ("float64", 256), ("float32", 2048)])
+    def test_adjoint_equals_oracle(self, cuda, dt, n):
+        """adjoint(y) = sum over quadrants of truncate(bdrt(y)), equal to the
+        oracle's bdrt followed by the same truncation and quadrant sum."""
+        import torch
+        from adrt_amd import _kernels
+
+        rng = np.random.default_rng(n)
+        y = (rng.standard_normal((2, 4, 2 * n - 1, n)) * 10).astype(dt)
+        got = _kernels.adjoint(torch.from_numpy(y).to(cuda), 1.0, _kernels.SUM).cpu().numpy()
+        t = O.truncate(O.bdrt(y))
+        want = ((t[:, 0] + t[:, 1]) + t[:, 2]) + t[:, 3]
+        assert np.array_equal(got + 0.0, want + 0.0)
+
+
 def _delta(size, q, h, s):
     a = np.zeros((4, 2 * size - 1, size))
     a[q, h, s] = 1.0

@@ -98,6 +98,10 @@ def test_entry_points_reject_before_launch():
     assert lib.adrt_hip_fmg_restriction(z, z, 0, 1, 1, z) == _lib.ERR_SHAPE
     assert lib.adrt_hip_fmg_highpass(z, z, 0, 1, 1, 5, z) == _lib.ERR_SHAPE
     assert lib.adrt_hip_truncate_combine(z, z, 0, 1, 4, 1.0, 3, z) == _lib.ERR_ARG
+    assert lib.adrt_hip_bdrt_rows(z, z, z, 0, 0, 1, 12, 5, z) == _lib.ERR_SHAPE
+    assert lib.adrt_hip_bdrt_rows(z, z, z, 0, 0, 1, 16, 0, z) == _lib.ERR_ARG
+    assert lib.adrt_hip_bdrt_rows(z, z, z, 0, 0, 1, 16, 32, z) == _lib.ERR_ARG
+    assert lib.adrt_hip_bdrt_rows(z, z, z, 0, 0, 1, 16, 31, z) == _lib.ERR_NULL
     one = ctypes.c_void_p(1)
     need = lib.adrt_hip_workspace_bytes(_lib.OP_ADRT, 0, 2, 128)
     assert need > 0

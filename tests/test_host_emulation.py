@@ -36,14 +36,14 @@ def emulator(tmp_path_factory):
     return exe
 
 
-def run_emu(exe, x, tmp_path, slots=0, op="fwd", variant=0, group_bytes=0):
+def run_emu(exe, x, tmp_path, slots=0, op="fwd", variant=0, group_bytes=0, rows=0):
     b, n = x.shape[0], x.shape[-1]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     x.tofile(fin)
     tag = "f32" if x.dtype == np.float32 else "f64"
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
     res = subprocess.run([exe, tag, str(b), str(n), str(fin), str(fout), str(slots), op,
-                          str(variant), str(group_bytes)],
+                          str(variant), str(group_bytes), str(rows)],
                          capture_output=True, text=True, env=env)
     assert res.returncode == 0, res.stderr[-4000:]
     return np.fromfile(fout, dtype=x.dtype).reshape(b, 4, 2 * n - 1, n)
@@ -60,6 +60,28 @@ def test_fused_bdrt_emulation(emulator, tmp_path, dt, n):
     y[0, 1, -1, :3] = [-0.0, -0.0, 0.0]
     got = run_emu(emulator, y, tmp_path, op="bdrt")
     assert np.array_equal(canonical(got).view(np.uint8), canonical(O.bdrt(y)).view(np.uint8))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("dt,n,rows", [("float32", 64, 64), ("float64", 128, 1),
+                                       ("float32", 256, 192), ("float32", 256, 256),
+                                       ("float32", 512, 193), ("float64", 512, 384),
+                                       ("float64", 512, 512), ("float32", 1024, 1024),
+                                       ("float32", 1024, 1152), ("float32", 1024, 2047)])
+def test_fused_bdrt_rows_emulation(emulator, tmp_path, dt, n, rows, variant):
+    """Back-projection of output rows [0, rows) only (adrt_hip_bdrt_rows):
+    pass B stores just those row chunks and pass A skips the items whose Z
+    rows they never read.  Z starts as NaN and the data is finite, so a kept
+    row that read a Z row no pass-A item wrote comes out NaN; kept rows must
+    equal the oracle's, and the rows past the last kept chunk stay unwritten."""
+    rng = np.random.default_rng(7 * n + rows)
+    y = (rng.standard_normal((2, 4, 2 * n - 1, n)) * 10).astype(dt)
+    got = run_emu(emulator, y, tmp_path, op="bdrt", variant=variant, rows=rows)
+    want = O.bdrt(y)
+    assert not np.isnan(got[:, :, :rows]).any()
+    assert np.array_equal(canonical(got[:, :, :rows]).view(np.uint8),
+                          canonical(want[:, :, :rows]).view(np.uint8))
+    assert np.isnan(got[:, :, rows:]).all()
 
 
 @pytest.mark.parametrize("slots", [8, 24])
