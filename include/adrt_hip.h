@@ -5,9 +5,10 @@
  * asynchronously on the given HIP stream (a hipStream_t passed as void*;
  * NULL means the default stream) and returns an int status (ADRT_HIP_OK or
  * one of the ADRT_HIP_ERR_* codes below).  Inputs are never modified,
- * outputs are written in full, and nothing is allocated inside the library:
- * temporary space comes from the caller through the `workspace` argument,
- * sized by adrt_hip_workspace_bytes().  No torch or Python types appear.
+ * outputs are written in full (adrt_hip_bdrt_rows: the rows asked for), and
+ * nothing is allocated inside the library: temporary space comes from the
+ * caller through the `workspace` argument, sized by
+ * adrt_hip_workspace_bytes().  No torch or Python types appear.
  *
  * Each transform entry point replaces one function of the reference's
  * CPython extension adrt._adrt_cdefs (src/adrt/adrt_cdefs_py.cpp, method
