@@ -1327,10 +1327,13 @@ template <typename T>
 static int run_native(const T *in, T *out, T *ws, size_t B, size_t n, hipStream_t s) {
     const size_t R = 2 * n - 1;
     const int L = num_iters(n);
-    if (L == 0) {
-        if (hipMemcpyAsync(out, in, B * 4 * R * n * sizeof(T), hipMemcpyDeviceToDevice, s) !=
-            hipSuccess)
-            return launch_status();
+    if (L == 0) {  // n = 1: the inverse of a single-element transform is a copy
+        const hipError_t e =
+            hipMemcpyAsync(out, in, B * 4 * R * n * sizeof(T), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            set_last_error(e);
+            return ADRT_HIP_ERR_DEVICE;
+        }
         return launch_status();
     }
     // the last S stages run fused (S = 1: no tail kernel); before them,
