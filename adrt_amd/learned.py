@@ -958,15 +958,15 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5, c
         x0, t = run(lambda: solver.initial_guess(b, train=False))
         times["net"].append(t)
     xs = plain[0]
-    # FP64 judge (untimed): plain CG in float64 on the first `judge` systems
-    # to a tight tolerance; the learned float32 solutions' relative distance
-    # to it
+    # FP64 reference (untimed): plain CG in float64 on the first `nref`
+    # systems to a tight tolerance; the learned float32 solutions' relative
+    # distance to it
     from . import cg as _cg
 
-    judge = min(64, count)
-    b64 = b[:judge].double()
+    nref = min(64, count)
+    b64 = b[:nref].double()
     x64, _, _ = _cg.cg_solve(_cg.ADRTNormalOperator(solver.n), b64, rtol=1e-10, maxiter=2000)
-    d = learned[0][:judge].double() - x64
+    d = learned[0][:nref].double() - x64
     fp64_err = float((bdot(d, d) / bdot(x64, x64)).sqrt().max())
 
     def med(k):
@@ -988,7 +988,7 @@ def evaluate(solver, count=1000, seed=2000, rtol=1e-4, maxiter=500, repeats=5, c
                                                     3),
         "speedup_preconditioner_only_vs_plain": round(med("plain") / med("prec"), 3),
         "fp64_error": fp64_err,
-        "fp64_error_basis": f"max over the first {judge} images of ||x - x64|| / ||x64||, x64 "
+        "fp64_error_basis": f"max over the first {nref} images of ||x - x64|| / ||x64||, x64 "
                             "= float64 plain CG to relative residual 1e-10",
         "learned_vs_plain_solution_rel_diff": float(
             (bdot(learned[0] - plain[0], learned[0] - plain[0]) / bdot(plain[0], plain[0]))

@@ -22,7 +22,7 @@ max |result - FP64| / max |FP64|):
   * the uneven-size batch (1000 float32 images, sides 32..1024 with
     p ~ 1/n, forward + bdrt, autotuned, one hipGraph replay);
   * least squares through autograd (32 x 512^2 float32, 200 CGLS steps),
-    with the float64 run of the same iteration as its FP64 judge;
+    with the float64 run of the same iteration as its FP64 reference;
 and (key "cg_time_to_1e-4") the learned-vs-plain CG time to relative
 residual 1e-4 on 1000 held-out 256^2 systems, 5 interleaved rounds, min /
 median / max per method.  The iadrt's fp64_error in float32 is large by the
