@@ -161,7 +161,7 @@ template <typename T>
 static hipError_t run_bdrt_group(const T *y, T *Z, T *out, int n, int L, int kA, int first,
                                  int count, int rows, hipStream_t s) {
     hipError_t e;
-    const int zrows = bdrt_z_rows(n, rows);
+    const int zrows = bdrt_z_rows(n, rows, L - kA);
     switch (kA - 3) {
         case 0: e = launch_bdrt_a<T, 0>(y, Z, n, first, count, zrows, s); break;
         case 1: e = launch_bdrt_a<T, 1>(y, Z, n, first, count, zrows, s); break;

@@ -818,12 +818,16 @@ struct Pass2 {
 constexpr int kZL = 32;
 __host__ __device__ __forceinline__ int z_pitch(int n) { return kZL + (2 * n - 1 + 8 + 31) / 32 * 32; }
 
-// Z rows that pass B reads when it stores output rows [0, rows): each of its
-// ceil(rows / kOut) row chunks reads kLanes * kW rows from the chunk start,
-// and no row past C + 4 (the last margin row pass A writes).  With rows =
-// C this is every row pass A writes.
-__host__ __device__ __forceinline__ int bdrt_z_rows(int n, int rows) {
-    const int reach = ((rows + kOut - 1) / kOut - 1) * kOut + kLanes * kW;
+// Z rows that the output rows [0, rows) of a kB-stage pass B are made of.
+// Output row d adds Z rows d .. d + 2^kB - 1 only: the level-3 strip reaches
+// 7 rows ahead (4 + 2 + 1), level 4 reads level-3 offsets up to
+// (2^(kB-3) - 1) * 7 ahead and its own strip reaches 2^(kB-3) - 1 more.
+// Rows from rows + 2^kB - 1 on, and past C + 4 (the last margin row pass A
+// writes), are not needed: a kept chunk's 256-row window may still load
+// them, but they reach only its rows from `rows` on, which are not stored.
+// With rows = C this is every row pass A writes.
+__host__ __device__ __forceinline__ int bdrt_z_rows(int n, int rows, int kB) {
+    const int reach = rows + (1 << kB) - 1;
     const int full = 2 * n - 1 + 5;
     return reach < full ? reach : full;
 }

@@ -155,7 +155,7 @@ __device__ __forceinline__ void run_fused(int pass, const T *in, T *out, int n, 
         case kPass2: fused::run_item_at<fused::Pass2<T, K>, T>(item, lds, in, out, n, k, 0); break;
         case kPassA:
             fused::run_item_at<fused::BdrtA<T, K>, T>(item, lds, in, out, n, 0,
-                                                      fused::bdrt_z_rows(n, 2 * n - 1));
+                                                      2 * n - 1 + 5);  // every Z row
             break;
         default: fused::run_item_at<fused::BdrtB<T, K>, T>(item, lds, in, out, n, k, 0, 2 * n - 1); break;
     }

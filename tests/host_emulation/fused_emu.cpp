@@ -152,10 +152,20 @@ static int emulate_bdrt_group(const T *y, T *Z, T *out, int n, int first, int co
     using PB = BdrtB<T, K4>;
     std::vector<int> seen;
     const int rows = g_rows > 0 ? g_rows : 2 * n - 1;
+    const int zrows = bdrt_z_rows(n, rows, PB::kB);
     const int itemsA = PA::num_items(n, count);
-    emulate_grid<PA, T>(itemsA, grid_for_slots(itemsA, g_slots), seen, y, Z, n, first,
-                        bdrt_z_rows(n, rows));
+    emulate_grid<PA, T>(itemsA, grid_for_slots(itemsA, g_slots), seen, y, Z, n, first, zrows);
     if (!once_each(seen)) return 5;
+    // every Z row from zrows on back to NaN, written or not: the kept output
+    // rows must not depend on any of them
+    const int PZ = z_pitch(n), nP = n >> PA::kA, nS = 1 << PA::kA;
+    for (int img = 0; img < count; ++img)
+        for (int q = 0; q < 4; ++q)
+            for (int p = 0; p < nP; ++p)
+                for (int sec = 0; sec < nS; ++sec) {
+                    T *row = Z + ((size_t(img * 4 + q) * nP + p) * nS + sec) * PZ + kZL;
+                    for (int r = zrows; r < PZ - kZL; ++r) row[r] = T(NAN);
+                }
     const int itemsB = PB::num_items(PA::kA, count, rows);
     emulate_grid<PB, T>(itemsB, grid_for_slots(itemsB, g_slots), seen, (const T *)Z, out, n, PA::kA,
                         first, rows);

@@ -71,9 +71,11 @@ def test_fused_bdrt_emulation(emulator, tmp_path, dt, n):
 def test_fused_bdrt_rows_emulation(emulator, tmp_path, dt, n, rows, variant):
     """Back-projection of output rows [0, rows) only (adrt_hip_bdrt_rows):
     pass B stores just those row chunks and pass A skips the items whose Z
-    rows they never read.  Z starts as NaN and the data is finite, so a kept
-    row that read a Z row no pass-A item wrote comes out NaN; kept rows must
-    equal the oracle's, and the rows past the last kept chunk stay unwritten."""
+    rows the kept rows do not depend on (from rows + 2^kB - 1 on).  Z starts
+    as NaN, and after pass A every Z row from that bound on is set back to
+    NaN; the data is finite, so a kept row that depends on such a row comes
+    out NaN (one row less in the bound fails here).  Kept rows must equal
+    the oracle's, and the rows from `rows` on stay unwritten."""
     rng = np.random.default_rng(7 * n + rows)
     y = (rng.standard_normal((2, 4, 2 * n - 1, n)) * 10).astype(dt)
     got = run_emu(emulator, y, tmp_path, op="bdrt", variant=variant, rows=rows)
