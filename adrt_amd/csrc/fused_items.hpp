@@ -957,29 +957,16 @@ struct BdrtA {
         }
     }
 
-    // Section s of the item goes to Z rows from c0 - p*s on, so with s <
-    // 2^kA every row it stores lies at or past c0 - p*(2^kA - 1).  When that
-    // is at or past the rows pass B reads, nothing it stores is read.
+    // Items that pass B does not need:
+    //  - section s of the item goes to Z rows from c0 - p*s on, so with s <
+    //    2^kA every row it stores lies at or past c0 - p*(2^kA - 1); when
+    //    that is at or past the rows pass B reads, nothing it stores is read;
+    //  - an item whose input rows all lie past the last row (c0 > C) sees
+    //    only +0 padding: all it would store is +0 (and -0 at row C) into Z
+    //    rows r > C - p*s, which pass B makes itself (BdrtB::load0).  About
+    //    a third of the items are such (they exist only for the skew).
     __host__ __device__ __forceinline__ static bool skip(const Item &it) {
-        return it.c0 - it.p * ((1 << kA) - 1) >= it.zrows;
-    }
-
-    // An item whose input rows all lie past the last row (c0 > C) sees only
-    // +0 padding, so every strip value is +0 and all it does is store +0
-    // (and -0 at row C) into its Z windows.  About a third of the items are
-    // such (they exist to cover the skewed section stores); they make the
-    // same stores directly, without the loads, strips and barriers.
-    __host__ __device__ __forceinline__ static bool padding_only(const Item &it) {
-        return it.c0 > it.C;
-    }
-    __host__ __device__ __forceinline__ static void store_padding(const Item &it, int wave,
-                                                                  int lane) {
-        T z[R1][kW];
-#pragma unroll
-        for (int a = 0; a < R1; ++a)
-#pragma unroll
-            for (int i = 0; i < kW; ++i) z[a][i] = T(0);
-        store1(it, wave, lane, z, nullptr);
+        return it.c0 > it.C || it.c0 - it.p * ((1 << kA) - 1) >= it.zrows;
     }
 
     // load0 from the staged block (same values, then the same padding).
@@ -1084,7 +1071,7 @@ struct BdrtB {
     struct Item {
         const T *Z;  // row 0 of (group 0, section s); group p at + p * pstride
         T *out;      // output plane (C, n)
-        int n, C, d0, abase;
+        int n, C, d0, abase, s;
         int rows;    // output rows stored: [0, rows), rows <= C
         size_t pstride;
     };
@@ -1116,22 +1103,40 @@ struct BdrtB {
         const int b = first_image + (block >> 2);
         it.d0 = chunk * kOut;
         it.pstride = (size_t)nS * PZ;
+        it.s = s;
         it.Z = Z_base + ((size_t)(b - first_image) * 4 + q) * (size_t)n * PZ + (size_t)s * PZ + kZL;
         it.out = out_base + ((size_t)b * 4 + q) * it.C * (size_t)n;
         it.abase = s << kB;
         return it;
     }
 
-    // Level-3 strip g': Z rows d0 + 4*lane + i of groups 8g' + v.  A window
-    // starting past C + 1 is read from C + 1 (all +0.0, as every row past C).
+    // Level-3 strip g': Z rows d0 + 4*lane + i of groups p = 8g' + j.  Z row
+    // r of (p, s) adds input rows from r + p*s on, so past lim = C - p*s it
+    // holds +0 from padding only (row C: the -0 margin), and pass A does not
+    // write the items that hold nothing else (BdrtA::skip).  Those rows are
+    // made here: a window wholly past lim is loaded from lim's window (a row
+    // that is written) and replaced, one that crosses lim is patched.  A
+    // window starting past C + 1 starts at C + 1.
     __host__ __device__ __forceinline__ static void load0(const Item &it, int g, int lane,
                                                           T (&v)[R0][kW]) {
         int c = it.d0 + kW * lane;
         c = c > it.C + 1 ? it.C + 1 : c;
 #pragma unroll
-        for (int j = 0; j < R0; ++j)
-            load_vec4(it.Z + (size_t)(8 * g + j) * it.pstride + unsigned(c), v[j][0], v[j][1], v[j][2],
-                      v[j][3]);
+        for (int j = 0; j < R0; ++j) {
+            const int lim = it.C - (8 * g + j) * it.s;  // > 0: p*s < C
+            const int cl = c > lim ? (lim > 3 ? lim - 3 : 0) : c;
+            load_vec4(it.Z + (size_t)(8 * g + j) * it.pstride + unsigned(cl), v[j][0], v[j][1],
+                      v[j][2], v[j][3]);
+        }
+#pragma unroll
+        for (int j = 0; j < R0; ++j) {
+            const int lim = it.C - (8 * g + j) * it.s;
+#pragma unroll
+            for (int i = 0; i < kW; ++i) {
+                const int r = c + i;
+                if (r > lim) v[j][i] = r == it.C ? neg_zero<T>() : T(0);
+            }
+        }
     }
 
     // Swizzled tile position of (row d, column a), as in the forward's tile.
@@ -1217,10 +1222,6 @@ __device__ __forceinline__ void run_item_at(int idx, T *lds, A... args) {
     const int wave = wave_uniform(threadIdx.x / kLanes);
     const int lane = threadIdx.x % kLanes;
     if constexpr (P::kStaged) {
-        if (P::padding_only(it)) {  // work-group uniform
-            P::store_padding(it, wave, lane);
-            return;
-        }
         // cooperative staged load; the staging area is the level-1 area, so
         // every strip reads its block before any strip stores
         P::stage(it, lds, threadIdx.x);

@@ -44,11 +44,6 @@ static void emulate_grid(int items, unsigned grid, std::vector<int> &seen, Args.
             for (auto &x : lds) x = T(NAN);
             T *l = P::lds_elems > 0 ? lds.data() : nullptr;
             if constexpr (P::kStaged) {
-                if (P::padding_only(it)) {  // the kernel's padding-only items
-                    for (int task = 0; task < P::tasks1; ++task)
-                        for (int lane = 0; lane < kLanes; ++lane) P::store_padding(it, task, lane);
-                    continue;
-                }
                 // staged load by the whole work-group, then every strip reads
                 // its block before any strip stores (they share the LDS area)
                 for (int tid = 0; tid < kThreads; ++tid) P::stage(it, l, tid);
